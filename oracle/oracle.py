@@ -1,0 +1,255 @@
+"""CPU oracle for the TDVMC hot path -- TEST INFRASTRUCTURE ONLY.
+
+ctypes wrapper over ``liboracle.so`` (the scalar C restatement in
+``tdvmc_oracle.c``) and over ``_ref/libref.so`` (the reference's own
+``BosonsBulk`` plug-in, compiled from the reference checkout by
+``oracle/Makefile``).  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+CPU-baseline leg of ``bench.py`` import this module; the product library never
+does.
+
+Parity status of each function is recorded in ``tdvmc_oracle.c``'s header.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import itertools
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+_dp = C.POINTER(C.c_double)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+class _CSystem(C.Structure):
+    _fields_ = [
+        ("N", C.c_int), ("D", C.c_int), ("L", C.c_double), ("P_s", C.c_int), ("rc", C.c_double),
+        ("K", C.c_int), ("kvec", _dp), ("one_body", C.c_int), ("hbar2_2m", C.c_double),
+        ("pot_type", C.c_int), ("n_pot", C.c_int), ("pot", C.c_double * 5),
+        ("pot_cutoff", C.c_double), ("trap_omega", C.c_double),
+    ]
+
+
+POT_NONE, POT_SQUARE, POT_GAUSS, POT_LJ = 0, 1, 2, 3
+
+
+@dataclass
+class System:
+    """Shape and physics of one system (mirrors the library's system record)."""
+    N: int
+    D: int
+    L: float            # periodic box (<= 0: open space)
+    P_s: int
+    rc: float = 0.0     # spline cutoff; default L/2
+    kvec: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    one_body: int = 0
+    hbar2_2m: float = 1.0
+    pot_type: int = POT_NONE
+    pot: tuple = ()
+    pot_cutoff: float = 0.0   # default: rc
+    trap_omega: float = 0.0
+
+    def __post_init__(self):
+        if self.rc <= 0:
+            self.rc = self.L / 2.0
+        if self.pot_cutoff <= 0:
+            self.pot_cutoff = self.rc
+        self.kvec = np.ascontiguousarray(np.asarray(self.kvec, dtype=np.float64).reshape(-1, self.D))
+
+    @property
+    def K(self):
+        return self.kvec.shape[0]
+
+    @property
+    def P(self):
+        return (1 if self.one_body else 0) + self.P_s + self.K
+
+    def c(self):
+        s = _CSystem()
+        s.N, s.D, s.L, s.P_s, s.rc = self.N, self.D, self.L, self.P_s, self.rc
+        s.K = self.K
+        s.kvec = _ptr(self.kvec) if self.K else None
+        s.one_body, s.hbar2_2m = self.one_body, self.hbar2_2m
+        s.pot_type, s.n_pot = self.pot_type, len(self.pot)
+        for i, v in enumerate(self.pot):
+            s.pot[i] = v
+        s.pot_cutoff, s.trap_omega = self.pot_cutoff, self.trap_omega
+        s._keep = self.kvec
+        return s
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        L.tdo_nic_coord.restype = C.c_double
+        L.tdo_nic_coord.argtypes = [C.c_double, C.c_double]
+        L.tdo_displacement.restype = C.c_double
+        L.tdo_displacement.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp]
+        L.tdo_bspline.restype = C.c_int
+        L.tdo_bspline.argtypes = [C.c_double, C.c_double, C.c_int, _dp, _dp, _dp]
+        L.tdo_pair_potential.restype = C.c_double
+        L.tdo_pair_potential.argtypes = [C.POINTER(_CSystem), C.c_double, C.c_double]
+        L.tdo_eval_batch.restype = C.c_int
+        L.tdo_eval_batch.argtypes = [C.POINTER(_CSystem), C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double,
+                                     _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.tdo_metropolis.restype = C.c_int64
+        L.tdo_metropolis.argtypes = [C.POINTER(_CSystem), _dp, _dp, C.c_double, C.c_uint64, C.c_int64,
+                                     C.c_int64, _dp]
+        L.tdo_moments.restype = None
+        L.tdo_moments.argtypes = [C.c_int64, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.tdo_cholesky_ref.restype = C.c_int
+        L.tdo_cholesky_ref.argtypes = [C.c_int, _dp]
+        L.tdo_chol_solve.restype = None
+        L.tdo_chol_solve.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.tdo_solve_param_dot.restype = C.c_int
+        L.tdo_solve_param_dot.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp,
+                                          _dp, _dp, _dp, _dp]
+        _LIB = L
+    return _LIB
+
+
+def ref_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libref.so"))
+
+
+def ref():
+    global _REF
+    if _REF is None:
+        path = os.path.join(HERE, "_ref", "libref.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/_ref/libref.so is not built (needs the reference checkout)")
+        L = C.CDLL(path)
+        L.ref_displacement_nic.restype = C.c_double
+        L.ref_displacement_nic.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp]
+        L.ref_bosonsbulk_eval.restype = C.c_int
+        L.ref_bosonsbulk_eval.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _dp, C.c_int, C.c_double,
+                                          C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double,
+                                          _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.ref_bosonsbulk_move.restype = C.c_double
+        L.ref_bosonsbulk_move.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _dp, C.c_int, _dp, _dp,
+                                          C.c_int, _dp, _dp]
+        L.ref_bosonsbulk_sample.restype = C.c_int64
+        L.ref_bosonsbulk_sample.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _dp, C.c_int, _dp, _dp,
+                                            C.c_double, C.c_uint64, C.c_int64, C.c_int64, _dp]
+        _REF = L
+    return _REF
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+# ---------------------------------------------------------------- geometry
+
+def nic_coord(x, L):
+    return lib().tdo_nic_coord(float(x), float(L))
+
+
+def displacement(ri, rj, L):
+    ri, rj = _f64(ri), _f64(rj)
+    out = np.zeros_like(ri)
+    d = lib().tdo_displacement(_ptr(ri), _ptr(rj), ri.size, float(L), _ptr(out))
+    return d, out
+
+
+def bspline(r, h, P_s):
+    b, db, d2b = np.zeros(4), np.zeros(4), np.zeros(4)
+    j = lib().tdo_bspline(float(r), float(h), int(P_s), _ptr(b), _ptr(db), _ptr(d2b))
+    return j, b, db, d2b
+
+
+def pair_potential(sysd: System, r, time=0.0):
+    cs = sysd.c()
+    return lib().tdo_pair_potential(C.byref(cs), float(r), float(time))
+
+
+# ---------------------------------------------------------------- evaluation
+
+def eval_configs(sysd: System, R, uR, uI=None, phiR=0.0, time=0.0):
+    """ln psi, O, E_L, gradient, Laplacian and the 9 'other' values per walker."""
+    R = _f64(R).reshape(-1, sysd.N, sysd.D)
+    W = R.shape[0]
+    P = sysd.P
+    uR = _f64(uR)
+    uI = np.zeros(P) if uI is None else _f64(uI)
+    assert uR.size == P and uI.size == P
+    out = dict(expo=np.zeros(W), O=np.zeros((W, P)), ER=np.zeros(W), EI=np.zeros(W),
+               grad=np.zeros((W, sysd.N, sysd.D, 2)), lap=np.zeros((W, sysd.N, 2)), other=np.zeros((W, 9)))
+    cs = sysd.c()
+    rc = lib().tdo_eval_batch(C.byref(cs), W, _ptr(R), _ptr(uR), _ptr(uI), float(phiR), float(time),
+                              _ptr(out["expo"]), _ptr(out["O"]), _ptr(out["ER"]), _ptr(out["EI"]),
+                              _ptr(out["grad"]), _ptr(out["lap"]), _ptr(out["other"]))
+    if rc:
+        raise MemoryError("oracle evaluation failed")
+    return out
+
+
+def metropolis(sysd: System, R, uR, step, seed, n_samples, n_therm):
+    R = _f64(R).copy()
+    uR = _f64(uR)
+    rec = np.zeros((n_samples, sysd.N, sysd.D))
+    cs = sysd.c()
+    acc = lib().tdo_metropolis(C.byref(cs), _ptr(R), _ptr(uR), float(step), int(seed), int(n_samples),
+                               int(n_therm), _ptr(rec))
+    return R, rec, acc
+
+
+# ---------------------------------------------------------------- TDVP
+
+def moments(O, ER, EI):
+    O = _f64(O)
+    W, P = O.shape
+    mO, mE = np.zeros(P), np.zeros(2)
+    S, FR, FI = np.zeros((P, P)), np.zeros(P), np.zeros(P)
+    lib().tdo_moments(W, P, _ptr(O), _ptr(_f64(ER)), _ptr(_f64(EI)), _ptr(mO), _ptr(mE), _ptr(S),
+                      _ptr(FR), _ptr(FI))
+    return dict(mO=mO, mE=mE, S=S, FR=FR, FI=FI)
+
+
+def cholesky_ref(A):
+    A = _f64(A).copy()
+    bad = lib().tdo_cholesky_ref(A.shape[0], _ptr(A))
+    return np.tril(A), bad
+
+
+def solve_param_dot(m, imag_time=1, precondition=0, eps=1e-3):
+    P = m["mO"].size
+    S = _f64(m["S"]).copy()
+    xR, xI = np.zeros(P), np.zeros(P)
+    pR, pI = C.c_double(0), C.c_double(0)
+    bad = lib().tdo_solve_param_dot(P, int(imag_time), int(precondition), float(eps), _ptr(S), _ptr(_f64(m["mO"])),
+                                    _ptr(_f64(m["mE"])), _ptr(_f64(m["FR"])), _ptr(_f64(m["FI"])),
+                                    _ptr(xR), _ptr(xI), C.byref(pR), C.byref(pI))
+    return dict(xR=xR, xI=xI, phiDotR=pR.value, phiDotI=pI.value, bad=bad)
+
+
+# ---------------------------------------------------------------- helpers
+
+def kvectors(D, K, L):
+    """K shortest non-zero reciprocal-lattice vectors 2 pi n / L, ordered by
+    (|n|^2, lexicographic n) -- both k and -k are kept."""
+    if K == 0:
+        return np.zeros((0, D))
+    m = 1
+    while True:
+        vs = [v for v in itertools.product(range(-m, m + 1), repeat=D) if any(v)]
+        vs.sort(key=lambda v: (sum(x * x for x in v), v))
+        if len(vs) >= K and sum(x * x for x in vs[K - 1]) < (m + 1) ** 2:
+            break
+        m += 1
+    n = np.array(vs[:K], dtype=np.float64)
+    return n * (2.0 * np.pi / L)

@@ -1,0 +1,261 @@
+// ref_harness.cpp -- C entry points over the reference's own BosonsBulk
+// physics plug-in, compiled from the sources where they lie in the reference
+// checkout (see oracle/Makefile).  TEST INFRASTRUCTURE ONLY: used in this
+// container to write golden vectors (tests/golden/make_golden.py), to pin the
+// C restatement in tdvmc_oracle.c, and as the CPU baseline of bench.py.
+//
+// The reference defines its shape globals in src/TDVMC.cpp:84-89 and binds
+// the *_DIM function pointers in Init (src/TDVMC.cpp:567-613); TDVMC.cpp needs
+// MPI, so this harness defines the same globals and binds the pointers itself.
+// BosonsBulk::InitSystem reads kVectors<D>D.json / kNorm<D>D.csv for its S(k)
+// observable only (src/PhysicalSystems/BosonsBulk.cpp:124-137); the harness
+// writes a small file set of that format into a scratch directory so the
+// plug-in can be constructed without the reference's data directory.
+
+#include "PhysicalSystems/BosonsBulk.h"
+#include "CSDataBulkSplines.h"
+#include "Utils.h"
+
+#include <cstdint>
+#include <cstdio>
+#include <fstream>
+#include <random>
+#include <string>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <vector>
+
+using namespace std;
+
+string OUT_DIR;
+int N;
+int DIM;
+int N_PARAM;
+int IMAGINARY_TIME;
+double LBOX;
+double LBOX_R;
+double LBOX_2;
+
+namespace {
+
+void bind_dim_functions(int dim)
+{
+    switch (dim) {
+    case 1:
+        VectorDotProduct_DIM = VectorDotProduct_1D;
+        VectorNorm2_DIM = VectorNorm2_1D;
+        VectorNorm_DIM = VectorNorm_1D;
+        VectorDisplacement_DIM = VectorDisplacement_1D;
+        GetVectorNIC_DIM = GetVectorNIC_1D;
+        VectorDiffNIC_DIM = VectorDiffNIC_1D;
+        VectorDisplacementNIC_DIM = VectorDisplacementNIC_1D;
+        AssignVector_DIM = AssignVector_1D;
+        break;
+    case 2:
+        VectorDotProduct_DIM = VectorDotProduct_2D;
+        VectorNorm2_DIM = VectorNorm2_2D;
+        VectorNorm_DIM = VectorNorm_2D;
+        VectorDisplacement_DIM = VectorDisplacement_2D;
+        GetVectorNIC_DIM = GetVectorNIC_2D;
+        VectorDiffNIC_DIM = VectorDiffNIC_2D;
+        VectorDisplacementNIC_DIM = VectorDisplacementNIC_2D;
+        AssignVector_DIM = AssignVector_2D;
+        break;
+    default:
+        VectorDotProduct_DIM = VectorDotProduct_3D;
+        VectorNorm2_DIM = VectorNorm2_3D;
+        VectorNorm_DIM = VectorNorm_3D;
+        VectorDisplacement_DIM = VectorDisplacement_3D;
+        GetVectorNIC_DIM = GetVectorNIC_3D;
+        VectorDiffNIC_DIM = VectorDiffNIC_3D;
+        VectorDisplacementNIC_DIM = VectorDisplacementNIC_3D;
+        AssignVector_DIM = AssignVector_3D;
+        break;
+    }
+}
+
+// 50 shells of one vector each: n = (s, 0, ...), |n| = s.
+string write_kvector_dir(int dim)
+{
+    char tmpl[] = "/tmp/tdvmc_refk_XXXXXX";
+    char* d = mkdtemp(tmpl);
+    if (!d) return "";
+    string dir = string(d) + "/";
+    ofstream kv(dir + "kVectors" + to_string(dim) + "D.json");
+    kv << "{\"data\": [";
+    for (int s = 1; s <= 50; s++) {
+        kv << (s > 1 ? "," : "") << "[[" << s;
+        for (int a = 1; a < dim; a++) kv << ",0";
+        kv << "]]";
+    }
+    kv << "]}";
+    kv.close();
+    ofstream kn(dir + "kNorm" + to_string(dim) + "D.csv");
+    for (int s = 1; s <= 50; s++) kn << s << "\n";
+    kn.close();
+    return dir;
+}
+
+struct Setup {
+    PhysicalSystems::BosonsBulk* sys = nullptr;
+    string dir;
+};
+
+Setup make_system(int n, int dim, double L, int nparam, const double* sysparams, int nsys, double time)
+{
+    N = n;
+    DIM = dim;
+    N_PARAM = nparam;
+    LBOX = L;
+    LBOX_R = 1.0 / L;
+    LBOX_2 = L / 2.0;
+    IMAGINARY_TIME = 1;
+    bind_dim_functions(dim);
+    Setup s;
+    s.dir = write_kvector_dir(dim);
+    vector<double> p(sysparams, sysparams + nsys);
+    s.sys = new PhysicalSystems::BosonsBulk(p, s.dir);
+    s.sys->SetPairDistributionBinCount(50);
+    s.sys->InitSystem();
+    s.sys->SetTime(time);
+    return s;
+}
+
+// The plug-in is never deleted: its base destructor (IPhysicalSystem.h:71-74)
+// deletes observables that BosonsBulk registers as member addresses
+// (BosonsBulk.cpp:154-155), and the reference itself never destroys it.
+void free_system(Setup& s)
+{
+    s.sys = nullptr;
+    if (!s.dir.empty()) {
+        unlink((s.dir + "kVectors" + to_string(DIM) + "D.json").c_str());
+        unlink((s.dir + "kNorm" + to_string(DIM) + "D.csv").c_str());
+        rmdir(s.dir.c_str());
+    }
+}
+
+vector<vector<double>> to_R(const double* R)
+{
+    vector<vector<double>> out(N, vector<double>(DIM));
+    for (int i = 0; i < N; i++)
+        for (int a = 0; a < DIM; a++) out[i][a] = R[i * DIM + a];
+    return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Reference NIC displacement (src/Utils.cpp:376-382) with global box L.
+double ref_displacement_nic(const double* ri, const double* rj, int dim, double L, double* out)
+{
+    LBOX = L;
+    LBOX_R = 1.0 / L;
+    LBOX_2 = L / 2.0;
+    vector<double> a(ri, ri + dim), b(rj, rj + dim), r(dim);
+    double d = VectorDisplacementNIC(a, b, r);
+    for (int i = 0; i < dim; i++) out[i] = r[i];
+    return d;
+}
+
+// Evaluate W fed configurations with BosonsBulk: full recompute of the basis
+// sums (CalculateWavefunction) followed by CalculateExpectationValues.
+// Outputs per walker: exponent, O[P], E_R, E_I, other[9],
+// sD[S][N][D] and sD2[S][N] (S = P + 2 splines; may be NULL).
+int ref_bosonsbulk_eval(int n, int dim, double L, int nparam, const double* sysparams, int nsys,
+                        double time, int64_t W, const double* R, const double* uR, const double* uI,
+                        double phiR, double phiI, double* expo, double* O, double* ER, double* EI,
+                        double* other, double* sD, double* sD2)
+{
+    Setup s = make_system(n, dim, L, nparam, sysparams, nsys, time);
+    vector<double> vR(uR, uR + nparam), vI(uI, uI + nparam);
+    const int S = nparam + 2;
+    for (int64_t w = 0; w < W; w++) {
+        auto Rw = to_R(R + w * n * dim);
+        s.sys->CalculateWavefunction(Rw, vR, vI, phiR, phiI);
+        s.sys->CalculateExpectationValues(Rw, vR, vI, phiR, phiI);
+        expo[w] = s.sys->GetExponent();
+        auto lo = s.sys->GetLocalOperators();
+        for (int k = 0; k < nparam; k++) O[w * nparam + k] = lo[k];
+        ER[w] = s.sys->GetLocalEnergyR();
+        EI[w] = s.sys->GetLocalEnergyI();
+        auto ov = s.sys->GetOtherExpectationValues();
+        for (int k = 0; k < 9; k++) other[w * 9 + k] = ov[k];
+        if (sD || sD2) {
+            CSDataBulkSplines cs;
+            s.sys->FillCorrelatedSamplingData(&cs);
+            for (int sp = 0; sp < S; sp++)
+                for (int i = 0; i < n; i++) {
+                    if (sD)
+                        for (int a = 0; a < dim; a++)
+                            sD[((w * S + sp) * n + i) * dim + a] = cs.splineSumsD[sp][i][a];
+                    if (sD2) sD2[(w * S + sp) * n + i] = cs.splineSumsD2[sp][i];
+                }
+        }
+    }
+    free_system(s);
+    return 0;
+}
+
+// Incremental move of one particle (CalculateWFQuotient, BosonsBulk.cpp:553-657):
+// R is the configuration before the move; particle p goes to newpos.
+// Returns exponentNew; *quotient receives exp(2 (exponentNew - exponent)).
+double ref_bosonsbulk_move(int n, int dim, double L, int nparam, const double* sysparams, int nsys,
+                           const double* R, const double* uR, int p, const double* newpos, double* quotient)
+{
+    Setup s = make_system(n, dim, L, nparam, sysparams, nsys, 0.0);
+    vector<double> vR(uR, uR + nparam), vI(nparam, 0.0);
+    auto Rw = to_R(R);
+    s.sys->CalculateWavefunction(Rw, vR, vI, 0.0, 0.0);
+    vector<double> old = Rw[p];
+    for (int a = 0; a < dim; a++) Rw[p][a] = newpos[a];
+    *quotient = s.sys->CalculateWFQuotient(Rw, vR, vI, 0.0, 0.0, p, old);
+    double e = s.sys->GetExponentNew();
+    free_system(s);
+    return e;
+}
+
+// CPU baseline: one walker doing n_samples x (n_therm single-particle
+// Metropolis moves, then one measurement) through the reference plug-in's own
+// incremental ratio and measurement; the move loop mirrors DoMetropolisStep
+// (src/TDVMC.cpp:858-916) with mt19937_64 and std::normal_distribution.
+// Returns the accepted-move count; *mean_E receives the sample mean of E_R.
+int64_t ref_bosonsbulk_sample(int n, int dim, double L, int nparam, const double* sysparams, int nsys,
+                              double* R, const double* uR, double step, uint64_t seed,
+                              int64_t n_samples, int64_t n_therm, double* mean_E)
+{
+    Setup s = make_system(n, dim, L, nparam, sysparams, nsys, 0.0);
+    vector<double> vR(uR, uR + nparam), vI(nparam, 0.0);
+    auto Rw = to_R(R);
+    mt19937_64 gen(seed);
+    uniform_real_distribution<double> uni(0.0, 1.0);
+    normal_distribution<double> nrm(0.0, 1.0);
+    uniform_int_distribution<int> pick(0, n - 1);
+    s.sys->CalculateWavefunction(Rw, vR, vI, 0.0, 0.0);
+    int64_t acc = 0;
+    double esum = 0.0;
+    for (int64_t smp = 0; smp < n_samples; smp++) {
+        for (int64_t t = 0; t < n_therm; t++) {
+            int p = pick(gen);
+            vector<double> old(Rw[p]);
+            for (int a = 0; a < dim; a++) Rw[p][a] += nrm(gen) * step;
+            double q = s.sys->CalculateWFQuotient(Rw, vR, vI, 0.0, 0.0, p, old);
+            bool ok = std::isfinite(q) && std::isfinite(s.sys->GetExponentNew());
+            if (!ok || q < uni(gen)) {
+                Rw[p] = old;
+            } else {
+                s.sys->AcceptMove();
+                acc++;
+            }
+        }
+        s.sys->CalculateExpectationValues(Rw, vR, vI, 0.0, 0.0);
+        esum += s.sys->GetLocalEnergyR();
+    }
+    for (int i = 0; i < n; i++)
+        for (int a = 0; a < dim; a++) R[i * dim + a] = Rw[i][a];
+    *mean_E = n_samples > 0 ? esum / (double)n_samples : 0.0;
+    free_system(s);
+    return acc;
+}
+
+}  // extern "C"

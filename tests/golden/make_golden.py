@@ -1,0 +1,178 @@
+"""Writes the golden vectors under tests/golden/ from the reference's own code.
+
+Run in a container that holds the reference checkout, after
+``make -C oracle ref`` has compiled oracle/_ref/libref.so from it:
+
+    python tests/golden/make_golden.py
+
+Outputs (inputs and expected outputs only; no reference source):
+  nic_cases.json      -- the 48 known answers of src/test/Tests.h:76-156
+                         (inputs and expected results transcribed as data),
+                         plus 200 random displacements evaluated by the
+                         reference's VectorDisplacementNIC
+  eval_<name>.npz     -- fed configurations R, parameters, and the reference
+                         BosonsBulk outputs: exponent, O, E_R, E_I, other[9],
+                         sD[S][N][D], sD2[S][N]
+  move_cases.npz      -- single-particle moves through CalculateWFQuotient
+  config_*.json       -- reference config files (data) for the schema loader
+"""
+import ctypes as C
+import json
+import os
+import shutil
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from oracle import oracle as o  # noqa: E402
+
+REF = "/root/reference"
+
+
+def f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# src/test/Tests.h:76-156 -- (L, D, ri, rj, expected |d|, expected d)
+NIC_CASES = [
+    (4, 3, (0, 0, 0), (0, 0, 0), 0, (0, 0, 0)),
+    (4, 3, (0, 0, 0), (1, 0, 0), 1, (-1, 0, 0)),
+    (4, 3, (1, 0, 0), (0, 0, 0), 1, (1, 0, 0)),
+    (4, 3, (1, 0, 0), (6, 0, 0), 1, (-1, 0, 0)),
+    (4, 3, (1, 0, 0), (3, 0, 0), 2, (2, 0, 0)),
+    (4, 3, (1, 0, 0), (-1, 0, 0), 2, (-2, 0, 0)),
+    (4, 3, (0, 0, 0), (10, 10, 10), 12 ** 0.5, (2, 2, 2)),
+    (4, 3, (4, 4, 4), (10, 10, 10), 12 ** 0.5, (2, 2, 2)),
+    (4, 3, (4, 4, 4), (10.1, 0, 0), 1.9, (1.9, 0, 0)),
+    (4, 3, (4, 4, 4), (9.9, 0, 0), 1.9, (-1.9, 0, 0)),
+    (4, 3, (0.1, 0, 0), (3.9, 0, 0), 0.2, (0.2, 0, 0)),
+    (4, 3, (0, 0, 0), (3.9, 3.9, 3.9), (0.03) ** 0.5, (0.1, 0.1, 0.1)),
+    (4, 3, (4.5, 2.1, -2.1), (0, 0, 0), (0.25 + 1.9 * 1.9 * 2) ** 0.5, (0.5, -1.9, 1.9)),
+    (4, 3, (0.5, 1.5, 2.5), (0, 0, 0), (0.25 + 2.25 * 2) ** 0.5, (0.5, 1.5, -1.5)),
+    (4, 3, (3.5, 4.5, 5.5), (0, 0, 0), (0.25 * 2 + 2.25) ** 0.5, (-0.5, 0.5, 1.5)),
+    (5, 3, (0, 0, 0), (0, 0, 0), 0, (0, 0, 0)),
+    (5, 3, (0, 0, 0), (1, 0, 0), 1, (-1, 0, 0)),
+    (5, 3, (1, 0, 0), (0, 0, 0), 1, (1, 0, 0)),
+    (5, 3, (1, 0, 0), (6, 0, 0), 0, (0, 0, 0)),
+    (5, 3, (1, 0, 0), (3, 0, 0), 2, (-2, 0, 0)),
+    (5, 3, (1, 0, 0), (-1, 0, 0), 2, (2, 0, 0)),
+    (5, 3, (0, 0, 0), (10, 10, 10), 0, (0, 0, 0)),
+    (5, 3, (4, 4, 4), (10, 10, 10), 3 ** 0.5, (-1, -1, -1)),
+    (5, 3, (4, 4, 4), (10.1, 0, 0), (1.1 * 1.1 + 2) ** 0.5, (-1.1, -1, -1)),
+    (5, 3, (4, 4, 4), (9.9, 0, 0), (2 + 0.81) ** 0.5, (-0.9, -1, -1)),
+    (5, 3, (0.1, 0, 0), (3.9, 0, 0), 1.2, (1.2, 0, 0)),
+    (5, 3, (0, 0, 0), (4.9, 4.9, 4.9), (0.03) ** 0.5, (0.1, 0.1, 0.1)),
+    (5, 3, (1.25, 0, 0), (-1.25, 0, 0), 2.5, (-2.5, 0, 0)),
+    (5, 3, (5.9, 2.6, -2.6), (0, 0, 0), (0.81 + 2.4 * 2.4 * 2) ** 0.5, (0.9, -2.4, 2.4)),
+    (5, 3, (0, 1, 2), (0, 0, 0), 5 ** 0.5, (0, 1, 2)),
+    (5, 3, (3, 4, 5), (0, 0, 0), 5 ** 0.5, (-2, -1, 0)),
+    (4, 1, (0,), (0,), 0, (0,)),
+    (4, 1, (0,), (1,), 1, (-1,)),
+    (4, 1, (1,), (0,), 1, (1,)),
+    (4, 1, (1,), (1,), 0, (0,)),
+    (4, 1, (0,), (2,), 2, (2,)),
+    (4, 1, (0,), (3,), 1, (1,)),
+    (4, 1, (0,), (4,), 0, (0,)),
+    (4, 1, (0,), (5,), 1, (-1,)),
+    (5, 1, (0,), (0,), 0, (0,)),
+    (5, 1, (0,), (1,), 1, (-1,)),
+    (5, 1, (1,), (0,), 1, (1,)),
+    (5, 1, (1,), (1,), 0, (0,)),
+    (5, 1, (0,), (2,), 2, (-2,)),
+    (5, 1, (0,), (3,), 2, (2,)),
+    (5, 1, (0,), (4,), 1, (1,)),
+    (5, 1, (0,), (5,), 0, (0,)),
+    (5, 1, (0,), (6,), 1, (-1,)),
+]
+
+
+def nic():
+    r = o.ref()
+    cases = []
+    for (L, D, ri, rj, dd, dv) in NIC_CASES:
+        cases.append(dict(L=L, D=D, ri=list(ri), rj=list(rj), dist=dd, disp=list(dv)))
+    rng = np.random.default_rng(7)
+    rand = []
+    for _ in range(200):
+        D = int(rng.integers(1, 4))
+        L = float(rng.uniform(2, 12))
+        ri = rng.uniform(-1.5 * L, 1.5 * L, D)
+        rj = rng.uniform(-1.5 * L, 1.5 * L, D)
+        out = np.zeros(D)
+        d = r.ref_displacement_nic(f(ri), f(rj), D, L, f(out))
+        rand.append(dict(L=L, D=D, ri=ri.tolist(), rj=rj.tolist(), dist=d, disp=out.tolist()))
+    with open(os.path.join(HERE, "nic_cases.json"), "w") as fh:
+        json.dump(dict(source="src/test/Tests.h:76-156 (known answers, tolerance 1e-9)", known=cases,
+                       random=rand), fh, indent=1)
+
+
+EVAL_CASES = [
+    # name, D, N, L, P, sysparams (square well a, b [, t_q, a', b']), W, seed, time
+    ("bulk1d_n20_p16", 1, 20, 20.0, 16, [0.5, 3.0], 6, 11, 0.0),
+    ("bulk2d_n25_p12", 2, 25, 6.0, 12, [0.6, 2.0], 6, 12, 0.0),
+    ("bulk3d_n32_p10", 3, 32, 4.5, 10, [0.7, 2.0], 6, 13, 0.0),
+    ("bulk3d_quench", 3, 16, 3.5, 8, [0.7, 2.0, 0.5, 0.9, 4.0], 4, 14, 1.0),
+    ("bulk3d_n64_p32", 3, 64, 5.5971, 32, [0.9, 1.5], 4, 15, 0.0),
+]
+
+
+def eval_cases():
+    r = o.ref()
+    for (name, D, N, L, P, sp, W, seed, time) in EVAL_CASES:
+        rng = np.random.default_rng(seed)
+        # lattice-jittered configurations spread over several images
+        R = rng.uniform(-L / 2, L / 2, (W, N, D))
+        R[W // 2:] += rng.integers(-2, 3, (W - W // 2, N, D)) * L
+        uR = rng.normal(0, 0.3, P)
+        uI = rng.normal(0, 0.2, P)
+        phiR, phiI = 0.05, -0.02
+        S = P + 2
+        spa = np.array(sp, dtype=np.float64)
+        expo, O, ER, EI = np.zeros(W), np.zeros((W, P)), np.zeros(W), np.zeros(W)
+        oth, sD, sD2 = np.zeros((W, 9)), np.zeros((W, S, N, D)), np.zeros((W, S, N))
+        R = np.ascontiguousarray(R)
+        r.ref_bosonsbulk_eval(N, D, L, P, f(spa), len(sp), time, W, f(R), f(uR), f(uI), phiR, phiI,
+                              f(expo), f(O), f(ER), f(EI), f(oth), f(sD), f(sD2))
+        np.savez_compressed(os.path.join(HERE, f"eval_{name}.npz"), D=D, N=N, L=L, P=P, sysparams=spa, time=time,
+                            R=R, uR=uR, uI=uI, phiR=phiR, phiI=phiI, expo=expo, O=O, ER=ER, EI=EI, other=oth,
+                            sD=sD, sD2=sD2)
+
+
+def move_cases():
+    r = o.ref()
+    rng = np.random.default_rng(21)
+    rows = []
+    for (D, N, L, P) in [(1, 12, 10.0, 8), (2, 20, 6.0, 10), (3, 27, 4.0, 12)]:
+        for _ in range(10):
+            R = rng.uniform(-L / 2, L / 2, (N, D))
+            uR = rng.normal(0, 0.3, P)
+            p = int(rng.integers(0, N))
+            newpos = R[p] + rng.normal(0, 0.4, D)
+            q = C.c_double(0)
+            spa = np.array([0.5, 1.0])
+            en = r.ref_bosonsbulk_move(N, D, L, P, f(spa), 2, f(np.ascontiguousarray(R)), f(uR), p,
+                                       f(np.ascontiguousarray(newpos)), C.byref(q))
+            rows.append(dict(D=D, N=N, L=L, P=P, R=R.ravel().tolist(), uR=uR.tolist(), p=p,
+                             newpos=newpos.tolist(), exponent_new=en, quotient=q.value))
+    with open(os.path.join(HERE, "move_cases.json"), "w") as fh:
+        json.dump(rows, fh)
+
+
+def configs():
+    for name in ["BosonsBulk1D.config", "InhContactBosonsLinearResponseToPulse.config"]:
+        src = os.path.join(REF, "config", name)
+        if os.path.exists(src):
+            shutil.copyfile(src, os.path.join(HERE, "config_" + name.replace(".config", ".json")))
+
+
+if __name__ == "__main__":
+    if not o.ref_available():
+        sys.exit("oracle/_ref/libref.so missing: run `make -C oracle ref` in a container with the reference")
+    nic()
+    eval_cases()
+    move_cases()
+    configs()
+    print("golden vectors written to", HERE)
