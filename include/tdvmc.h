@@ -1,0 +1,196 @@
+/*
+ * tdvmc.h -- C ABI of the MI355X (gfx950) TDVMC hot path: batched Metropolis
+ * sampling of spline-Jastrow (+ k-space, + one-body) bosonic wavefunctions,
+ * local energies, log-derivatives O_k, TDVP moments S / F, the FP64
+ * regularised solve and the ODE step.
+ *
+ * The reference exposes this path as the in-process C++ plug-in interface
+ * PhysicalSystems::IPhysicalSystem (src/PhysicalSystems/IPhysicalSystem.h:98-122)
+ * driven per walker by DoMetropolisStep (src/TDVMC.cpp:858-916) and
+ * UpdateExpectationValues (src/TDVMC.cpp:1038-1150); its per-move granularity
+ * is too fine for an ABI crossing, so this boundary sits at the batch level:
+ * one call sweeps / measures every walker of a context.  Each entry point
+ * below names the reference interface it replaces.
+ *
+ * Conventions
+ *   - every function returns TDVMC_OK (0) or an error code; the message of the
+ *     last failure is tdvmc_last_error(ctx) (ctx may be NULL for failures of
+ *     tdvmc_create);
+ *   - host arrays are caller-owned, row-major, FP64;  R is [W][N][DIM];
+ *     O is [W][P];  S is [P][P];
+ *   - parameter vector layout: [one-body (0 or 1)][pair spline N_PARAM][k-space K];
+ *   - one context owns its device buffers and one HIP stream; calls on one
+ *     context are not thread-safe, distinct contexts may be used concurrently;
+ *     calls returning host data synchronise the context's stream;
+ *   - precision: TDVMC_FP64 (plain form) or TDVMC_MIXED (FP32 pair / spline /
+ *     phase arithmetic, bf16x3 MFMA moments, FP64 solve); TDVMC_PRECISION_DEFAULT
+ *     selects the context's configured precision.
+ */
+#ifndef TDVMC_H
+#define TDVMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDVMC_OK 0
+#define TDVMC_ERR_BAD_ARG 1
+#define TDVMC_ERR_BAD_CONFIG 2
+#define TDVMC_ERR_HIP 3
+#define TDVMC_ERR_NOT_PD 4
+#define TDVMC_ERR_NONFINITE 5
+#define TDVMC_ERR_STATE 6
+
+#define TDVMC_PRECISION_DEFAULT (-1)
+#define TDVMC_FP64 0
+#define TDVMC_MIXED 1
+
+typedef struct tdvmc_ctx tdvmc_ctx;
+
+typedef struct {
+    int N;                 /* particles per walker */
+    int DIM;
+    int P;                 /* total parameters */
+    int P_pair;            /* pair-spline parameters (config N_PARAM) */
+    int K;                 /* k-space terms */
+    int one_body;          /* 1 if parameter 0 is the one-body width parameter */
+    int64_t n_walkers;
+    double L;              /* box length (0: open space) */
+    double rc;             /* spline cutoff */
+    double hbar2_2m;
+    int imaginary_time;
+    int ode_solver_type;
+    int precision;
+    int samples_per_walker;
+    double timestep;
+    double mc_step;
+    int64_t therm_moves;   /* MC_NTHERMSTEPS: trial moves between measurements */
+    int64_t init_moves;    /* MC_NINITIALIZATIONSTEPS: trial moves at the start of a stage */
+    double time;
+    double reg_eps;
+    int precondition;
+} tdvmc_info;
+
+/* Moments of one sampling stage; fields mirror SimulationStepData
+ * (src/SimulationStepData.h:7-31).  Array pointers are caller-owned and may be
+ * NULL when not wanted. */
+typedef struct {
+    double time;
+    int step;
+    int64_t n_samples;
+    double localEnergyR;                 /* <E_R> */
+    double localEnergyI;                 /* <E_I> */
+    double* localOperators;              /* [P]   <O_k> */
+    double* localOperatorsMatrix;        /* [P*P] <O_k O_j> */
+    double* localOperatorlocalEnergyR;   /* [P]   <O_k E_R> */
+    double* localOperatorlocalEnergyI;   /* [P]   <O_k E_I> */
+    double* S;                           /* [P*P] <O O^T> - <O><O>^T */
+    double* FR;                          /* [P]   <O E_R> - <O><E_R> */
+    double* FI;                          /* [P]   <O E_I> - <O><E_I> */
+    double* otherExpectationValues;      /* [9]   order of BosonsBulk.cpp:449-457 */
+} tdvmc_step_data;
+
+/* Replaces ReadConfig + InitializePhysicalSystem + IPhysicalSystem::InitSystem
+ * (src/TDVMC.cpp:351-375, :2786-2998; src/PhysicalSystems/BosonsBulk.cpp:51-156).
+ * config: the text of a config/*.config JSON document or a path to one. */
+int tdvmc_create(const char* config, tdvmc_ctx** out);
+int tdvmc_destroy(tdvmc_ctx* ctx);
+/* Host-only: parse and validate a config without touching the GPU (the
+ * checks of ReadConfig / BosonsBulk::InitSystem, src/TDVMC.cpp:351-375 and
+ * src/PhysicalSystems/BosonsBulk.cpp:85-91); info receives the derived shape. */
+int tdvmc_parse_config(const char* config, tdvmc_info* info);
+const char* tdvmc_last_error(const tdvmc_ctx* ctx);
+int tdvmc_get_info(tdvmc_ctx* ctx, tdvmc_info* info);
+
+/* Number of walkers (one walker per MPI rank in the reference, src/TDVMC.cpp:3082). */
+int tdvmc_set_walker_count(tdvmc_ctx* ctx, int64_t W);
+
+/* Replaces BroadcastNewParameters (src/TDVMC.cpp:506-512): alpha in. */
+int tdvmc_set_params(tdvmc_ctx* ctx, const double* uR, const double* uI, double phiR, double phiI, int P);
+int tdvmc_get_params(tdvmc_ctx* ctx, double* uR, double* uI, double* phiR, double* phiI, int P);
+
+/* Replaces IPhysicalSystem::SetTime (src/PhysicalSystems/IPhysicalSystem.h:77):
+ * selects the quenched pair potential (src/PhysicalSystems/BosonsBulk.cpp:236-243). */
+int tdvmc_set_time(tdvmc_ctx* ctx, double t);
+
+/* Walker coordinates [W][N][DIM] in / out (replaces the per-rank R of mainMPI). */
+int tdvmc_set_walkers(tdvmc_ctx* ctx, const double* R, int64_t W);
+int tdvmc_get_walkers(tdvmc_ctx* ctx, double* R, int64_t W);
+/* Replaces InitCoordinateConfiguration (src/TDVMC.cpp:672-785): jittered
+ * lattice in the box, or a Gaussian cloud for open systems. */
+int tdvmc_init_walkers(tdvmc_ctx* ctx, uint64_t seed);
+
+/* Replaces DoMetropolisSteps (src/TDVMC.cpp:918-924) for every walker:
+ * n_moves single-particle trial moves per walker. */
+int tdvmc_sweep(tdvmc_ctx* ctx, int64_t n_moves, uint64_t seed, int precision);
+int tdvmc_get_acceptance(tdvmc_ctx* ctx, int64_t* accepted, int64_t* trials);
+
+/* Replaces UpdateExpectationValues + ParallelUpdateExpectationValues
+ * (src/TDVMC.cpp:1038-1220): init_moves, then samples_per_walker x
+ * (therm_moves + one measurement) for every walker, then the moments. */
+int tdvmc_sample(tdvmc_ctx* ctx, uint64_t seed, int precision, tdvmc_step_data* out);
+
+/* Measurement of the current walkers only (IPhysicalSystem::CalculateExpectationValues,
+ * src/PhysicalSystems/BosonsBulk.cpp:460-466, batched) followed by the moments. */
+int tdvmc_measure(tdvmc_ctx* ctx, int precision, tdvmc_step_data* out);
+
+/* Parity entry on fed configurations (IPhysicalSystem::CalculateWavefunction +
+ * CalculateExpectationValues, src/PhysicalSystems/BosonsBulk.cpp:179-539).
+ * Outputs may be NULL: expo [W], ER [W], EI [W], O [W][P], grad [W][N][DIM][2]
+ * (Re, Im of grad ln psi), lap [W][N][2], other [W][9]. */
+int tdvmc_eval_configs(tdvmc_ctx* ctx, const double* R, int64_t W, int precision, double* expo, double* ER,
+                       double* EI, double* O, double* grad, double* lap, double* other);
+
+/* Device-pointer variant of tdvmc_eval_configs (no host copies; enqueued on
+ * `stream` (a hipStream_t) or the context stream when NULL; returns without
+ * synchronising).  O_dev is FP64 [W][P]. */
+int tdvmc_eval_configs_dev(tdvmc_ctx* ctx, const double* R_dev, int64_t W, int precision, double* expo_dev,
+                           double* ER_dev, double* EI_dev, double* O_dev, double* grad_dev, double* lap_dev,
+                           void* stream);
+
+/* TDVP moments of given samples (the accumulation of src/TDVMC.cpp:1103-1109
+ * and the centring of :1506-1537).  mean [P+2] = (<O>, <E_R>, <E_I>),
+ * S [P][P], F [2][P] = (cov(O, E_R), cov(O, E_I)). */
+int tdvmc_moments(tdvmc_ctx* ctx, const double* O, const double* ER, const double* EI, int64_t W, int P,
+                  int precision, double* mean, double* S, double* F);
+
+/* SolveForParametersDot, solver type 0 (src/TDVMC.cpp:1713-1762):
+ * xdot [2P+2] = (alpha-dot R, alpha-dot I, phi-dot R, phi-dot I);
+ * *n_bad = count of non-positive Cholesky pivots (TDVMC_ERR_NOT_PD if > 0). */
+int tdvmc_solve(tdvmc_ctx* ctx, const double* mean, const double* S, const double* F, int P, int imag_time,
+                int precondition, double eps, double* xdot, int* n_bad);
+
+/* One TDVP time step on the device: CalculateNextParametersEuler (type 0,
+ * src/TDVMC.cpp:1834-1853) or CalculateNextParametersRK4 (type 2,
+ * src/TDVMC.cpp:1969-2035, fresh samples per stage, chain continued).
+ * ode_type < 0 uses the configured ODE_SOLVER_TYPE.  With use_graph != 0 the
+ * step is captured once into a hipGraph and replayed. */
+int tdvmc_tdvp_step(tdvmc_ctx* ctx, double dt, int ode_type, int precision, int use_graph);
+
+/* Adjoint of Re ln psi over fed configurations: given gbar [W],
+ * dalpha [P] = sum_w gbar_w d(Re ln psi_w)/d(alpha)  (= O_w, plus the count of
+ * pairs beyond the cutoff for the last pair parameter, whose outer-pair term
+ * src/PhysicalSystems/BosonsBulk.cpp:530-534 leaves out of O) and
+ * dR [W][N][DIM] = gbar_w Re grad ln psi_w.
+ * No counterpart in the reference (which has no differentiation API); it is
+ * the backward of a torch.autograd.Function test wrapper. */
+int tdvmc_adjoint(tdvmc_ctx* ctx, const double* R, int64_t W, const double* gbar, int precision, double* dR,
+                  double* dalpha);
+
+/* Kernel timing with HIP events on the context stream (Timer, src/Timer.h).
+ * names: "sweep", "eval", "moments", "solve".  Accumulated since enable. */
+int tdvmc_profile_enable(tdvmc_ctx* ctx, int enable);
+int tdvmc_profile_get(tdvmc_ctx* ctx, const char* name, double* total_ms, int64_t* launches);
+
+/* Lanes per walker of the sweep kernel (16, 32, 64; 0 = automatic). */
+int tdvmc_set_sweep_lanes(tdvmc_ctx* ctx, int lanes_per_walker);
+
+int tdvmc_synchronize(tdvmc_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TDVMC_H */

@@ -1,0 +1,1323 @@
+// C ABI of the TDVMC hot path: context, configuration, device buffers and the
+// host-side orchestration of the kernels (see include/tdvmc.h).
+
+#include "../../include/tdvmc.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "config.hpp"
+#include "kernels.hpp"
+
+using namespace tdvmc;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct Prof {
+    double ms = 0.0;
+    int64_t launches = 0;
+};
+
+template <typename T>
+struct DevTables {
+    T* spl = nullptr;
+    T* kv = nullptr;
+    T* alpha = nullptr;
+    bool dirty = true;
+};
+
+}  // namespace
+
+struct tdvmc_ctx {
+    Config cfg;
+    KSys sy{};
+    std::vector<double> pot;          // SYSTEM_PARAMS
+    int pot_type = 0;
+    double time = 0.0;
+    int step = 0;
+    int precision = TDVMC_FP64;
+    int ode_type = 0, imag_time = 0, precondition = 0, spw = 1;
+    double reg_eps = 1e-3, dt = 0.0;
+    int64_t therm_moves = 0, init_moves = 0;
+    uint64_t seed = 1;
+    int lanes = 0;
+    std::string err;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    std::vector<double> uR, uI;
+    double phiR = 0.0, phiI = 0.0;
+    bool host_params_stale = false;
+
+    int64_t W = 0;
+    double* dR = nullptr;
+    unsigned long long* dAcc = nullptr;
+    uint32_t* dEpoch = nullptr;
+    double* dUR = nullptr;
+    double* dUI = nullptr;
+    double* dX = nullptr;       // packed parameters [2P+2]
+    double* dXtmp = nullptr;
+    double* dK[4] = {nullptr, nullptr, nullptr, nullptr};
+    DevTables<float> tf;
+    DevTables<double> td;
+    std::vector<float> kv_f;
+    std::vector<double> kv_d;
+
+    int64_t cap_samples = 0;
+    double* dO64 = nullptr;
+    float* dO32 = nullptr;
+    double* dER = nullptr;
+    double* dEI = nullptr;
+    double* dExpo = nullptr;
+    double* dOther = nullptr;
+
+    double* dMean = nullptr;
+    double* dS = nullptr;
+    double* dF = nullptr;
+    MomentsWork mw{};
+    int64_t mw_samples = 0;
+
+    double* dWork = nullptr;
+    int* dStatus = nullptr;
+
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_src = nullptr;
+    std::string graph_key;
+
+    bool prof = false;
+    struct Pending {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::map<std::string, Prof> prof_acc;
+    int64_t trials_total = 0;
+};
+
+namespace {
+
+#define HIPC(expr)                                                                  \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);           \
+            return TDVMC_ERR_HIP;                                                   \
+        }                                                                           \
+    } while (0)
+
+#define CHECK_ARG(cond, msg)            \
+    do {                                \
+        if (!(cond)) {                  \
+            ctx->err = (msg);           \
+            return TDVMC_ERR_BAD_ARG;   \
+        }                               \
+    } while (0)
+
+int P_of(const tdvmc_ctx* ctx) { return ctx->sy.P; }
+
+template <typename T>
+hipError_t dalloc(T** p, size_t n)
+{
+    if (*p) {
+        hipFree(*p);
+        *p = nullptr;
+    }
+    if (n == 0) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+}
+
+template <typename T>
+void dfree(T*& p)
+{
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+// K shortest non-zero integer vectors ordered by (|n|^2, lexicographic n);
+// both n and -n are kept.
+std::vector<std::vector<int>> shortest_lattice_vectors(int D, int K)
+{
+    std::vector<std::vector<int>> out;
+    if (K <= 0) return out;
+    for (int m = 1;; m++) {
+        std::vector<std::vector<int>> vs;
+        std::vector<int> v(D, -m);
+        while (true) {
+            bool nz = false;
+            for (int a = 0; a < D; a++) nz |= v[a] != 0;
+            if (nz) vs.push_back(v);
+            int a = D - 1;
+            while (a >= 0 && v[a] == m) {
+                v[a] = -m;
+                a--;
+            }
+            if (a < 0) break;
+            v[a]++;
+        }
+        auto n2 = [](const std::vector<int>& x) {
+            int s = 0;
+            for (int e : x) s += e * e;
+            return s;
+        };
+        std::sort(vs.begin(), vs.end(), [&](const std::vector<int>& a, const std::vector<int>& b) {
+            int na = n2(a), nb = n2(b);
+            if (na != nb) return na < nb;
+            return a < b;
+        });
+        if ((int)vs.size() >= K && n2(vs[K - 1]) < (m + 1) * (m + 1)) {
+            vs.resize(K);
+            return vs;
+        }
+    }
+}
+
+void resolve_potential(tdvmc_ctx* ctx)
+{
+    const auto& p = ctx->pot;
+    double a = p.size() > 0 ? p[0] : 0.0, b = p.size() > 1 ? p[1] : 0.0;
+    if ((ctx->pot_type == 1 || ctx->pot_type == 2) && p.size() > 4 && ctx->time >= p[2]) {
+        a = p[3];
+        b = p[4];
+    }
+    ctx->sy.pot_a = a;
+    ctx->sy.pot_b = b;
+}
+
+int build_system(tdvmc_ctx* ctx)
+{
+    Config& c = ctx->cfg;
+    KSys& s = ctx->sy;
+    if (c.DIM < 1 || c.DIM > 3) {
+        ctx->err = "DIM must be 1, 2 or 3";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    if (c.N < 2) {
+        ctx->err = "N must be >= 2";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    if (c.N_PARAM < 4) {
+        ctx->err = "N_PARAM (pair-spline parameters) must be >= 4";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    if (c.USE_NURBS) {
+        ctx->err = "USE_NURBS: only uniform knots are implemented";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    if (c.USE_PARAM_START != 0 || c.USE_PARAM_END != 0) {
+        ctx->err = "USE_PARAM_START/END: only the full parameter set is implemented";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    s.N = c.N;
+    s.D = c.DIM;
+    s.P_s = c.N_PARAM;
+    s.K = c.K_VECTORS;
+    s.one_body = c.ONE_BODY ? 1 : 0;
+    s.off_pair = s.one_body;
+    s.off_k = s.off_pair + s.P_s;
+    s.P = s.off_k + s.K;
+    double L = c.LBOX;
+    if (L <= 0 && c.TRAP_OMEGA == 0.0 && c.RHO > 0) L = std::pow((double)c.N / c.RHO, 1.0 / c.DIM);
+    s.periodic = L > 0 ? 1 : 0;
+    s.L = L > 0 ? L : 0.0;
+    s.Linv = L > 0 ? 1.0 / L : 0.0;
+    s.rc = c.SPLINE_CUTOFF > 0 ? c.SPLINE_CUTOFF : L / 2.0;
+    if (!(s.rc > 0)) {
+        ctx->err = "open systems need SPLINE_CUTOFF > 0";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    if (!s.periodic && s.K > 0) {
+        ctx->err = "K_VECTORS needs a periodic box (LBOX > 0)";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    if (s.K < 0 || s.K > 512) {
+        ctx->err = "K_VECTORS must be in [0, 512]";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    s.h = s.rc / (s.P_s - 1);
+    s.hinv = 1.0 / s.h;
+    s.hbar2_2m = c.HBAR2_2M > 0 ? c.HBAR2_2M : 1.0;
+    s.trap_omega = c.TRAP_OMEGA;
+    std::string pp = c.PAIR_POTENTIAL;
+    if (pp.empty()) {
+        if (c.SYSTEM_TYPE == "BosonsBulkDamped") pp = "GAUSSIAN";
+        else if (c.SYSTEM_TYPE == "BosonsBulk" || c.SYSTEM_TYPE.empty()) pp = "SQUARE_WELL";
+        else pp = "NONE";
+    }
+    if (pp == "NONE") ctx->pot_type = 0;
+    else if (pp == "SQUARE_WELL") ctx->pot_type = 1;
+    else if (pp == "GAUSSIAN") ctx->pot_type = 2;
+    else if (pp == "LJ") ctx->pot_type = 3;
+    else {
+        ctx->err = "unknown PAIR_POTENTIAL '" + pp + "'";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    ctx->pot = c.SYSTEM_PARAMS;
+    if (ctx->pot_type != 0 && ctx->pot.size() < 2) {
+        ctx->err = "SYSTEM_PARAMS needs the two pair-potential parameters";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    s.pot_type = ctx->pot_type;
+    s.pot_cutoff = s.periodic ? s.rc : 1e300;
+    s.outer_idx = s.off_pair + s.P_s - 1;
+    s.move_order = (c.MOVE_ORDER == "SEQUENTIAL") ? 1 : 0;
+    s.step = c.MC_STEP;
+    resolve_potential(ctx);
+
+    ctx->precision = (c.PRECISION == "MIXED") ? TDVMC_MIXED : TDVMC_FP64;
+    ctx->ode_type = c.ODE_SOLVER_TYPE;
+    ctx->imag_time = c.IMAGINARY_TIME;
+    ctx->precondition = c.USE_PRECONDITIONING;
+    ctx->reg_eps = c.REG_EPS > 0 ? c.REG_EPS : 1e-3;
+    ctx->dt = c.TIMESTEP;
+    ctx->therm_moves = c.MC_NTHERMSTEPS > 0 ? c.MC_NTHERMSTEPS : 0;
+    ctx->init_moves = c.MC_NINITIALIZATIONSTEPS > 0 ? c.MC_NINITIALIZATIONSTEPS : 0;
+    ctx->spw = c.SAMPLES_PER_WALKER > 0 ? c.SAMPLES_PER_WALKER : 1;
+    ctx->seed = c.SEED > 0 ? (uint64_t)c.SEED : 1u;
+
+    // parameters: PARAMS_REAL / PARAMS_IMAGINARY of length P, or of the pair block only
+    const int P = s.P;
+    ctx->uR.assign(P, 0.0);
+    ctx->uI.assign(P, 0.0);
+    auto fill = [&](const std::vector<double>& src, std::vector<double>& dst) {
+        if ((int)src.size() == s.P_s && s.P != s.P_s) {
+            for (int k = 0; k < s.P_s; k++) dst[s.off_pair + k] = src[k];
+        } else {
+            for (int k = 0; k < P && k < (int)src.size(); k++) dst[k] = src[k];
+        }
+    };
+    fill(c.PARAMS_REAL, ctx->uR);
+    fill(c.PARAMS_IMAGINARY, ctx->uI);
+    ctx->phiR = c.PARAM_PHIR;
+    ctx->phiI = c.PARAM_PHII;
+
+    // k-vectors: stored as n/L (phase in revolutions) and |k|^2
+    auto ns = shortest_lattice_vectors(s.D, s.K);
+    ctx->kv_d.assign(4 * (size_t)std::max(s.K, 1), 0.0);
+    ctx->kv_f.assign(4 * (size_t)std::max(s.K, 1), 0.0f);
+    for (int k = 0; k < s.K; k++) {
+        double n2 = 0;
+        for (int a = 0; a < s.D; a++) {
+            ctx->kv_d[4 * k + a] = ns[k][a] * s.Linv;
+            n2 += (double)ns[k][a] * ns[k][a];
+        }
+        const double tp = 2.0 * M_PI * s.Linv;
+        ctx->kv_d[4 * k + 3] = tp * tp * n2;
+    }
+    for (size_t i = 0; i < ctx->kv_d.size(); i++) ctx->kv_f[i] = (float)ctx->kv_d[i];
+
+    // resource limits of the kernels
+    const size_t lds_max = 160 * 1024;
+    if (eval_lds_bytes<double>(s, true) > lds_max) {
+        ctx->err = "system too large for the evaluation kernel's LDS budget";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    return TDVMC_OK;
+}
+
+int upload_params(tdvmc_ctx* ctx)
+{
+    const int P = P_of(ctx);
+    std::vector<double> packed(2 * P + 2);
+    std::copy(ctx->uR.begin(), ctx->uR.end(), packed.begin());
+    std::copy(ctx->uI.begin(), ctx->uI.end(), packed.begin() + P);
+    packed[2 * P] = ctx->phiR;
+    packed[2 * P + 1] = ctx->phiI;
+    HIPC(hipMemcpyAsync(ctx->dX, packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(ctx->dUR, ctx->uR.data(), P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(ctx->dUI, ctx->uI.data(), P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    ctx->tf.dirty = ctx->td.dirty = true;
+    ctx->host_params_stale = false;
+    return TDVMC_OK;
+}
+
+int pull_params(tdvmc_ctx* ctx)
+{
+    if (!ctx->host_params_stale) return TDVMC_OK;
+    const int P = P_of(ctx);
+    std::vector<double> packed(2 * P + 2);
+    HIPC(hipMemcpyAsync(packed.data(), ctx->dX, packed.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    std::copy(packed.begin(), packed.begin() + P, ctx->uR.begin());
+    std::copy(packed.begin() + P, packed.begin() + 2 * P, ctx->uI.begin());
+    ctx->phiR = packed[2 * P];
+    ctx->phiI = packed[2 * P + 1];
+    ctx->host_params_stale = false;
+    return TDVMC_OK;
+}
+
+template <typename T>
+KTables<T> tables_of(tdvmc_ctx* ctx, DevTables<T>& t)
+{
+    KTables<T> k;
+    k.spl = t.spl;
+    k.kv = t.kv;
+    k.alpha = t.alpha;
+    k.uR = ctx->dUR;
+    k.uI = ctx->dUI;
+    return k;
+}
+
+template <typename T>
+int ensure_tables(tdvmc_ctx* ctx, DevTables<T>& t, bool force)
+{
+    if (!t.dirty && !force) return TDVMC_OK;
+    HIPC(launch_prep_tables<T>(ctx->sy, ctx->dUR, ctx->dUI, t.spl, t.alpha, ctx->stream));
+    t.dirty = false;
+    return TDVMC_OK;
+}
+
+int prof_begin(tdvmc_ctx* ctx, const char* name, hipEvent_t* a)
+{
+    if (!ctx->prof) return TDVMC_OK;
+    HIPC(hipEventCreate(a));
+    HIPC(hipEventRecord(*a, ctx->stream));
+    (void)name;
+    return TDVMC_OK;
+}
+
+int prof_end(tdvmc_ctx* ctx, const char* name, hipEvent_t a)
+{
+    if (!ctx->prof) return TDVMC_OK;
+    hipEvent_t b;
+    HIPC(hipEventCreate(&b));
+    HIPC(hipEventRecord(b, ctx->stream));
+    ctx->pending.push_back({name, a, b});
+    return TDVMC_OK;
+}
+
+int prof_flush(tdvmc_ctx* ctx)
+{
+    if (ctx->pending.empty()) return TDVMC_OK;
+    HIPC(hipStreamSynchronize(ctx->stream));
+    for (auto& p : ctx->pending) {
+        float ms = 0.f;
+        HIPC(hipEventElapsedTime(&ms, p.a, p.b));
+        auto& acc = ctx->prof_acc[p.name];
+        acc.ms += ms;
+        acc.launches++;
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    ctx->pending.clear();
+    return TDVMC_OK;
+}
+
+int ensure_samples(tdvmc_ctx* ctx, int64_t n)
+{
+    if (n <= ctx->cap_samples) return TDVMC_OK;
+    const int P = P_of(ctx);
+    HIPC(dalloc(&ctx->dO64, (size_t)n * P));
+    HIPC(dalloc(&ctx->dO32, (size_t)n * P));
+    HIPC(dalloc(&ctx->dER, (size_t)n));
+    HIPC(dalloc(&ctx->dEI, (size_t)n));
+    HIPC(dalloc(&ctx->dExpo, (size_t)n));
+    HIPC(dalloc(&ctx->dOther, (size_t)n * 9));
+    ctx->cap_samples = n;
+    return TDVMC_OK;
+}
+
+int ensure_moments(tdvmc_ctx* ctx, int64_t n)
+{
+    if (n <= ctx->mw_samples) return TDVMC_OK;
+    const int P = P_of(ctx);
+    size_t x0, p0, c0, x1, p1, c1;
+    moments_work_bytes(n, P, 0, &x0, &p0, &c0);
+    moments_work_bytes(n, P, 1, &x1, &p1, &c1);
+    size_t xb = std::max(x0, x1), pb = std::max(p0, p1), cb = std::max(c0, c1);
+    void* xt = ctx->mw.xt;
+    if (xt) hipFree(xt);
+    ctx->mw.xt = nullptr;
+    HIPC(hipMalloc(&ctx->mw.xt, xb));
+    HIPC(dalloc(&ctx->mw.partial, (pb + 7) / 8));
+    HIPC(dalloc(&ctx->mw.colsum, (cb + 7) / 8));
+    ctx->mw.xt_bytes = xb;
+    ctx->mw.partial_bytes = pb;
+    ctx->mw.colsum_bytes = cb;
+    ctx->mw_samples = n;
+    return TDVMC_OK;
+}
+
+int resolve_precision(tdvmc_ctx* ctx, int precision, int* out)
+{
+    if (precision == TDVMC_PRECISION_DEFAULT) precision = ctx->precision;
+    if (precision != TDVMC_FP64 && precision != TDVMC_MIXED) {
+        ctx->err = "precision must be TDVMC_FP64 or TDVMC_MIXED";
+        return TDVMC_ERR_BAD_ARG;
+    }
+    *out = precision;
+    return TDVMC_OK;
+}
+
+int enqueue_sweep(tdvmc_ctx* ctx, int prec, int64_t n_moves, uint64_t seed)
+{
+    if (n_moves <= 0) return TDVMC_OK;
+    SweepArgs a;
+    a.R = ctx->dR;
+    a.W = ctx->W;
+    a.n_moves = n_moves;
+    a.seed = seed;
+    a.epoch = ctx->dEpoch;
+    a.acc = ctx->dAcc;
+    a.lanes_per_walker = ctx->lanes;
+    hipEvent_t ev{};
+    int rc = prof_begin(ctx, "sweep", &ev);
+    if (rc) return rc;
+    if (prec == TDVMC_MIXED) {
+        HIPC(launch_sweep<float>(ctx->sy, tables_of(ctx, ctx->tf), a, ctx->stream));
+    } else {
+        HIPC(launch_sweep<double>(ctx->sy, tables_of(ctx, ctx->td), a, ctx->stream));
+    }
+    rc = prof_end(ctx, "sweep", ev);
+    if (rc) return rc;
+    HIPC(launch_add_u32(ctx->dEpoch, 1u, ctx->stream));
+    ctx->trials_total += n_moves * ctx->W;
+    return TDVMC_OK;
+}
+
+// eval of the context walkers into sample rows [row0, row0 + W)
+int enqueue_eval(tdvmc_ctx* ctx, int prec, int64_t row0)
+{
+    const int P = P_of(ctx);
+    hipEvent_t ev{};
+    int rc = prof_begin(ctx, "eval", &ev);
+    if (rc) return rc;
+    if (prec == TDVMC_MIXED) {
+        HIPC((launch_eval<float, float>(ctx->sy, tables_of(ctx, ctx->tf), ctx->dR, ctx->W, ctx->dExpo + row0,
+                                        ctx->dER + row0, ctx->dEI + row0, ctx->dO32 + row0 * P,
+                                        ctx->dOther + row0 * 9, nullptr, nullptr, nullptr, ctx->phiR, ctx->stream)));
+    } else {
+        HIPC((launch_eval<double, double>(ctx->sy, tables_of(ctx, ctx->td), ctx->dR, ctx->W, ctx->dExpo + row0,
+                                          ctx->dER + row0, ctx->dEI + row0, ctx->dO64 + row0 * P,
+                                          ctx->dOther + row0 * 9, nullptr, nullptr, nullptr, ctx->phiR, ctx->stream)));
+    }
+    return prof_end(ctx, "eval", ev);
+}
+
+int enqueue_moments(tdvmc_ctx* ctx, int prec, int64_t n)
+{
+    const int P = P_of(ctx);
+    hipEvent_t ev{};
+    int rc = prof_begin(ctx, "moments", &ev);
+    if (rc) return rc;
+    if (prec == TDVMC_MIXED) {
+        HIPC(launch_moments<float>(ctx->dO32, ctx->dER, ctx->dEI, n, P, 1, ctx->mw, ctx->dMean, ctx->dS, ctx->dF,
+                                   ctx->stream));
+    } else {
+        HIPC(launch_moments<double>(ctx->dO64, ctx->dER, ctx->dEI, n, P, 0, ctx->mw, ctx->dMean, ctx->dS, ctx->dF,
+                                    ctx->stream));
+    }
+    return prof_end(ctx, "moments", ev);
+}
+
+int enqueue_solve(tdvmc_ctx* ctx, double* xdot)
+{
+    SolveArgs a;
+    a.P = P_of(ctx);
+    a.imag_time = ctx->imag_time;
+    a.precondition = ctx->precondition;
+    a.eps = ctx->reg_eps;
+    a.mean = ctx->dMean;
+    a.S = ctx->dS;
+    a.F = ctx->dF;
+    a.work = ctx->dWork;
+    a.xdot = xdot;
+    a.status = ctx->dStatus;
+    hipEvent_t ev{};
+    int rc = prof_begin(ctx, "solve", &ev);
+    if (rc) return rc;
+    HIPC(launch_solve(a, ctx->stream));
+    return prof_end(ctx, "solve", ev);
+}
+
+// one sampling stage with the current device tables: moves, measurements, moments
+int enqueue_stage_samples(tdvmc_ctx* ctx, int prec, uint64_t seed)
+{
+    int rc = enqueue_sweep(ctx, prec, ctx->init_moves, seed);
+    if (rc) return rc;
+    for (int s = 0; s < ctx->spw; s++) {
+        rc = enqueue_sweep(ctx, prec, ctx->therm_moves, seed);
+        if (rc) return rc;
+        rc = enqueue_eval(ctx, prec, (int64_t)s * ctx->W);
+        if (rc) return rc;
+    }
+    return enqueue_moments(ctx, prec, (int64_t)ctx->spw * ctx->W);
+}
+
+int enqueue_prep(tdvmc_ctx* ctx, int prec)
+{
+    if (prec == TDVMC_MIXED) return ensure_tables(ctx, ctx->tf, true);
+    return ensure_tables(ctx, ctx->td, true);
+}
+
+// f(x): parameters x (packed, device) -> xdot
+int enqueue_rhs(tdvmc_ctx* ctx, int prec, const double* x, double* xdot)
+{
+    const int P = P_of(ctx);
+    HIPC(launch_unpack_params(P, x, ctx->dUR, ctx->dUI, ctx->stream));
+    int rc = enqueue_prep(ctx, prec);
+    if (rc) return rc;
+    rc = enqueue_stage_samples(ctx, prec, ctx->seed);
+    if (rc) return rc;
+    return enqueue_solve(ctx, xdot);
+}
+
+int enqueue_tdvp_step(tdvmc_ctx* ctx, int prec, int ode, double dt)
+{
+    const int P = P_of(ctx);
+    int rc;
+    if (ode == 0) {
+        rc = enqueue_rhs(ctx, prec, ctx->dX, ctx->dK[0]);
+        if (rc) return rc;
+        HIPC(launch_axpy_params(P, ctx->dX, ctx->dK[0], dt, ctx->dX, ctx->stream));
+    } else if (ode == 2) {
+        rc = enqueue_rhs(ctx, prec, ctx->dX, ctx->dK[0]);
+        if (rc) return rc;
+        HIPC(launch_axpy_params(P, ctx->dX, ctx->dK[0], dt / 2.0, ctx->dXtmp, ctx->stream));
+        rc = enqueue_rhs(ctx, prec, ctx->dXtmp, ctx->dK[1]);
+        if (rc) return rc;
+        HIPC(launch_axpy_params(P, ctx->dX, ctx->dK[1], dt / 2.0, ctx->dXtmp, ctx->stream));
+        rc = enqueue_rhs(ctx, prec, ctx->dXtmp, ctx->dK[2]);
+        if (rc) return rc;
+        HIPC(launch_axpy_params(P, ctx->dX, ctx->dK[2], dt, ctx->dXtmp, ctx->stream));
+        rc = enqueue_rhs(ctx, prec, ctx->dXtmp, ctx->dK[3]);
+        if (rc) return rc;
+        HIPC(launch_rk4_combine(P, ctx->dX, ctx->dK[0], ctx->dK[1], ctx->dK[2], ctx->dK[3], dt, ctx->stream));
+    } else {
+        ctx->err = "ODE_SOLVER_TYPE " + std::to_string(ode) + " is not implemented (0 Euler, 2 RK4)";
+        return TDVMC_ERR_BAD_ARG;
+    }
+    HIPC(launch_unpack_params(P, ctx->dX, ctx->dUR, ctx->dUI, ctx->stream));
+    return TDVMC_OK;
+}
+
+int fill_step_data(tdvmc_ctx* ctx, int64_t n, tdvmc_step_data* out)
+{
+    if (!out) return TDVMC_OK;
+    const int P = P_of(ctx);
+    std::vector<double> mean(P + 2), S((size_t)P * P), F(2 * (size_t)P);
+    HIPC(hipMemcpyAsync(mean.data(), ctx->dMean, mean.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipMemcpyAsync(S.data(), ctx->dS, S.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipMemcpyAsync(F.data(), ctx->dF, F.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> other;
+    if (out->otherExpectationValues) {
+        other.resize((size_t)n * 9);
+        HIPC(hipMemcpyAsync(other.data(), ctx->dOther, other.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    }
+    HIPC(hipStreamSynchronize(ctx->stream));
+    out->time = ctx->time;
+    out->step = ctx->step;
+    out->n_samples = n;
+    out->localEnergyR = mean[P];
+    out->localEnergyI = mean[P + 1];
+    for (int i = 0; i < P; i++) {
+        if (out->localOperators) out->localOperators[i] = mean[i];
+        if (out->FR) out->FR[i] = F[i];
+        if (out->FI) out->FI[i] = F[P + i];
+        if (out->localOperatorlocalEnergyR) out->localOperatorlocalEnergyR[i] = F[i] + mean[i] * mean[P];
+        if (out->localOperatorlocalEnergyI) out->localOperatorlocalEnergyI[i] = F[P + i] + mean[i] * mean[P + 1];
+        for (int j = 0; j < P; j++) {
+            const double s = S[(size_t)i * P + j];
+            if (out->S) out->S[(size_t)i * P + j] = s;
+            if (out->localOperatorsMatrix) out->localOperatorsMatrix[(size_t)i * P + j] = s + mean[i] * mean[j];
+        }
+    }
+    if (out->otherExpectationValues) {
+        for (int k = 0; k < 9; k++) {
+            double s = 0.0;
+            for (int64_t w = 0; w < n; w++) s += other[(size_t)w * 9 + k];
+            out->otherExpectationValues[k] = s / (double)n;
+        }
+    }
+    return TDVMC_OK;
+}
+
+int alloc_ctx(tdvmc_ctx* ctx)
+{
+    const int P = P_of(ctx);
+    HIPC(hipGetDevice(&ctx->device));
+    HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIPC(dalloc(&ctx->dUR, P));
+    HIPC(dalloc(&ctx->dUI, P));
+    HIPC(dalloc(&ctx->dX, 2 * P + 2));
+    HIPC(dalloc(&ctx->dXtmp, 2 * P + 2));
+    for (int i = 0; i < 4; i++) HIPC(dalloc(&ctx->dK[i], 2 * P + 2));
+    HIPC(dalloc(&ctx->dEpoch, 1));
+    HIPC(launch_fill_u32(ctx->dEpoch, 0u, 1, ctx->stream));
+    const int Ps = ctx->sy.P_s, K = std::max(ctx->sy.K, 1);
+    HIPC(dalloc(&ctx->tf.spl, (size_t)(Ps - 1) * 8));
+    HIPC(dalloc(&ctx->tf.kv, 4 * (size_t)K));
+    HIPC(dalloc(&ctx->tf.alpha, 2 * (size_t)K));
+    HIPC(dalloc(&ctx->td.spl, (size_t)(Ps - 1) * 8));
+    HIPC(dalloc(&ctx->td.kv, 4 * (size_t)K));
+    HIPC(dalloc(&ctx->td.alpha, 2 * (size_t)K));
+    HIPC(hipMemcpyAsync(ctx->tf.kv, ctx->kv_f.data(), ctx->kv_f.size() * sizeof(float), hipMemcpyHostToDevice,
+                        ctx->stream));
+    HIPC(hipMemcpyAsync(ctx->td.kv, ctx->kv_d.data(), ctx->kv_d.size() * sizeof(double), hipMemcpyHostToDevice,
+                        ctx->stream));
+    HIPC(dalloc(&ctx->dMean, P + 2));
+    HIPC(dalloc(&ctx->dS, (size_t)P * P));
+    HIPC(dalloc(&ctx->dF, 2 * (size_t)P));
+    HIPC(dalloc(&ctx->dWork, (size_t)P * P + 4 * (size_t)P));
+    HIPC(dalloc(&ctx->dStatus, 1));
+    int rc = upload_params(ctx);
+    if (rc) return rc;
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return TDVMC_OK;
+}
+
+void release(tdvmc_ctx* ctx)
+{
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (auto& p : ctx->pending) {
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    if (ctx->graph) hipGraphExecDestroy(ctx->graph);
+    if (ctx->graph_src) hipGraphDestroy(ctx->graph_src);
+    dfree(ctx->dR);
+    dfree(ctx->dAcc);
+    dfree(ctx->dEpoch);
+    dfree(ctx->dUR);
+    dfree(ctx->dUI);
+    dfree(ctx->dX);
+    dfree(ctx->dXtmp);
+    for (auto& k : ctx->dK) dfree(k);
+    dfree(ctx->tf.spl);
+    dfree(ctx->tf.kv);
+    dfree(ctx->tf.alpha);
+    dfree(ctx->td.spl);
+    dfree(ctx->td.kv);
+    dfree(ctx->td.alpha);
+    dfree(ctx->dO64);
+    dfree(ctx->dO32);
+    dfree(ctx->dER);
+    dfree(ctx->dEI);
+    dfree(ctx->dExpo);
+    dfree(ctx->dOther);
+    dfree(ctx->dMean);
+    dfree(ctx->dS);
+    dfree(ctx->dF);
+    if (ctx->mw.xt) hipFree(ctx->mw.xt);
+    dfree(ctx->mw.partial);
+    dfree(ctx->mw.colsum);
+    dfree(ctx->dWork);
+    dfree(ctx->dStatus);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tdvmc_create(const char* config, tdvmc_ctx** out)
+{
+    if (!config || !out) {
+        g_create_error = "tdvmc_create: null argument";
+        return TDVMC_ERR_BAD_ARG;
+    }
+    *out = nullptr;
+    tdvmc_ctx* ctx = new tdvmc_ctx();
+    try {
+        ctx->cfg = parse_config(read_config_text(config));
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        delete ctx;
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    int rc = build_system(ctx);
+    if (rc == TDVMC_OK) rc = alloc_ctx(ctx);
+    if (rc == TDVMC_OK) {
+        const int64_t W = ctx->cfg.N_WALKERS > 0 ? ctx->cfg.N_WALKERS : 1;
+        rc = tdvmc_set_walker_count(ctx, W);
+        if (rc == TDVMC_OK) rc = tdvmc_init_walkers(ctx, ctx->seed);
+    }
+    if (rc != TDVMC_OK) {
+        g_create_error = ctx->err;
+        release(ctx);
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return TDVMC_OK;
+}
+
+int tdvmc_parse_config(const char* config, tdvmc_info* info)
+{
+    if (!config || !info) {
+        g_create_error = "tdvmc_parse_config: null argument";
+        return TDVMC_ERR_BAD_ARG;
+    }
+    tdvmc_ctx tmp;
+    try {
+        tmp.cfg = parse_config(read_config_text(config));
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    int rc = build_system(&tmp);
+    if (rc != TDVMC_OK) {
+        g_create_error = tmp.err;
+        return rc;
+    }
+    tmp.W = tmp.cfg.N_WALKERS > 0 ? tmp.cfg.N_WALKERS : 1;
+    return tdvmc_get_info(&tmp, info);
+}
+
+int tdvmc_destroy(tdvmc_ctx* ctx)
+{
+    if (!ctx) return TDVMC_OK;
+    release(ctx);
+    delete ctx;
+    return TDVMC_OK;
+}
+
+const char* tdvmc_last_error(const tdvmc_ctx* ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int tdvmc_get_info(tdvmc_ctx* ctx, tdvmc_info* info)
+{
+    if (!ctx || !info) return TDVMC_ERR_BAD_ARG;
+    std::memset(info, 0, sizeof(*info));
+    info->N = ctx->sy.N;
+    info->DIM = ctx->sy.D;
+    info->P = ctx->sy.P;
+    info->P_pair = ctx->sy.P_s;
+    info->K = ctx->sy.K;
+    info->one_body = ctx->sy.one_body;
+    info->n_walkers = ctx->W;
+    info->L = ctx->sy.L;
+    info->rc = ctx->sy.rc;
+    info->hbar2_2m = ctx->sy.hbar2_2m;
+    info->imaginary_time = ctx->imag_time;
+    info->ode_solver_type = ctx->ode_type;
+    info->precision = ctx->precision;
+    info->samples_per_walker = ctx->spw;
+    info->timestep = ctx->dt;
+    info->mc_step = ctx->sy.step;
+    info->therm_moves = ctx->therm_moves;
+    info->init_moves = ctx->init_moves;
+    info->time = ctx->time;
+    info->reg_eps = ctx->reg_eps;
+    info->precondition = ctx->precondition;
+    return TDVMC_OK;
+}
+
+int tdvmc_set_walker_count(tdvmc_ctx* ctx, int64_t W)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(W >= 1 && W <= (int64_t)1 << 31, "walker count out of range");
+    if (W == ctx->W) return TDVMC_OK;
+    HIPC(hipStreamSynchronize(ctx->stream));
+    HIPC(dalloc(&ctx->dR, (size_t)W * ctx->sy.N * ctx->sy.D));
+    HIPC(dalloc(&ctx->dAcc, (size_t)W));
+    HIPC(hipMemsetAsync(ctx->dAcc, 0, (size_t)W * sizeof(unsigned long long), ctx->stream));
+    ctx->W = W;
+    ctx->trials_total = 0;
+    int rc = ensure_samples(ctx, W * ctx->spw);
+    if (rc) return rc;
+    rc = ensure_moments(ctx, W * ctx->spw);
+    if (rc) return rc;
+    if (ctx->graph) {
+        hipGraphExecDestroy(ctx->graph);
+        ctx->graph = nullptr;
+    }
+    if (ctx->graph_src) {
+        hipGraphDestroy(ctx->graph_src);
+        ctx->graph_src = nullptr;
+    }
+    ctx->graph_key.clear();
+    return TDVMC_OK;
+}
+
+int tdvmc_set_params(tdvmc_ctx* ctx, const double* uR, const double* uI, double phiR, double phiI, int P)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(P == P_of(ctx), "parameter count does not match the configuration (P = " + std::to_string(P_of(ctx)) + ")");
+    CHECK_ARG(uR != nullptr, "uR is null");
+    for (int i = 0; i < P; i++) {
+        ctx->uR[i] = uR[i];
+        ctx->uI[i] = uI ? uI[i] : 0.0;
+    }
+    ctx->phiR = phiR;
+    ctx->phiI = phiI;
+    return upload_params(ctx);
+}
+
+int tdvmc_get_params(tdvmc_ctx* ctx, double* uR, double* uI, double* phiR, double* phiI, int P)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(P == P_of(ctx), "parameter count does not match the configuration");
+    int rc = pull_params(ctx);
+    if (rc) return rc;
+    for (int i = 0; i < P; i++) {
+        if (uR) uR[i] = ctx->uR[i];
+        if (uI) uI[i] = ctx->uI[i];
+    }
+    if (phiR) *phiR = ctx->phiR;
+    if (phiI) *phiI = ctx->phiI;
+    return TDVMC_OK;
+}
+
+int tdvmc_set_time(tdvmc_ctx* ctx, double t)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    ctx->time = t;
+    resolve_potential(ctx);
+    return TDVMC_OK;
+}
+
+int tdvmc_set_walkers(tdvmc_ctx* ctx, const double* R, int64_t W)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(R != nullptr, "R is null");
+    int rc = tdvmc_set_walker_count(ctx, W);
+    if (rc) return rc;
+    HIPC(hipMemcpyAsync(ctx->dR, R, (size_t)W * ctx->sy.N * ctx->sy.D * sizeof(double), hipMemcpyHostToDevice,
+                        ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return TDVMC_OK;
+}
+
+int tdvmc_get_walkers(tdvmc_ctx* ctx, double* R, int64_t W)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(R != nullptr && W == ctx->W, "R is null or W does not match the walker count");
+    HIPC(hipMemcpyAsync(R, ctx->dR, (size_t)W * ctx->sy.N * ctx->sy.D * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return TDVMC_OK;
+}
+
+int tdvmc_init_walkers(tdvmc_ctx* ctx, uint64_t seed)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    const int N = ctx->sy.N, D = ctx->sy.D;
+    const int64_t W = ctx->W;
+    std::vector<double> R((size_t)W * N * D, 0.0);
+    if (ctx->sy.periodic) {
+        // jittered simple lattice of InitCoordinateConfiguration (src/TDVMC.cpp:713-760):
+        // spacing l = L / n, jitter (u - 0.5) l / 10; n = ceil(N^(1/D)) so every
+        // particle gets a site when N is not a perfect power
+        int n = (int)std::lround(std::pow((double)N, 1.0 / D));
+        while (std::pow((double)n, D) < N) n++;
+        const double l = ctx->sy.L / n;
+        for (int64_t w = 0; w < W; w++) {
+            std::mt19937_64 gen(seed * 1000003ull + (uint64_t)w);
+            std::uniform_real_distribution<double> u(0.0, 1.0);
+            double* Rw = R.data() + (size_t)w * N * D;
+            for (int p = 0; p < N; p++) {
+                int idx = p;
+                int c[3] = {0, 0, 0};
+                for (int a = D - 1; a >= 0; a--) {
+                    c[a] = idx % n;
+                    idx /= n;
+                }
+                for (int a = 0; a < D; a++) {
+                    double x = c[a] * l + (u(gen) - 0.5) * l / 10.0;
+                    if (D == 1) x -= ctx->sy.L / 2.0;
+                    Rw[(size_t)p * D + a] = x;
+                }
+            }
+        }
+    } else {
+        // open space: Gaussian cloud of the one-body width (or unit width)
+        double sigma = 1.0;
+        if (ctx->sy.one_body && ctx->uR[0] > 0) sigma = std::sqrt(1.0 / (4.0 * ctx->uR[0]));
+        for (int64_t w = 0; w < W; w++) {
+            std::mt19937_64 gen(seed * 1000003ull + (uint64_t)w);
+            std::normal_distribution<double> g(0.0, sigma);
+            for (size_t i = 0; i < (size_t)N * D; i++) R[(size_t)w * N * D + i] = g(gen);
+        }
+    }
+    HIPC(hipMemcpyAsync(ctx->dR, R.data(), R.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return TDVMC_OK;
+}
+
+int tdvmc_sweep(tdvmc_ctx* ctx, int64_t n_moves, uint64_t seed, int precision)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(n_moves >= 0, "n_moves must be >= 0");
+    int prec, rc = resolve_precision(ctx, precision, &prec);
+    if (rc) return rc;
+    rc = prec == TDVMC_MIXED ? ensure_tables(ctx, ctx->tf, false) : ensure_tables(ctx, ctx->td, false);
+    if (rc) return rc;
+    rc = enqueue_sweep(ctx, prec, n_moves, seed);
+    if (rc) return rc;
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return prof_flush(ctx);
+}
+
+int tdvmc_get_acceptance(tdvmc_ctx* ctx, int64_t* accepted, int64_t* trials)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    std::vector<unsigned long long> acc((size_t)ctx->W);
+    HIPC(hipMemcpyAsync(acc.data(), ctx->dAcc, acc.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    int64_t s = 0;
+    for (auto v : acc) s += (int64_t)v;
+    if (accepted) *accepted = s;
+    if (trials) *trials = ctx->trials_total;
+    return TDVMC_OK;
+}
+
+int tdvmc_sample(tdvmc_ctx* ctx, uint64_t seed, int precision, tdvmc_step_data* out)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    int prec, rc = resolve_precision(ctx, precision, &prec);
+    if (rc) return rc;
+    rc = prec == TDVMC_MIXED ? ensure_tables(ctx, ctx->tf, false) : ensure_tables(ctx, ctx->td, false);
+    if (rc) return rc;
+    rc = enqueue_stage_samples(ctx, prec, seed);
+    if (rc) return rc;
+    rc = fill_step_data(ctx, (int64_t)ctx->spw * ctx->W, out);
+    if (rc) return rc;
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return prof_flush(ctx);
+}
+
+int tdvmc_measure(tdvmc_ctx* ctx, int precision, tdvmc_step_data* out)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    int prec, rc = resolve_precision(ctx, precision, &prec);
+    if (rc) return rc;
+    rc = prec == TDVMC_MIXED ? ensure_tables(ctx, ctx->tf, false) : ensure_tables(ctx, ctx->td, false);
+    if (rc) return rc;
+    rc = enqueue_eval(ctx, prec, 0);
+    if (rc) return rc;
+    rc = enqueue_moments(ctx, prec, ctx->W);
+    if (rc) return rc;
+    rc = fill_step_data(ctx, ctx->W, out);
+    if (rc) return rc;
+    return prof_flush(ctx);
+}
+
+int tdvmc_eval_configs_dev(tdvmc_ctx* ctx, const double* R_dev, int64_t W, int precision, double* expo_dev,
+                           double* ER_dev, double* EI_dev, double* O_dev, double* grad_dev, double* lap_dev,
+                           void* stream)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(R_dev && expo_dev && ER_dev && EI_dev && O_dev && W >= 1, "null device pointer or W < 1");
+    CHECK_ARG((grad_dev == nullptr) == (lap_dev == nullptr), "grad and lap must be both set or both null");
+    int prec, rc = resolve_precision(ctx, precision, &prec);
+    if (rc) return rc;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    if (st != ctx->stream) HIPC(hipStreamSynchronize(ctx->stream));
+    hipStream_t keep = ctx->stream;
+    ctx->stream = st;
+    rc = prec == TDVMC_MIXED ? ensure_tables(ctx, ctx->tf, false) : ensure_tables(ctx, ctx->td, false);
+    ctx->stream = keep;
+    if (rc) return rc;
+    if (prec == TDVMC_MIXED) {
+        HIPC((launch_eval<float, double>(ctx->sy, tables_of(ctx, ctx->tf), R_dev, W, expo_dev, ER_dev, EI_dev, O_dev,
+                                         nullptr, grad_dev, lap_dev, nullptr, ctx->phiR, st)));
+    } else {
+        HIPC((launch_eval<double, double>(ctx->sy, tables_of(ctx, ctx->td), R_dev, W, expo_dev, ER_dev, EI_dev, O_dev,
+                                          nullptr, grad_dev, lap_dev, nullptr, ctx->phiR, st)));
+    }
+    return TDVMC_OK;
+}
+
+int tdvmc_eval_configs(tdvmc_ctx* ctx, const double* R, int64_t W, int precision, double* expo, double* ER,
+                       double* EI, double* O, double* grad, double* lap, double* other)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(R != nullptr && W >= 1, "R is null or W < 1");
+    int prec, rc = resolve_precision(ctx, precision, &prec);
+    if (rc) return rc;
+    const int N = ctx->sy.N, D = ctx->sy.D, P = P_of(ctx);
+    const bool pp = grad != nullptr || lap != nullptr;
+    double *dR = nullptr, *dEx = nullptr, *dER = nullptr, *dEI = nullptr, *dO = nullptr, *dG = nullptr,
+           *dL = nullptr, *dOt = nullptr;
+    auto cleanup = [&]() {
+        dfree(dR); dfree(dEx); dfree(dER); dfree(dEI); dfree(dO); dfree(dG); dfree(dL); dfree(dOt);
+    };
+    auto hipc = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess) {
+            ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+            return false;
+        }
+        return true;
+    };
+    bool ok = hipc(dalloc(&dR, (size_t)W * N * D), "hipMalloc") && hipc(dalloc(&dEx, (size_t)W), "hipMalloc") &&
+              hipc(dalloc(&dER, (size_t)W), "hipMalloc") && hipc(dalloc(&dEI, (size_t)W), "hipMalloc") &&
+              hipc(dalloc(&dO, (size_t)W * P), "hipMalloc") && hipc(dalloc(&dOt, (size_t)W * 9), "hipMalloc");
+    if (ok && pp) ok = hipc(dalloc(&dG, (size_t)W * N * D * 2), "hipMalloc") && hipc(dalloc(&dL, (size_t)W * N * 2), "hipMalloc");
+    if (ok) ok = hipc(hipMemcpyAsync(dR, R, (size_t)W * N * D * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "hipMemcpy");
+    if (ok) {
+        rc = prec == TDVMC_MIXED ? ensure_tables(ctx, ctx->tf, false) : ensure_tables(ctx, ctx->td, false);
+        if (rc) {
+            cleanup();
+            return rc;
+        }
+        hipEvent_t ev{};
+        prof_begin(ctx, "eval", &ev);
+        if (prec == TDVMC_MIXED)
+            ok = hipc((launch_eval<float, double>(ctx->sy, tables_of(ctx, ctx->tf), dR, W, dEx, dER, dEI, dO, dOt, dG,
+                                                  dL, nullptr, ctx->phiR, ctx->stream)), "eval kernel");
+        else
+            ok = hipc((launch_eval<double, double>(ctx->sy, tables_of(ctx, ctx->td), dR, W, dEx, dER, dEI, dO, dOt,
+                                                   dG, dL, nullptr, ctx->phiR, ctx->stream)), "eval kernel");
+        prof_end(ctx, "eval", ev);
+    }
+    auto down = [&](double* h, double* d, size_t n) {
+        if (ok && h) ok = hipc(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream), "hipMemcpy");
+    };
+    down(expo, dEx, (size_t)W);
+    down(ER, dER, (size_t)W);
+    down(EI, dEI, (size_t)W);
+    down(O, dO, (size_t)W * P);
+    down(other, dOt, (size_t)W * 9);
+    if (pp) {
+        down(grad, dG, (size_t)W * N * D * 2);
+        down(lap, dL, (size_t)W * N * 2);
+    }
+    if (ok) ok = hipc(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    cleanup();
+    if (!ok) return TDVMC_ERR_HIP;
+    return prof_flush(ctx);
+}
+
+int tdvmc_moments(tdvmc_ctx* ctx, const double* O, const double* ER, const double* EI, int64_t W, int P,
+                  int precision, double* mean, double* S, double* F)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(O && ER && EI && W >= 1 && P >= 1, "bad moments arguments");
+    int prec, rc = resolve_precision(ctx, precision, &prec);
+    if (rc) return rc;
+    double *dO = nullptr, *dE = nullptr, *dM = nullptr, *dS = nullptr, *dF = nullptr, *dPart = nullptr, *dCol = nullptr;
+    void* dXt = nullptr;
+    size_t xb, pb, cb;
+    moments_work_bytes(W, P, prec == TDVMC_MIXED ? 1 : 0, &xb, &pb, &cb);
+    hipError_t e = hipSuccess;
+    auto step = [&](hipError_t x) {
+        if (e == hipSuccess) e = x;
+    };
+    step(dalloc(&dO, (size_t)W * P));
+    step(dalloc(&dE, 2 * (size_t)W));
+    step(dalloc(&dM, (size_t)P + 2));
+    step(dalloc(&dS, (size_t)P * P));
+    step(dalloc(&dF, 2 * (size_t)P));
+    step(dalloc(&dPart, (pb + 7) / 8));
+    step(dalloc(&dCol, (cb + 7) / 8));
+    if (e == hipSuccess) e = hipMalloc(&dXt, xb);
+    step(hipMemcpyAsync(dO, O, (size_t)W * P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    step(hipMemcpyAsync(dE, ER, (size_t)W * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    step(hipMemcpyAsync(dE + W, EI, (size_t)W * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if (e == hipSuccess) {
+        MomentsWork wk{dXt, dPart, dCol, xb, pb, cb};
+        e = launch_moments<double>(dO, dE, dE + W, W, P, prec == TDVMC_MIXED ? 1 : 0, wk, dM, dS, dF, ctx->stream);
+    }
+    if (mean) step(hipMemcpyAsync(mean, dM, ((size_t)P + 2) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (S) step(hipMemcpyAsync(S, dS, (size_t)P * P * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (F) step(hipMemcpyAsync(F, dF, 2 * (size_t)P * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    step(hipStreamSynchronize(ctx->stream));
+    dfree(dO); dfree(dE); dfree(dM); dfree(dS); dfree(dF); dfree(dPart); dfree(dCol);
+    if (dXt) hipFree(dXt);
+    if (e != hipSuccess) {
+        ctx->err = std::string("moments: ") + hipGetErrorString(e);
+        return TDVMC_ERR_HIP;
+    }
+    return TDVMC_OK;
+}
+
+int tdvmc_solve(tdvmc_ctx* ctx, const double* mean, const double* S, const double* F, int P, int imag_time,
+                int precondition, double eps, double* xdot, int* n_bad)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(mean && S && F && xdot && P >= 1, "bad solve arguments");
+    double *dM = nullptr, *dS = nullptr, *dF = nullptr, *dW = nullptr, *dX = nullptr;
+    int* dSt = nullptr;
+    hipError_t e = hipSuccess;
+    auto step = [&](hipError_t x) {
+        if (e == hipSuccess) e = x;
+    };
+    step(dalloc(&dM, (size_t)P + 2));
+    step(dalloc(&dS, (size_t)P * P));
+    step(dalloc(&dF, 2 * (size_t)P));
+    step(dalloc(&dW, (size_t)P * P + 4 * (size_t)P));
+    step(dalloc(&dX, 2 * (size_t)P + 2));
+    step(dalloc(&dSt, 1));
+    step(hipMemcpyAsync(dM, mean, ((size_t)P + 2) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    step(hipMemcpyAsync(dS, S, (size_t)P * P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    step(hipMemcpyAsync(dF, F, 2 * (size_t)P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    int bad = 0;
+    if (e == hipSuccess) {
+        SolveArgs a{P, imag_time, precondition, eps, dM, dS, dF, dW, dX, dSt};
+        e = launch_solve(a, ctx->stream);
+    }
+    step(hipMemcpyAsync(xdot, dX, (2 * (size_t)P + 2) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    step(hipMemcpyAsync(&bad, dSt, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    step(hipStreamSynchronize(ctx->stream));
+    dfree(dM); dfree(dS); dfree(dF); dfree(dW); dfree(dX); dfree(dSt);
+    if (e != hipSuccess) {
+        ctx->err = std::string("solve: ") + hipGetErrorString(e);
+        return TDVMC_ERR_HIP;
+    }
+    if (n_bad) *n_bad = bad;
+    if (bad > 0) {
+        ctx->err = "non-positive pivot in the Cholesky factorisation";
+        return TDVMC_ERR_NOT_PD;
+    }
+    return TDVMC_OK;
+}
+
+int tdvmc_tdvp_step(tdvmc_ctx* ctx, double dt, int ode_type, int precision, int use_graph)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    int prec, rc = resolve_precision(ctx, precision, &prec);
+    if (rc) return rc;
+    const int ode = ode_type < 0 ? ctx->ode_type : ode_type;
+    CHECK_ARG(ode == 0 || ode == 2, "ODE type must be 0 (Euler) or 2 (RK4)");
+    if (use_graph) {
+        char key[256];
+        std::snprintf(key, sizeof(key), "%d/%d/%.17g/%lld/%.17g/%.17g", prec, ode, dt, (long long)ctx->W,
+                      ctx->sy.pot_a, ctx->sy.pot_b);
+        if (!ctx->graph || ctx->graph_key != key) {
+            if (ctx->graph) hipGraphExecDestroy(ctx->graph);
+            if (ctx->graph_src) hipGraphDestroy(ctx->graph_src);
+            ctx->graph = nullptr;
+            ctx->graph_src = nullptr;
+            const bool prof = ctx->prof;
+            ctx->prof = false;
+            HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_tdvp_step(ctx, prec, ode, dt);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            ctx->prof = prof;
+            if (rc) {
+                if (g) hipGraphDestroy(g);
+                return rc;
+            }
+            HIPC(e);
+            ctx->graph_src = g;
+            HIPC(hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0));
+            ctx->graph_key = key;
+        }
+        HIPC(hipGraphLaunch(ctx->graph, ctx->stream));
+        ctx->trials_total += (ode == 2 ? 4 : 1) * (ctx->init_moves + ctx->spw * ctx->therm_moves) * ctx->W;
+    } else {
+        rc = enqueue_tdvp_step(ctx, prec, ode, dt);
+        if (rc) return rc;
+    }
+    ctx->tf.dirty = ctx->td.dirty = true;
+    ctx->host_params_stale = true;
+    ctx->time += dt;
+    ctx->step += 1;
+    resolve_potential(ctx);
+    return prof_flush(ctx);
+}
+
+int tdvmc_adjoint(tdvmc_ctx* ctx, const double* R, int64_t W, const double* gbar, int precision, double* dR,
+                  double* dalpha)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(R && gbar && W >= 1, "bad adjoint arguments");
+    int prec, rc = resolve_precision(ctx, precision, &prec);
+    if (rc) return rc;
+    const int N = ctx->sy.N, D = ctx->sy.D, P = P_of(ctx);
+    double *dRin = nullptr, *dEx = nullptr, *dER = nullptr, *dEI = nullptr, *dO = nullptr, *dG = nullptr,
+           *dL = nullptr, *dGb = nullptr, *dOutR = nullptr, *dOutA = nullptr, *dOuter = nullptr;
+    hipError_t e = hipSuccess;
+    auto step = [&](hipError_t x) {
+        if (e == hipSuccess) e = x;
+    };
+    step(dalloc(&dRin, (size_t)W * N * D));
+    step(dalloc(&dEx, (size_t)W));
+    step(dalloc(&dER, (size_t)W));
+    step(dalloc(&dEI, (size_t)W));
+    step(dalloc(&dO, (size_t)W * P));
+    step(dalloc(&dG, (size_t)W * N * D * 2));
+    step(dalloc(&dL, (size_t)W * N * 2));
+    step(dalloc(&dGb, (size_t)W));
+    step(dalloc(&dOutR, (size_t)W * N * D));
+    step(dalloc(&dOutA, (size_t)P));
+    step(dalloc(&dOuter, (size_t)W));
+    step(hipMemcpyAsync(dRin, R, (size_t)W * N * D * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    step(hipMemcpyAsync(dGb, gbar, (size_t)W * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if (e == hipSuccess) {
+        rc = prec == TDVMC_MIXED ? ensure_tables(ctx, ctx->tf, false) : ensure_tables(ctx, ctx->td, false);
+        if (rc == TDVMC_OK) {
+            if (prec == TDVMC_MIXED)
+                e = launch_eval<float, double>(ctx->sy, tables_of(ctx, ctx->tf), dRin, W, dEx, dER, dEI, dO, nullptr,
+                                               dG, dL, dOuter, ctx->phiR, ctx->stream);
+            else
+                e = launch_eval<double, double>(ctx->sy, tables_of(ctx, ctx->td), dRin, W, dEx, dER, dEI, dO, nullptr,
+                                                dG, dL, dOuter, ctx->phiR, ctx->stream);
+        }
+    }
+    if (e == hipSuccess && rc == TDVMC_OK) {
+        e = launch_adjoint_scale(dG, dGb, W, N * D, dOutR, ctx->stream);
+        if (e == hipSuccess) e = launch_adjoint_reduce(dO, dGb, W, P, dOutA, ctx->stream);
+        if (e == hipSuccess) e = launch_adjoint_outer(dOuter, dGb, W, ctx->sy.outer_idx, dOutA, ctx->stream);
+    }
+    if (dR) step(hipMemcpyAsync(dR, dOutR, (size_t)W * N * D * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (dalpha) step(hipMemcpyAsync(dalpha, dOutA, (size_t)P * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    step(hipStreamSynchronize(ctx->stream));
+    dfree(dRin); dfree(dEx); dfree(dER); dfree(dEI); dfree(dO); dfree(dG); dfree(dL); dfree(dGb); dfree(dOutR);
+    dfree(dOutA);
+    dfree(dOuter);
+    if (rc) return rc;
+    if (e != hipSuccess) {
+        ctx->err = std::string("adjoint: ") + hipGetErrorString(e);
+        return TDVMC_ERR_HIP;
+    }
+    return TDVMC_OK;
+}
+
+int tdvmc_profile_enable(tdvmc_ctx* ctx, int enable)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    int rc = prof_flush(ctx);
+    if (rc) return rc;
+    ctx->prof = enable != 0;
+    ctx->prof_acc.clear();
+    return TDVMC_OK;
+}
+
+int tdvmc_profile_get(tdvmc_ctx* ctx, const char* name, double* total_ms, int64_t* launches)
+{
+    if (!ctx || !name) return TDVMC_ERR_BAD_ARG;
+    int rc = prof_flush(ctx);
+    if (rc) return rc;
+    auto it = ctx->prof_acc.find(name);
+    if (total_ms) *total_ms = it == ctx->prof_acc.end() ? 0.0 : it->second.ms;
+    if (launches) *launches = it == ctx->prof_acc.end() ? 0 : it->second.launches;
+    return TDVMC_OK;
+}
+
+int tdvmc_set_sweep_lanes(tdvmc_ctx* ctx, int lanes_per_walker)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(lanes_per_walker == 0 || lanes_per_walker == 16 || lanes_per_walker == 32 || lanes_per_walker == 64,
+              "lanes per walker must be 0, 16, 32 or 64");
+    if (lanes_per_walker) {
+        const int kpl = (ctx->sy.K + lanes_per_walker - 1) / lanes_per_walker;
+        CHECK_ARG(kpl <= 8, "too many k-vectors for this lane count");
+    }
+    ctx->lanes = lanes_per_walker;
+    return TDVMC_OK;
+}
+
+int tdvmc_synchronize(tdvmc_ctx* ctx)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    HIPC(hipStreamSynchronize(ctx->stream));
+    return prof_flush(ctx);
+}
+
+}  // extern "C"

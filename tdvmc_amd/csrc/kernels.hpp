@@ -1,0 +1,88 @@
+// Host-side launchers of the TDVMC kernels (all enqueue on the given stream).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace tdvmc {
+
+// ---- parameter-derived tables (kernels_misc.hip)
+template <typename T>
+hipError_t launch_prep_tables(const KSys& sy, const double* uR, const double* uI, T* spl, T* alpha, hipStream_t st);
+
+// ---- evaluation (kernels_eval.hip); grad/lap non-null selects the per-particle variant
+template <typename T>
+size_t eval_lds_bytes(const KSys& sy, bool per_particle);
+template <typename T, typename TO>
+hipError_t launch_eval(const KSys& sy, const KTables<T>& tb, const double* R, int64_t W, double* expo, double* ER,
+                       double* EI, TO* O, double* other, double* grad, double* lap, double* outer, double phiR,
+                       hipStream_t st);
+
+// ---- Metropolis sweeps (kernels_sweep.hip)
+struct SweepArgs {
+    double* R;                 // [W][N][D] in/out
+    int64_t W;
+    int64_t n_moves;           // single-particle trial moves per walker
+    uint64_t seed;
+    const uint32_t* epoch;     // device counter selecting the random stream of this call
+    unsigned long long* acc;   // [W] accepted moves (accumulated)
+    int lanes_per_walker;      // 16, 32 or 64 (0 = choose)
+};
+template <typename T>
+hipError_t launch_sweep(const KSys& sy, const KTables<T>& tb, const SweepArgs& a, hipStream_t st);
+template <typename T>
+size_t sweep_lds_bytes(const KSys& sy, int lanes_per_walker);
+
+// ---- moments (kernels_moments.hip)
+// O [W][P] (float or double), E_R, E_I [W] (double) ->
+//   mean[P+2] = (<O>, <E_R>, <E_I>), S [P][P] = cov(O, O), F [2][P] = cov(O, E_R), cov(O, E_I)
+// mode 0: FP64 (MFMA f64); mode 1: bf16x3 split products on MFMA with FP32
+// accumulation over 256-walker slices and FP64 reduction across slices.
+struct MomentsWork {
+    void* xt;                  // packed centred operands (size from moments_work_bytes)
+    double* partial;           // per-slice partial tiles
+    double* colsum;            // per-block column sums
+    size_t xt_bytes, partial_bytes, colsum_bytes;
+};
+void moments_work_bytes(int64_t W, int P, int mode, size_t* xt, size_t* partial, size_t* colsum);
+template <typename TO>
+hipError_t launch_moments(const TO* O, const double* ER, const double* EI, int64_t W, int P, int mode,
+                          const MomentsWork& wk, double* mean, double* S, double* F, hipStream_t st);
+
+// ---- TDVP linear system, Cholesky and parameter update (kernels_solve.hip)
+struct SolveArgs {
+    int P;
+    int imag_time;
+    int precondition;
+    double eps;
+    const double* mean;        // [P+2]
+    const double* S;           // [P][P]
+    const double* F;           // [2][P]
+    double* work;              // [P][P] scratch (factor), + 4P
+    double* xdot;              // [2P+2] out: alpha-dot R, alpha-dot I, phi-dot R, phi-dot I
+    int* status;               // [1] count of non-positive pivots
+};
+hipError_t launch_solve(const SolveArgs& a, hipStream_t st);
+
+// y = x0 + c * xdot  over the 2P+2 entries (alpha R, alpha I, phi R, phi I)
+hipError_t launch_axpy_params(int P, const double* x0, const double* xdot, double c, double* y, hipStream_t st);
+// x += dt/6 (k1 + 2 k2 + 2 k3 + k4)
+hipError_t launch_rk4_combine(int P, double* x, const double* k1, const double* k2, const double* k3,
+                              const double* k4, double dt, hipStream_t st);
+// params (2P+2 packed) -> uR, uI arrays used by the tables
+hipError_t launch_unpack_params(int P, const double* packed, double* uR, double* uI, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
+hipError_t launch_add_u32(uint32_t* p, uint32_t v, hipStream_t st);
+
+// ---- adjoint (kernels_misc.hip): dalpha[P] = sum_w gbar_w O_w ; dR = gbar_w grad_R ln psi
+hipError_t launch_adjoint_reduce(const double* O, const double* gbar, int64_t W, int P, double* dalpha,
+                                 hipStream_t st);
+hipError_t launch_adjoint_scale(const double* grad, const double* gbar, int64_t W, int ND, double* dR,
+                                hipStream_t st);
+// dalpha[idx] += sum_w gbar_w outer_w (the outer-pair term of the last pair parameter)
+hipError_t launch_adjoint_outer(const double* outer, const double* gbar, int64_t W, int idx, double* dalpha,
+                                hipStream_t st);
+
+}  // namespace tdvmc

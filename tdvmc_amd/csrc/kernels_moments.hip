@@ -1,0 +1,320 @@
+// TDVP moments of one sampling stage:
+//   <O>, <E_R>, <E_I>, S = <O O^T> - <O><O>^T, F = <O E> - <O><E>
+// (the sums of src/TDVMC.cpp:1103-1109 combined as in :1506-1537).
+//
+// The walker-by-parameter matrix X = [O | E_R | E_I] (W x Pa, Pa = P + 2) is
+// centred column-wise (two passes: column means in FP64, then centring while
+// packing), and S_aug = Xc^T Xc / W is a dense contraction over walkers:
+//   mode 0 (FP64):  v_mfma_f64_16x16x4 on FP64 operands;
+//   mode 1 (mixed): every centred entry is split into two bf16 terms
+//                   x = hi + lo (16 significant bits) and each product is
+//                   formed from the split terms on v_mfma_f32_32x32x16_bf16:
+//                   hi*hi + hi*lo + lo*hi (the "bf16x3" products) plus the
+//                   lo*lo term -- without it the diagonal of S carries a
+//                   one-signed bias of ~(2^-8)^2/3 relative, above the 1e-6
+//                   target; FP32 accumulation is folded into FP64 registers
+//                   every 256 walkers, and the walker slices are reduced in
+//                   FP64.
+// Only tiles on or above the diagonal are computed; F is column Pa-2 / Pa-1.
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace tdvmc {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double double4v __attribute__((ext_vector_type(4)));
+
+constexpr int kColBlock = 256;
+constexpr int kMaxChunks = 256;
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+struct Geometry {
+    int P, Pa, tile, Pp, nb, ntiles, groups;
+    int64_t W, Wp, ksteps;   // Wp: padded walkers; ksteps: MFMA k-steps over Wp
+    int chunks;
+};
+
+Geometry geometry(int64_t W, int P, int mode)
+{
+    Geometry g;
+    g.P = P;
+    g.Pa = P + 2;
+    g.tile = mode == 0 ? 16 : 32;
+    g.Pp = (int)ceil_div(g.Pa, g.tile) * g.tile;
+    g.nb = g.Pp / g.tile;
+    g.ntiles = g.nb * (g.nb + 1) / 2;
+    const int kstep = mode == 0 ? 4 : 16;
+    const int fold = 256;                               // walkers per FP32 fold (mode 1)
+    g.W = W;
+    g.Wp = ceil_div(W, fold) * fold;
+    g.ksteps = g.Wp / kstep;
+    int64_t groups = g.Wp / fold;
+    const int64_t cap = mode == 0 ? 64 : 32;
+    if (groups > cap) groups = cap;
+    if (groups < 1) groups = 1;
+    g.groups = (int)groups;
+    int64_t ch = ceil_div(W, 256);
+    g.chunks = (int)(ch > kMaxChunks ? kMaxChunks : (ch < 1 ? 1 : ch));
+    return g;
+}
+
+// ---- pass 1: column sums of X in FP64, per row chunk
+template <typename TO>
+__global__ void colsum_kernel(const TO* __restrict__ O, const double* __restrict__ ER, const double* __restrict__ EI,
+                              int64_t W, int P, int chunks, double* __restrict__ part)
+{
+    const int Pa = P + 2;
+    const int c = blockIdx.x * kColBlock + threadIdx.x;
+    const int ch = blockIdx.y;
+    const int64_t rows = (W + chunks - 1) / chunks;
+    const int64_t w0 = ch * rows, w1 = min(W, w0 + rows);
+    if (c >= Pa) return;
+    double s = 0.0;
+    if (c < P)
+        for (int64_t w = w0; w < w1; w++) s += (double)O[w * P + c];
+    else if (c == P)
+        for (int64_t w = w0; w < w1; w++) s += ER[w];
+    else
+        for (int64_t w = w0; w < w1; w++) s += EI[w];
+    part[(size_t)ch * Pa + c] = s;
+}
+
+__global__ void colmean_kernel(const double* __restrict__ part, int chunks, int Pa, int64_t W, double* __restrict__ mean)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= Pa) return;
+    double s = 0.0;
+    for (int ch = 0; ch < chunks; ch++) s += part[(size_t)ch * Pa + c];
+    mean[c] = s / (double)W;
+}
+
+__device__ __forceinline__ double xval(const float* O, const double* ER, const double* EI, int64_t W, int P,
+                                       const double* mean, int64_t w, int c)
+{
+    if (w >= W || c >= P + 2) return 0.0;
+    const double v = c < P ? (double)O[w * P + c] : (c == P ? ER[w] : EI[w]);
+    return v - mean[c];
+}
+__device__ __forceinline__ double xval(const double* O, const double* ER, const double* EI, int64_t W, int P,
+                                       const double* mean, int64_t w, int c)
+{
+    if (w >= W || c >= P + 2) return 0.0;
+    const double v = c < P ? O[w * P + c] : (c == P ? ER[w] : EI[w]);
+    return v - mean[c];
+}
+
+// ---- pass 2a (mode 0): pack centred FP64 operands as [cb][s][kk][pp]
+//      column c = 16 cb + pp, walker w = 4 s + kk
+template <typename TO>
+__global__ void pack64_kernel(const TO* __restrict__ O, const double* __restrict__ ER, const double* __restrict__ EI,
+                              int64_t W, int P, const double* __restrict__ mean, int64_t Wp, int Pp,
+                              double* __restrict__ xt)
+{
+    const int64_t total = Wp * Pp;
+    const int64_t S4 = Wp / 4;
+    for (int64_t idx = threadIdx.x + (int64_t)blockIdx.x * blockDim.x; idx < total;
+         idx += (int64_t)blockDim.x * gridDim.x) {
+        // idx enumerates the packed layout; recover (w, c)
+        const int pp = (int)(idx & 15);
+        const int kk = (int)((idx >> 4) & 3);
+        const int64_t rest = idx >> 6;
+        const int64_t s = rest % S4;
+        const int cb = (int)(rest / S4);
+        const int64_t w = 4 * s + kk;
+        const int c = 16 * cb + pp;
+        xt[idx] = xval(O, ER, EI, W, P, mean, w, c);
+    }
+}
+
+// ---- pass 2b (mode 1): pack centred operands split into bf16 hi / lo as
+//      [cb][s][part][h][l32][j], column c = 32 cb + l32, walker w = 16 s + 8 h + j
+template <typename TO>
+__global__ void pack_bf16_kernel(const TO* __restrict__ O, const double* __restrict__ ER, const double* __restrict__ EI,
+                                 int64_t W, int P, const double* __restrict__ mean, int64_t Wp, int Pp,
+                                 __bf16* __restrict__ xt)
+{
+    // one thread per (cb, s, h, l32): 8 walkers of one column
+    const int64_t S16 = Wp / 16;
+    const int64_t total = (int64_t)(Pp / 32) * S16 * 64;
+    for (int64_t idx = threadIdx.x + (int64_t)blockIdx.x * blockDim.x; idx < total;
+         idx += (int64_t)blockDim.x * gridDim.x) {
+        const int l32 = (int)(idx & 31);
+        const int h = (int)((idx >> 5) & 1);
+        const int64_t rest = idx >> 6;
+        const int64_t s = rest % S16;
+        const int cb = (int)(rest / S16);
+        const int c = 32 * cb + l32;
+        __bf16* base = xt + ((((size_t)cb * S16 + s) * 2) * 64) * 8;
+        __bf16* hi = base + ((size_t)h * 32 + l32) * 8;
+        __bf16* lo = base + ((size_t)64 + h * 32 + l32) * 8;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int64_t w = 16 * s + 8 * h + j;
+            const float x = (float)xval(O, ER, EI, W, P, mean, w, c);
+            const __bf16 xh = (__bf16)x;
+            const __bf16 xl = (__bf16)(x - (float)xh);
+            hi[j] = xh;
+            lo[j] = xl;
+        }
+    }
+}
+
+__device__ __forceinline__ void tile_of(int t, int nb, int* I, int* J)
+{
+    // t enumerates the upper triangle row by row
+    int i = 0;
+    while (t >= nb - i) {
+        t -= nb - i;
+        i++;
+    }
+    *I = i;
+    *J = i + t;
+}
+
+// ---- pass 3a (mode 0): FP64 MFMA over walker slices, one wave per (tile, group)
+__global__ __launch_bounds__(64) void gemm64_kernel(const double* __restrict__ xt, int64_t ksteps, int nb, int groups,
+                                                    double* __restrict__ partial)
+{
+    const int t = blockIdx.x, g = blockIdx.y, lane = threadIdx.x;
+    int I, J;
+    tile_of(t, nb, &I, &J);
+    const int64_t per = (ksteps + groups - 1) / groups;
+    const int64_t s0 = g * per, s1 = min(ksteps, s0 + per);
+    const double* A = xt + (size_t)I * ksteps * 64;
+    const double* B = xt + (size_t)J * ksteps * 64;
+    double4v acc = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t s = s0; s < s1; s++) {
+        const double a = A[s * 64 + lane];
+        const double b = B[s * 64 + lane];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    // D[i][j]: j = lane % 16, i = 4 v + lane / 16
+    double* out = partial + (((size_t)g * gridDim.x + t) * 256);
+#pragma unroll
+    for (int v = 0; v < 4; v++) out[(4 * v + lane / 16) * 16 + (lane % 16)] = acc[v];
+}
+
+// ---- pass 3b (mode 1): split-bf16 MFMA (hi*hi + hi*lo + lo*hi + lo*lo)
+__global__ __launch_bounds__(64) void gemm_bf16x3_kernel(const __bf16* __restrict__ xt, int64_t ksteps, int nb,
+                                                         int groups, double* __restrict__ partial)
+{
+    const int t = blockIdx.x, g = blockIdx.y, lane = threadIdx.x;
+    int I, J;
+    tile_of(t, nb, &I, &J);
+    // per k-step block: 2 parts x 64 lanes x 8 bf16
+    const size_t step_elems = 2 * 64 * 8;
+    const bf16x8* A = reinterpret_cast<const bf16x8*>(xt + (size_t)I * ksteps * step_elems);
+    const bf16x8* B = reinterpret_cast<const bf16x8*>(xt + (size_t)J * ksteps * step_elems);
+    // groups split the k-steps in whole folds of 16 steps (256 walkers)
+    const int64_t folds = ksteps / 16;
+    const int64_t per = (folds + groups - 1) / groups;
+    const int64_t f0 = g * per, f1 = min(folds, f0 + per);
+    double acc64[16];
+#pragma unroll
+    for (int v = 0; v < 16; v++) acc64[v] = 0.0;
+    for (int64_t f = f0; f < f1; f++) {
+        floatx16 acc;
+#pragma unroll
+        for (int v = 0; v < 16; v++) acc[v] = 0.0f;
+        for (int64_t s = f * 16; s < f * 16 + 16; s++) {
+            const bf16x8 ah = A[s * 128 + lane], al = A[s * 128 + 64 + lane];
+            const bf16x8 bh = B[s * 128 + lane], bl = B[s * 128 + 64 + lane];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bl, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 16; v++) acc64[v] += (double)acc[v];
+    }
+    // D[i][j]: j = lane % 32, i = 8 (v / 4) + 4 (lane / 32) + (v % 4)
+    double* out = partial + (((size_t)g * gridDim.x + t) * 1024);
+#pragma unroll
+    for (int v = 0; v < 16; v++) out[(8 * (v / 4) + 4 * (lane / 32) + (v % 4)) * 32 + (lane % 32)] = acc64[v];
+}
+
+// ---- pass 4: sum slices, scale 1/W, scatter to S (both triangles) and F
+__global__ void reduce_tiles_kernel(const double* __restrict__ partial, int ntiles, int nb, int tile, int groups,
+                                    int P, int64_t W, double* __restrict__ S, double* __restrict__ F)
+{
+    const int t = blockIdx.x;
+    int I, J;
+    tile_of(t, nb, &I, &J);
+    const int tt = tile * tile;
+    const double inv = 1.0 / (double)W;
+    for (int e = threadIdx.x; e < tt; e += blockDim.x) {
+        // diagonal tiles: take the upper element for both triangles
+        if (I == J && e / tile > e % tile) continue;
+        double s = 0.0;
+        for (int g = 0; g < groups; g++) s += partial[((size_t)g * ntiles + t) * tt + e];
+        s *= inv;
+        const int i = I * tile + e / tile, j = J * tile + e % tile;
+        if (i < P && j < P) {
+            S[(size_t)i * P + j] = s;
+            S[(size_t)j * P + i] = s;
+        } else if (i < P && (j == P || j == P + 1)) {
+            F[(size_t)(j - P) * P + i] = s;
+        }
+    }
+}
+
+}  // namespace
+
+void moments_work_bytes(int64_t W, int P, int mode, size_t* xt, size_t* partial, size_t* colsum)
+{
+    Geometry g = geometry(W, P, mode);
+    if (mode == 0)
+        *xt = (size_t)g.Wp * g.Pp * sizeof(double);
+    else
+        *xt = (size_t)g.Wp * g.Pp * 2 * sizeof(__bf16);
+    *partial = (size_t)g.groups * g.ntiles * g.tile * g.tile * sizeof(double);
+    *colsum = (size_t)g.chunks * g.Pa * sizeof(double);
+}
+
+template <typename TO>
+hipError_t launch_moments(const TO* O, const double* ER, const double* EI, int64_t W, int P, int mode,
+                          const MomentsWork& wk, double* mean, double* S, double* F, hipStream_t st)
+{
+    if (W <= 0) return hipErrorInvalidValue;
+    Geometry g = geometry(W, P, mode);
+    size_t xb, pb, cb;
+    moments_work_bytes(W, P, mode, &xb, &pb, &cb);
+    if (wk.xt_bytes < xb || wk.partial_bytes < pb || wk.colsum_bytes < cb) return hipErrorInvalidValue;
+
+    hipLaunchKernelGGL(colsum_kernel<TO>, dim3((unsigned)ceil_div(g.Pa, kColBlock), g.chunks), dim3(kColBlock), 0, st,
+                       O, ER, EI, W, P, g.chunks, wk.colsum);
+    hipLaunchKernelGGL(colmean_kernel, dim3((unsigned)ceil_div(g.Pa, 256)), dim3(256), 0, st, wk.colsum, g.chunks,
+                       g.Pa, W, mean);
+    if (mode == 0) {
+        const int64_t total = g.Wp * g.Pp;
+        int64_t blocks = ceil_div(total, 256);
+        if (blocks > 16384) blocks = 16384;
+        hipLaunchKernelGGL(pack64_kernel<TO>, dim3((unsigned)blocks), dim3(256), 0, st, O, ER, EI, W, P, mean, g.Wp,
+                           g.Pp, static_cast<double*>(wk.xt));
+        hipLaunchKernelGGL(gemm64_kernel, dim3(g.ntiles, g.groups), dim3(64), 0, st, static_cast<const double*>(wk.xt),
+                           g.ksteps, g.nb, g.groups, wk.partial);
+    } else {
+        const int64_t total = (int64_t)(g.Pp / 32) * (g.Wp / 16) * 64;
+        int64_t blocks = ceil_div(total, 256);
+        if (blocks > 16384) blocks = 16384;
+        hipLaunchKernelGGL(pack_bf16_kernel<TO>, dim3((unsigned)blocks), dim3(256), 0, st, O, ER, EI, W, P, mean,
+                           g.Wp, g.Pp, static_cast<__bf16*>(wk.xt));
+        hipLaunchKernelGGL(gemm_bf16x3_kernel, dim3(g.ntiles, g.groups), dim3(64), 0, st,
+                           static_cast<const __bf16*>(wk.xt), g.ksteps, g.nb, g.groups, wk.partial);
+    }
+    hipLaunchKernelGGL(reduce_tiles_kernel, dim3(g.ntiles), dim3(256), 0, st, wk.partial, g.ntiles, g.nb, g.tile,
+                       g.groups, P, W, S, F);
+    return hipGetLastError();
+}
+
+template hipError_t launch_moments<float>(const float*, const double*, const double*, int64_t, int, int,
+                                          const MomentsWork&, double*, double*, double*, hipStream_t);
+template hipError_t launch_moments<double>(const double*, const double*, const double*, int64_t, int, int,
+                                           const MomentsWork&, double*, double*, double*, hipStream_t);
+
+}  // namespace tdvmc

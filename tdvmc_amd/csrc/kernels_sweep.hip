@@ -1,0 +1,319 @@
+// Persistent Metropolis sweeps: every walker performs n_moves single-particle
+// trial moves inside one launch (a sweep is N moves).
+//
+// Restates the move of src/TDVMC.cpp:858-916 (DoMetropolisStep: random
+// particle, Gaussian displacement of width MC_STEP in every component, accept
+// with probability min(1, |psi_new / psi_old|^2), restore on reject) with the
+// incremental ratio of src/PhysicalSystems/BosonsBulk.cpp:553-657
+// (CalculateWFChange: old and new pair terms of the moved particle, pairs
+// beyond the cutoff contribute the last pair parameter), plus the k-space
+// term (incremental rho_k update) and the one-body factor.
+//
+// Mapping: a walker is a group of LPW lanes (LPW = 16, 32 or 64) inside a
+// wavefront; a 256-thread workgroup holds 256/LPW walkers.  The walker's
+// coordinates live in LDS (structure of arrays) for the whole launch; lane g of the group handles
+// partners i = g, g + LPW, ... and k-vectors k = g + m LPW (their rho_k stay
+// in that lane's registers).  The change of ln|psi| is summed with an xor
+// butterfly (bitwise identical in every lane), so all lanes take the same
+// accept decision without further communication.  Random numbers come from
+// Philox4x32-10 (counter = trial, call epoch, walker): each lane draws the
+// numbers of one future trial, and trial t of a batch reads them from lane t.
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace tdvmc {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+template <typename T>
+struct Vec4 { T x, y, z, w; };
+
+template <typename T, int D, int LPW, int KPL>
+__global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb, SweepArgs a)
+{
+    constexpr int WPB = kThreads / LPW;   // walkers per block
+    const int N = sy.N, Ps = sy.P_s, K = sy.K;
+    const int tid = threadIdx.x;
+    const int grp = tid / LPW, gl = tid % LPW;
+    const int64_t w = (int64_t)blockIdx.x * WPB + grp;
+
+    extern __shared__ __align__(16) unsigned char smem[];
+    Vec4<T>* spl = reinterpret_cast<Vec4<T>*>(smem);                      // (Ps-1) x (A0..A3) of Re u
+    Vec4<T>* kvl = spl + (Ps - 1);                                         // K x (n/L, |k|^2)
+    T* all = reinterpret_cast<T*>(kvl + K);                                // K x Re alpha
+    T* posb = all + ((K + 3) & ~3);                                        // WPB x D x N (structure of arrays)
+    T* pos = posb + (size_t)grp * N * D;
+
+    for (int i = tid; i < Ps - 1; i += kThreads) {
+        spl[i].x = tb.spl[i * 8 + 0];
+        spl[i].y = tb.spl[i * 8 + 1];
+        spl[i].z = tb.spl[i * 8 + 2];
+        spl[i].w = tb.spl[i * 8 + 3];
+    }
+    for (int i = tid; i < K; i += kThreads) {
+        kvl[i].x = tb.kv[i * 4 + 0];
+        kvl[i].y = tb.kv[i * 4 + 1];
+        kvl[i].z = tb.kv[i * 4 + 2];
+        kvl[i].w = tb.kv[i * 4 + 3];
+        all[i] = tb.alpha[2 * i];
+    }
+    const bool active = w < a.W;
+    double* Rw = a.R + (active ? w : 0) * (size_t)N * D;
+    if (active) {
+        for (int i = gl; i < N; i += LPW) {
+#pragma unroll
+            for (int c = 0; c < D; c++) {
+                double x = Rw[(size_t)i * D + c];
+                if (sy.periodic) x = x - sy.L * rint(x * sy.Linv);
+                pos[c * N + i] = (T)x;
+            }
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+
+    const T L = (T)sy.L, Linv = (T)sy.Linv, hinv = (T)sy.hinv, rc = (T)sy.rc, step = (T)sy.step;
+    const T u_out = (T)tb.uR[sy.outer_idx];
+    const T ob = sy.one_body ? (T)tb.uR[0] : (T)0;
+    const int periodic = sy.periodic;
+
+    // rho_k of this lane's k-vectors
+    T rhoR[KPL > 0 ? KPL : 1], rhoI[KPL > 0 ? KPL : 1];
+#pragma unroll
+    for (int m = 0; m < KPL; m++) {
+        rhoR[m] = rhoI[m] = (T)0;
+        const int k = gl + m * LPW;
+        if (k < K) {
+            const Vec4<T> q = kvl[k];
+            for (int n = 0; n < N; n++) {
+                T ph = q.x * pos[n];
+                if (D > 1) ph += q.y * pos[N + n];
+                if (D > 2) ph += q.z * pos[2 * N + n];
+                T s, c;
+                Prec<T>::sincos_rev(ph, &s, &c);
+                rhoR[m] += c;
+                rhoI[m] += s;
+            }
+        }
+    }
+
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const uint32_t ep = a.epoch ? *a.epoch : 0u;
+    const int64_t ntr = a.n_moves;
+    unsigned long long nacc = 0;
+
+    for (int64_t base = 0; base < ntr; base += LPW) {
+        // random numbers of trial base + gl
+        const uint32_t tr = (uint32_t)(base + gl);
+        Philox4 x = philox4x32_10(tr, ep, (uint32_t)w, 0u, k0, k1);
+        int pm;
+        if (sy.move_order == 0)
+            pm = (int)__umulhi(x.v[0], (uint32_t)N);
+        else
+            pm = (int)((base + gl) % N);
+        T z[3];
+        T logu;
+        {
+            // Box-Muller on (v1, v2) -> two normals; D = 3 draws a second block
+            const float u1 = u01f(x.v[1]), u2 = u01f(x.v[2]);
+            const float rr = sqrtf(-2.0f * __logf(u1));
+            float sn, cs;
+            Prec<float>::sincos_rev(u2, &sn, &cs);
+            z[0] = (T)(rr * cs);
+            z[1] = (T)(rr * sn);
+            if (D == 3) {
+                Philox4 y = philox4x32_10(tr, ep, (uint32_t)w, 1u, k0, k1);
+                const float v1 = u01f(y.v[0]), v2 = u01f(y.v[1]);
+                const float r2 = sqrtf(-2.0f * __logf(v1));
+                float s2, c2;
+                Prec<float>::sincos_rev(v2, &s2, &c2);
+                z[2] = (T)(r2 * c2);
+            } else {
+                z[2] = (T)0;
+            }
+            logu = (T)__logf(u01f(x.v[3]));
+        }
+        const int nb = (int)((ntr - base) < LPW ? (ntr - base) : LPW);
+        for (int tt = 0; tt < nb; tt++) {
+            const int p = __shfl(pm, tt, LPW);
+            T dz[3];
+#pragma unroll
+            for (int c = 0; c < D; c++) dz[c] = __shfl(z[c], tt, LPW);
+            const T lu = __shfl(logu, tt, LPW);
+
+            T old[D], nw[D];
+#pragma unroll
+            for (int c = 0; c < D; c++) {
+                old[c] = pos[c * N + p];
+                T v = old[c] + step * dz[c];
+                if (periodic) v = v - L * Prec<T>::rint_(v * Linv);
+                nw[c] = v;
+            }
+            T delta = (T)0;
+            for (int i = gl; i < N; i += LPW) {
+                if (i == p) continue;
+                T ro2 = (T)0, rn2 = (T)0;
+#pragma unroll
+                for (int c = 0; c < D; c++) {
+                    const T pi = pos[c * N + i];
+                    const T dOld = min_image(pi - old[c], L, Linv, periodic);
+                    const T dNew = min_image(pi - nw[c], L, Linv, periodic);
+                    ro2 += dOld * dOld;
+                    rn2 += dNew * dNew;
+                }
+                const T ro = Prec<T>::sqrt_(ro2), rn = Prec<T>::sqrt_(rn2);
+                T uo = u_out, un = u_out;
+                if (ro <= rc) {
+                    T t;
+                    const int j = knot_interval(ro, hinv, Ps, &t);
+                    const Vec4<T> cf = spl[j];
+                    uo = cf.x + t * (cf.y + t * (cf.z + t * cf.w));
+                }
+                if (rn <= rc) {
+                    T t;
+                    const int j = knot_interval(rn, hinv, Ps, &t);
+                    const Vec4<T> cf = spl[j];
+                    un = cf.x + t * (cf.y + t * (cf.z + t * cf.w));
+                }
+                delta += un - uo;
+            }
+            T dR[KPL > 0 ? KPL : 1], dI[KPL > 0 ? KPL : 1];
+#pragma unroll
+            for (int m = 0; m < KPL; m++) {
+                dR[m] = dI[m] = (T)0;
+                const int k = gl + m * LPW;
+                if (k < K) {
+                    const Vec4<T> q = kvl[k];
+                    T po = q.x * old[0], pn = q.x * nw[0];
+                    if (D > 1) {
+                        po += q.y * old[1];
+                        pn += q.y * nw[1];
+                    }
+                    if (D > 2) {
+                        po += q.z * old[2];
+                        pn += q.z * nw[2];
+                    }
+                    T so, co, sn, cn;
+                    Prec<T>::sincos_rev(po, &so, &co);
+                    Prec<T>::sincos_rev(pn, &sn, &cn);
+                    dR[m] = cn - co;
+                    dI[m] = sn - so;
+                    // |rho + d|^2 - |rho|^2 = 2 Re(conj(rho) d) + |d|^2
+                    delta += all[k] * ((T)2 * (rhoR[m] * dR[m] + rhoI[m] * dI[m]) + dR[m] * dR[m] + dI[m] * dI[m]);
+                }
+            }
+            if (gl == 0 && sy.one_body) {
+                T ro2 = (T)0, rn2 = (T)0;
+#pragma unroll
+                for (int c = 0; c < D; c++) {
+                    ro2 += old[c] * old[c];
+                    rn2 += nw[c] * nw[c];
+                }
+                delta += -ob * (rn2 - ro2);
+            }
+#pragma unroll
+            for (int o = LPW / 2; o > 0; o >>= 1) delta += __shfl_xor(delta, o, LPW);
+            const bool accept = ((T)2 * delta >= lu);
+            if (accept) {
+                if (gl == 0) {
+#pragma unroll
+                    for (int c = 0; c < D; c++) pos[c * N + p] = nw[c];
+                }
+#pragma unroll
+                for (int m = 0; m < KPL; m++) {
+                    rhoR[m] += dR[m];
+                    rhoI[m] += dI[m];
+                }
+                nacc++;
+            }
+            // the moved particle's LDS slot is read by every lane of the group next trial
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+
+    for (int i = gl; i < N; i += LPW) {
+#pragma unroll
+        for (int c = 0; c < D; c++) Rw[(size_t)i * D + c] = (double)pos[c * N + i];
+    }
+    if (gl == 0 && a.acc) a.acc[w] += nacc;
+}
+
+int pick_lpw(int N, int K)
+{
+    int l = N >= 96 ? 64 : (N >= 24 ? 32 : 16);
+    while (l < 64 && K > 8 * l) l *= 2;
+    return l;
+}
+
+}  // namespace
+
+template <typename T>
+size_t sweep_lds_bytes(const KSys& sy, int lpw)
+{
+    if (lpw <= 0) lpw = pick_lpw(sy.N, sy.K);
+    const size_t wpb = kThreads / lpw;
+    return sizeof(T) * (4 * (size_t)(sy.P_s - 1) + 4 * (size_t)sy.K + (((size_t)sy.K + 3) & ~(size_t)3) +
+                        wpb * sy.N * sy.D);
+}
+
+template <typename T, int D, int LPW, int KPL>
+static hipError_t launch_sweep_k(const KSys& sy, const KTables<T>& tb, const SweepArgs& a, hipStream_t st)
+{
+    const size_t lds = sweep_lds_bytes<T>(sy, LPW);
+    auto kern = sweep_kernel<T, D, LPW, KPL>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const int wpb = kThreads / LPW;
+    const int64_t blocks = (a.W + wpb - 1) / wpb;
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kThreads), lds, st, sy, tb, a);
+    return hipGetLastError();
+}
+
+template <typename T, int D, int LPW>
+static hipError_t launch_sweep_l(const KSys& sy, const KTables<T>& tb, const SweepArgs& a, hipStream_t st)
+{
+    const int kpl = (sy.K + LPW - 1) / LPW;
+    if (kpl == 0) return launch_sweep_k<T, D, LPW, 0>(sy, tb, a, st);
+    if (kpl <= 2) return launch_sweep_k<T, D, LPW, 2>(sy, tb, a, st);
+    if (kpl <= 4) return launch_sweep_k<T, D, LPW, 4>(sy, tb, a, st);
+    if (kpl <= 8) return launch_sweep_k<T, D, LPW, 8>(sy, tb, a, st);
+    return hipErrorInvalidValue;
+}
+
+template <typename T, int D>
+static hipError_t launch_sweep_d(const KSys& sy, const KTables<T>& tb, const SweepArgs& a, hipStream_t st)
+{
+    const int lpw = a.lanes_per_walker > 0 ? a.lanes_per_walker : pick_lpw(sy.N, sy.K);
+    switch (lpw) {
+    case 16: return launch_sweep_l<T, D, 16>(sy, tb, a, st);
+    case 32: return launch_sweep_l<T, D, 32>(sy, tb, a, st);
+    case 64: return launch_sweep_l<T, D, 64>(sy, tb, a, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t launch_sweep(const KSys& sy, const KTables<T>& tb, const SweepArgs& a, hipStream_t st)
+{
+    if (a.W <= 0 || a.n_moves <= 0) return hipSuccess;
+    switch (sy.D) {
+    case 1: return launch_sweep_d<T, 1>(sy, tb, a, st);
+    case 2: return launch_sweep_d<T, 2>(sy, tb, a, st);
+    case 3: return launch_sweep_d<T, 3>(sy, tb, a, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template hipError_t launch_sweep<float>(const KSys&, const KTables<float>&, const SweepArgs&, hipStream_t);
+template hipError_t launch_sweep<double>(const KSys&, const KTables<double>&, const SweepArgs&, hipStream_t);
+template size_t sweep_lds_bytes<float>(const KSys&, int);
+template size_t sweep_lds_bytes<double>(const KSys&, int);
+
+}  // namespace tdvmc

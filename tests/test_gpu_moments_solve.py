@@ -1,0 +1,111 @@
+"""GPU parity of the TDVP moments and of the FP64 solve, through the C ABI.
+
+Moments: FP64 mode within 1e-12 (relative Frobenius) of the oracle's two-pass
+FP64 moments; split-bf16 MFMA mode within 1e-6 (relative Frobenius of S and of
+F) at 65536 samples with the statistics of a real measurement.
+Solve: within 1e-10 of the oracle's restatement of SolveForParametersDot
+(real / imaginary time, with and without diagonal scaling), and the count of
+non-positive pivots on a singular system.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+tdvmc = pytest.importorskip("tdvmc_amd")
+
+from tests.helpers import make_cfg  # noqa: E402
+
+
+def _ctx(P_s=8, K=0):
+    return tdvmc.Context(make_cfg(N=4, D=3, L=4.0, P_s=P_s, K=K))
+
+
+def _samples(W, P, seed, scale=1.0, mean=0.0):
+    rng = np.random.default_rng(seed)
+    A = rng.normal(size=(P, P)) / np.sqrt(P)
+    O = mean + scale * (rng.normal(size=(W, P)) @ A + 0.3 * rng.normal(size=(W, 1)))
+    ER = O @ rng.normal(size=P) * 0.1 + rng.normal(size=W)
+    EI = rng.normal(size=W) * 0.2 + 0.05 * O[:, 0]
+    return O, ER, EI
+
+
+def fro(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("W,P", [(1000, 10), (4097, 37), (3000, 296)])
+def test_moments_fp64(W, P):
+    O, ER, EI = _samples(W, P, seed=P, scale=3.0, mean=50.0)
+    print("moments fp64", W, P, flush=True)
+    with _ctx() as ctx:
+        m = ctx.moments(O, ER, EI, precision=tdvmc.FP64)
+    ref = o.moments(O, ER, EI)
+    assert fro(m["mO"], ref["mO"]) < 1e-13
+    assert abs(m["mE"][0] - ref["mE"][0]) < 1e-12 * max(1, abs(ref["mE"][0]))
+    assert fro(m["S"], ref["S"]) < 1e-12
+    assert fro(m["FR"], ref["FR"]) < 1e-12
+    assert fro(m["FI"], ref["FI"]) < 1e-12
+    assert np.array_equal(m["S"], m["S"].T)
+
+
+def test_moments_bf16x3_realistic():
+    # O with the statistics of a 343-particle measurement: large means, small
+    # relative fluctuations (this is where un-centred bf16 products would fail)
+    W, P = 65536, 296
+    O, ER, EI = _samples(W, P, seed=11, scale=30.0, mean=1000.0)
+    print("moments bf16x3", W, P, flush=True)
+    with _ctx() as ctx:
+        m = ctx.moments(O, ER, EI, precision=tdvmc.MIXED)
+        m64 = ctx.moments(O, ER, EI, precision=tdvmc.FP64)
+    ref = o.moments(O, ER, EI)
+    eS, eR, eI = fro(m["S"], ref["S"]), fro(m["FR"], ref["FR"]), fro(m["FI"], ref["FI"])
+    print(f"bf16x3 relative Frobenius errors: S {eS:.2e}  F_R {eR:.2e}  F_I {eI:.2e}")
+    assert eS < 1e-6 and eR < 1e-6 and eI < 1e-6
+    assert fro(m64["S"], ref["S"]) < 1e-12
+    assert fro(m["mO"], ref["mO"]) < 1e-13
+
+
+def test_moments_bf16x3_small_sample():
+    W, P = 1000, 20
+    O, ER, EI = _samples(W, P, seed=5, scale=2.0, mean=10.0)
+    with _ctx() as ctx:
+        m = ctx.moments(O, ER, EI, precision=tdvmc.MIXED)
+    ref = o.moments(O, ER, EI)
+    e = fro(m["S"], ref["S"])
+    print(f"bf16x3 W=1000 relative Frobenius error of S: {e:.2e}")
+    assert e < 1e-5
+
+
+@pytest.mark.parametrize("imag,pre", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("P", [12, 296])
+def test_solve_vs_oracle(imag, pre, P):
+    rng = np.random.default_rng(P + 10 * imag + pre)
+    A = rng.normal(size=(P, P))
+    S = A @ A.T / P + 0.01 * np.eye(P)
+    m = dict(mO=rng.normal(size=P), mE=np.array([1.3, -0.4]), S=S, FR=rng.normal(size=P), FI=rng.normal(size=P))
+    print("solve", P, imag, pre, flush=True)
+    with _ctx() as ctx:
+        r = ctx.solve(m["mO"], m["mE"], S, m["FR"], m["FI"], imag_time=imag, precondition=pre, eps=1e-3)
+    ref = o.solve_param_dot(m, imag_time=imag, precondition=pre, eps=1e-3)
+    assert r["bad"] == 0 and ref["bad"] == 0
+    assert fro(r["xR"], ref["xR"]) < 1e-10
+    assert fro(r["xI"], ref["xI"]) < 1e-10
+    assert abs(r["phiDotR"] - ref["phiDotR"]) < 1e-10 * max(1, abs(ref["phiDotR"]))
+    assert abs(r["phiDotI"] - ref["phiDotI"]) < 1e-10 * max(1, abs(ref["phiDotI"]))
+
+
+def test_solve_nonpositive_pivot_flagged():
+    P = 6
+    S = np.zeros((P, P))
+    S[0, 0] = 1.0
+    S[1, 1] = -0.5          # negative after the +eps shift
+    for i in range(2, P):
+        S[i, i] = 2.0
+    m = dict(mO=np.zeros(P), mE=np.array([0.5, 0.0]), S=S, FR=np.ones(P), FI=np.zeros(P))
+    with _ctx() as ctx:
+        r = ctx.solve(m["mO"], m["mE"], S, m["FR"], m["FI"], imag_time=1, eps=1e-3, check=False)
+    ref = o.solve_param_dot(m, imag_time=1, eps=1e-3)
+    assert r["bad"] == ref["bad"] == 1
+    assert np.allclose(r["xR"], ref["xR"], rtol=1e-12, atol=1e-14)

@@ -1,0 +1,102 @@
+"""GPU TDVP step (Euler, RK4) against the oracle, and hipGraph replay.
+
+With no Metropolis moves between stages (MC_NTHERMSTEPS = 0) every RK stage
+measures the same fed walkers, so the step is deterministic: the parameters
+after one step must match the oracle's restatement of
+CalculateNextParametersRK4 / ...Euler (src/TDVMC.cpp:1834-1853, :1969-2035)
+built from the oracle's evaluation, moments and solve, within 1e-9.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as o
+from tests.helpers import make_cfg, oracle_system
+
+pytestmark = pytest.mark.gpu
+tdvmc = pytest.importorskip("tdvmc_amd")
+
+
+def _setup(imag, K=4):
+    N, D, L, P = 9, 3, 3.2, 6
+    cfg = make_cfg(N=N, D=D, L=L, P_s=P, pot="GAUSSIAN", sysparams=(0.5, 1.2), K=K, hbar2_2m=0.5, W=512,
+                   therm=0, init=0, IMAGINARY_TIME=imag, REG_EPS=1e-3)
+    s = oracle_system(cfg)
+    rng = np.random.default_rng(7 + imag)
+    R = rng.uniform(-L / 2, L / 2, (512, N, D))
+    uR = rng.normal(0, 0.2, s.P)
+    uI = rng.normal(0, 0.05, s.P)
+    uR[P:] *= 0.05
+    uI[P:] *= 0.05
+    return cfg, s, R, uR, uI
+
+
+def _f(s, R, x, imag, eps=1e-3):
+    P = s.P
+    uR, uI = x[:P], x[P:2 * P]
+    out = o.eval_configs(s, R, uR, uI)
+    m = o.moments(out["O"], out["ER"], out["EI"])
+    r = o.solve_param_dot(m, imag_time=imag, precondition=0, eps=eps)
+    return np.concatenate([r["xR"], r["xI"], [r["phiDotR"], r["phiDotI"]]])
+
+
+@pytest.mark.parametrize("imag", [0, 1])
+@pytest.mark.parametrize("ode", [0, 2])
+def test_step_vs_oracle(imag, ode):
+    cfg, s, R, uR, uI = _setup(imag)
+    dt = 0.01
+    x0 = np.concatenate([uR, uI, [0.1, -0.2]])
+    if ode == 0:
+        x1 = x0 + dt * _f(s, R, x0, imag)
+    else:
+        k1 = _f(s, R, x0, imag)
+        k2 = _f(s, R, x0 + dt / 2 * k1, imag)
+        k3 = _f(s, R, x0 + dt / 2 * k2, imag)
+        k4 = _f(s, R, x0 + dt * k3, imag)
+        x1 = x0 + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+    print("tdvp", imag, ode, flush=True)
+    with tdvmc.Context(cfg) as ctx:
+        ctx.set_params(uR, uI, 0.1, -0.2)
+        ctx.set_walkers(R)
+        ctx.tdvp_step(dt, ode_type=ode, precision=tdvmc.FP64)
+        gR, gI, pR, pI = ctx.get_params()
+        assert np.array_equal(ctx.get_walkers(), R)
+    got = np.concatenate([gR, gI, [pR, pI]])
+    step = x1 - x0
+    assert np.max(np.abs(got - x1)) < 1e-9 * np.max(np.abs(step)) + 1e-13
+
+
+def test_mixed_step_close_to_fp64():
+    cfg, s, R, uR, uI = _setup(1)
+    res = {}
+    for prec in (0, 1):
+        with tdvmc.Context(cfg) as ctx:
+            ctx.set_params(uR, uI)
+            ctx.set_walkers(R)
+            ctx.tdvp_step(0.01, ode_type=2, precision=prec)
+            res[prec] = np.concatenate(ctx.get_params()[:2])
+    step = res[0] - np.concatenate([uR, uI])
+    err = np.max(np.abs(res[1] - res[0])) / np.max(np.abs(step))
+    print(f"mixed vs fp64 RK4 step relative difference {err:.2e}")
+    assert err < 1e-3
+
+
+@pytest.mark.parametrize("therm", [0, 9])
+def test_graph_replays_match_eager(therm):
+    cfg, s, R, uR, uI = _setup(1)
+    cfg["MC_NTHERMSTEPS"] = therm
+    cfg["MC_STEP"] = 0.3
+    res = []
+    for graph in (False, True):
+        with tdvmc.Context(cfg) as ctx:
+            ctx.set_params(uR, uI)
+            ctx.set_walkers(R)
+            traj = []
+            for _ in range(3):
+                ctx.tdvp_step(0.01, ode_type=2, precision=tdvmc.FP64, use_graph=graph)
+                traj.append(np.concatenate(ctx.get_params()[:2]))
+                ctx.measure(precision=tdvmc.FP64)      # eager work between replays
+            res.append((np.array(traj), ctx.get_walkers()))
+    # the evaluation sums pair terms with LDS atomics, whose order (and so the
+    # last bit of FP64 sums) varies between launches
+    assert np.allclose(res[0][0], res[1][0], rtol=1e-11, atol=1e-14)
+    assert np.allclose(res[0][1], res[1][1], rtol=1e-12, atol=1e-12)
