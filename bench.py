@@ -42,7 +42,7 @@ def measure_flops(N, D, K):
     return N * (N - 1) / 2 * (8 * D + 29) + N * K * (2 * D + 12) + N * (4 * D + 2)
 
 
-def run_gpu(cfg, steps, warmup, precision, use_graph=True, profile_stages=2):
+def run_gpu(cfg, steps, warmup, precision, use_graph=True, profile_stages=1):
     import tdvmc_amd as t
 
     ctx = t.Context(cfg)
@@ -61,10 +61,11 @@ def run_gpu(cfg, steps, warmup, precision, use_graph=True, profile_stages=2):
     elapsed = time.perf_counter() - t0
     stages = 4 * steps
     samples = stages * W * info.samples_per_walker
-    # per-kernel durations with HIP events on the context stream (eager launches)
+    # per-kernel durations with HIP events on the context stream: the same
+    # TDVP step launched eagerly (events cannot be read back inside a graph)
     ctx.profile_enable(True)
     for _ in range(profile_stages):
-        ctx.sample(seed=99, precision=precision)
+        ctx.tdvp_step(dt, ode_type=2, precision=precision, use_graph=False)
     prof = {k: ctx.profile_get(k) for k in ("sweep", "eval", "moments", "solve")}
     acc, trials = ctx.acceptance()
     uR, _, _, _ = ctx.get_params()

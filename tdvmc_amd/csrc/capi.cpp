@@ -61,6 +61,7 @@ struct tdvmc_ctx {
 
     int64_t W = 0;
     double* dR = nullptr;
+    double* dRho = nullptr;     // [W][K][2] rho_k left by the last sweep (float or double)
     unsigned long long* dAcc = nullptr;
     uint32_t* dEpoch = nullptr;
     double* dUR = nullptr;
@@ -458,10 +459,11 @@ int resolve_precision(tdvmc_ctx* ctx, int precision, int* out)
     return TDVMC_OK;
 }
 
-int enqueue_sweep(tdvmc_ctx* ctx, int prec, int64_t n_moves, uint64_t seed)
+int enqueue_sweep(tdvmc_ctx* ctx, int prec, int64_t n_moves, uint64_t seed, bool want_rho = false)
 {
     if (n_moves <= 0) return TDVMC_OK;
     SweepArgs a;
+    a.rho_out = (want_rho && ctx->sy.K > 0) ? static_cast<void*>(ctx->dRho) : nullptr;
     a.R = ctx->dR;
     a.W = ctx->W;
     a.n_moves = n_moves;
@@ -484,9 +486,11 @@ int enqueue_sweep(tdvmc_ctx* ctx, int prec, int64_t n_moves, uint64_t seed)
     return TDVMC_OK;
 }
 
-// eval of the context walkers into sample rows [row0, row0 + W)
-int enqueue_eval(tdvmc_ctx* ctx, int prec, int64_t row0)
+// eval of the context walkers into sample rows [row0, row0 + W); use_rho:
+// take rho_k from the sweep that just ran instead of recomputing it
+int enqueue_eval(tdvmc_ctx* ctx, int prec, int64_t row0, bool use_rho = false)
 {
+    const bool rho = use_rho && ctx->sy.K > 0;
     const int P = P_of(ctx);
     hipEvent_t ev{};
     int rc = prof_begin(ctx, "eval", &ev);
@@ -494,11 +498,13 @@ int enqueue_eval(tdvmc_ctx* ctx, int prec, int64_t row0)
     if (prec == TDVMC_MIXED) {
         HIPC((launch_eval<float, float>(ctx->sy, tables_of(ctx, ctx->tf), ctx->dR, ctx->W, ctx->dExpo + row0,
                                         ctx->dER + row0, ctx->dEI + row0, ctx->dO32 + row0 * P,
-                                        ctx->dOther + row0 * 9, nullptr, nullptr, nullptr, ctx->phiR, ctx->stream)));
+                                        ctx->dOther + row0 * 9, nullptr, nullptr, nullptr, ctx->phiR,
+                                        rho ? reinterpret_cast<const float*>(ctx->dRho) : nullptr, ctx->stream)));
     } else {
         HIPC((launch_eval<double, double>(ctx->sy, tables_of(ctx, ctx->td), ctx->dR, ctx->W, ctx->dExpo + row0,
                                           ctx->dER + row0, ctx->dEI + row0, ctx->dO64 + row0 * P,
-                                          ctx->dOther + row0 * 9, nullptr, nullptr, nullptr, ctx->phiR, ctx->stream)));
+                                          ctx->dOther + row0 * 9, nullptr, nullptr, nullptr, ctx->phiR,
+                                          rho ? ctx->dRho : nullptr, ctx->stream)));
     }
     return prof_end(ctx, "eval", ev);
 }
@@ -542,12 +548,13 @@ int enqueue_solve(tdvmc_ctx* ctx, double* xdot)
 // one sampling stage with the current device tables: moves, measurements, moments
 int enqueue_stage_samples(tdvmc_ctx* ctx, int prec, uint64_t seed)
 {
-    int rc = enqueue_sweep(ctx, prec, ctx->init_moves, seed);
+    int rc = enqueue_sweep(ctx, prec, ctx->init_moves, seed, true);
     if (rc) return rc;
     for (int s = 0; s < ctx->spw; s++) {
-        rc = enqueue_sweep(ctx, prec, ctx->therm_moves, seed);
+        rc = enqueue_sweep(ctx, prec, ctx->therm_moves, seed, true);
         if (rc) return rc;
-        rc = enqueue_eval(ctx, prec, (int64_t)s * ctx->W);
+        const bool swept = ctx->therm_moves > 0 || (s == 0 && ctx->init_moves > 0);
+        rc = enqueue_eval(ctx, prec, (int64_t)s * ctx->W, swept);
         if (rc) return rc;
     }
     return enqueue_moments(ctx, prec, (int64_t)ctx->spw * ctx->W);
@@ -686,6 +693,7 @@ void release(tdvmc_ctx* ctx)
     if (ctx->graph) hipGraphExecDestroy(ctx->graph);
     if (ctx->graph_src) hipGraphDestroy(ctx->graph_src);
     dfree(ctx->dR);
+    dfree(ctx->dRho);
     dfree(ctx->dAcc);
     dfree(ctx->dEpoch);
     dfree(ctx->dUR);
@@ -823,6 +831,7 @@ int tdvmc_set_walker_count(tdvmc_ctx* ctx, int64_t W)
     HIPC(hipStreamSynchronize(ctx->stream));
     HIPC(dalloc(&ctx->dR, (size_t)W * ctx->sy.N * ctx->sy.D));
     HIPC(dalloc(&ctx->dAcc, (size_t)W));
+    HIPC(dalloc(&ctx->dRho, (size_t)W * 2 * std::max(ctx->sy.K, 1)));
     HIPC(hipMemsetAsync(ctx->dAcc, 0, (size_t)W * sizeof(unsigned long long), ctx->stream));
     ctx->W = W;
     ctx->trials_total = 0;
@@ -1024,10 +1033,10 @@ int tdvmc_eval_configs_dev(tdvmc_ctx* ctx, const double* R_dev, int64_t W, int p
     if (rc) return rc;
     if (prec == TDVMC_MIXED) {
         HIPC((launch_eval<float, double>(ctx->sy, tables_of(ctx, ctx->tf), R_dev, W, expo_dev, ER_dev, EI_dev, O_dev,
-                                         nullptr, grad_dev, lap_dev, nullptr, ctx->phiR, st)));
+                                         nullptr, grad_dev, lap_dev, nullptr, ctx->phiR, nullptr, st)));
     } else {
         HIPC((launch_eval<double, double>(ctx->sy, tables_of(ctx, ctx->td), R_dev, W, expo_dev, ER_dev, EI_dev, O_dev,
-                                          nullptr, grad_dev, lap_dev, nullptr, ctx->phiR, st)));
+                                          nullptr, grad_dev, lap_dev, nullptr, ctx->phiR, nullptr, st)));
     }
     return TDVMC_OK;
 }
@@ -1068,10 +1077,10 @@ int tdvmc_eval_configs(tdvmc_ctx* ctx, const double* R, int64_t W, int precision
         prof_begin(ctx, "eval", &ev);
         if (prec == TDVMC_MIXED)
             ok = hipc((launch_eval<float, double>(ctx->sy, tables_of(ctx, ctx->tf), dR, W, dEx, dER, dEI, dO, dOt, dG,
-                                                  dL, nullptr, ctx->phiR, ctx->stream)), "eval kernel");
+                                                  dL, nullptr, ctx->phiR, nullptr, ctx->stream)), "eval kernel");
         else
             ok = hipc((launch_eval<double, double>(ctx->sy, tables_of(ctx, ctx->td), dR, W, dEx, dER, dEI, dO, dOt,
-                                                   dG, dL, nullptr, ctx->phiR, ctx->stream)), "eval kernel");
+                                                   dG, dL, nullptr, ctx->phiR, nullptr, ctx->stream)), "eval kernel");
         prof_end(ctx, "eval", ev);
     }
     auto down = [&](double* h, double* d, size_t n) {
@@ -1254,10 +1263,10 @@ int tdvmc_adjoint(tdvmc_ctx* ctx, const double* R, int64_t W, const double* gbar
         if (rc == TDVMC_OK) {
             if (prec == TDVMC_MIXED)
                 e = launch_eval<float, double>(ctx->sy, tables_of(ctx, ctx->tf), dRin, W, dEx, dER, dEI, dO, nullptr,
-                                               dG, dL, dOuter, ctx->phiR, ctx->stream);
+                                               dG, dL, dOuter, ctx->phiR, nullptr, ctx->stream);
             else
                 e = launch_eval<double, double>(ctx->sy, tables_of(ctx, ctx->td), dRin, W, dEx, dER, dEI, dO, nullptr,
-                                                dG, dL, dOuter, ctx->phiR, ctx->stream);
+                                                dG, dL, dOuter, ctx->phiR, nullptr, ctx->stream);
         }
     }
     if (e == hipSuccess && rc == TDVMC_OK) {

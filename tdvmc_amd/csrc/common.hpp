@@ -133,4 +133,54 @@ __device__ __forceinline__ int knot_interval(T r, T hinv, int Ps, T* t)
     return j;
 }
 
+// ------------------------------------------------------------- lane exchange
+// DPP controls (GFX9 encoding)
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kDppRowMirror = 0x140;  // lane l <- 15 - l within a row of 16
+constexpr int kDppHalfMirror = 0x141; // lane l <- 7 - l within a half row of 8
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f(double v)
+{
+    const int2 x = *reinterpret_cast<int2*>(&v);
+    int2 y;
+    y.x = __builtin_amdgcn_update_dpp(0, x.x, CTRL, 0xF, 0xF, false);
+    y.y = __builtin_amdgcn_update_dpp(0, x.y, CTRL, 0xF, 0xF, false);
+    return *reinterpret_cast<double*>(&y);
+}
+
+__device__ __forceinline__ float readlane_f(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ __forceinline__ double readlane_f(double v, int l)
+{
+    const int2 x = *reinterpret_cast<int2*>(&v);
+    int2 y;
+    y.x = __builtin_amdgcn_readlane(x.x, l);
+    y.y = __builtin_amdgcn_readlane(x.y, l);
+    return *reinterpret_cast<double*>(&y);
+}
+__device__ __forceinline__ int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+// Sum over each aligned group of G lanes (G = 16, 32 or 64).  Every lane of a
+// group receives the same bits: the in-row steps pair lanes symmetrically
+// (xor within quads, then mirrors), so each add sees the same two operands in
+// either order, and the rows are combined in a fixed order.
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v)
+{
+    v += dpp_f<kDppXor1>(v);
+    v += dpp_f<kDppXor2>(v);
+    v += dpp_f<kDppHalfMirror>(v);
+    v += dpp_f<kDppRowMirror>(v);
+    if (G == 16) return v;
+    const T r0 = readlane_f(v, 0), r1 = readlane_f(v, 16), r2 = readlane_f(v, 32), r3 = readlane_f(v, 48);
+    if (G == 64) return (r0 + r1) + (r2 + r3);
+    return ((threadIdx.x & 63) < 32) ? (r0 + r1) : (r2 + r3);
+}
+
 }  // namespace tdvmc

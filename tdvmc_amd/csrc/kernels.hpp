@@ -15,10 +15,12 @@ hipError_t launch_prep_tables(const KSys& sy, const double* uR, const double* uI
 // ---- evaluation (kernels_eval.hip); grad/lap non-null selects the per-particle variant
 template <typename T>
 size_t eval_lds_bytes(const KSys& sy, bool per_particle);
+// rho_in: [W][K][2] rho_k of the walkers as left by the sweep (skips the
+// rho_k pass), or null to compute it from the coordinates
 template <typename T, typename TO>
 hipError_t launch_eval(const KSys& sy, const KTables<T>& tb, const double* R, int64_t W, double* expo, double* ER,
                        double* EI, TO* O, double* other, double* grad, double* lap, double* outer, double phiR,
-                       hipStream_t st);
+                       const T* rho_in, hipStream_t st);
 
 // ---- Metropolis sweeps (kernels_sweep.hip)
 struct SweepArgs {
@@ -29,6 +31,7 @@ struct SweepArgs {
     const uint32_t* epoch;     // device counter selecting the random stream of this call
     unsigned long long* acc;   // [W] accepted moves (accumulated)
     int lanes_per_walker;      // 16, 32 or 64 (0 = choose)
+    void* rho_out;             // [W][K][2] (T) final rho_k of every walker, or null
 };
 template <typename T>
 hipError_t launch_sweep(const KSys& sy, const KTables<T>& tb, const SweepArgs& a, hipStream_t st);

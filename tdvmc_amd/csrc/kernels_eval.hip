@@ -65,14 +65,15 @@ __device__ __forceinline__ double block_sum(double v, double* red)
     return s;
 }
 
-template <typename T, int D, bool PP, typename TO>
+template <typename T, int D, bool PP, bool PER, typename TO>
 __global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, const double* __restrict__ Rg,
-                                                         int64_t W, EvalOut<TO> out)
+                                                         int64_t W, EvalOut<TO> out, const T* __restrict__ rho_in)
 {
     const int64_t w = blockIdx.x;
     if (w >= W) return;
     const int N = sy.N, Ps = sy.P_s, K = sy.K, S = Ps + 2;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (scalar registers)
 
     extern __shared__ __align__(16) unsigned char smem[];
     // LDS carve-up (sizes mirrored by eval_lds_bytes)
@@ -83,17 +84,18 @@ __global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, 
     T* pos = reinterpret_cast<T*>(hist + (size_t)S * kHistCols);   // D * N (structure of arrays)
     T* rho = pos + (size_t)N * D;                              // 2K
     T* spl = rho + 2 * (size_t)K;                              // (Ps-1) * 8
-    T* kv = spl + (size_t)(Ps - 1) * 8;                        // K * 4
-    T* al = kv + 4 * (size_t)K;                                // K * 2
+    const T* kv = tb.kv;                                       // K * 4 (global, read with uniform k)
 
-    const T L = (T)sy.L, Linv = (T)sy.Linv;
+    // periodic box: coordinates in box units (x / L, wrapped into the primary
+    // cell -- every quantity is invariant under it), so the minimum image is
+    // d - rint(d); lengths below are scaled to match
+    const double scale = PER ? sy.Linv : 1.0;
     const double* Rw = Rg + w * (size_t)N * D;
     for (int i = tid; i < N; i += kThreads) {
 #pragma unroll
         for (int a = 0; a < D; a++) {
-            double x = Rw[(size_t)i * D + a];
-            // wrap into the primary cell: every quantity is invariant under it
-            if (sy.periodic) x = x - sy.L * rint(x * sy.Linv);
+            double x = Rw[(size_t)i * D + a] * scale;
+            if (PER) x = x - rint(x);
             pos[a * N + i] = (T)x;
         }
 #pragma unroll
@@ -105,13 +107,13 @@ __global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, 
     }
     for (int i = tid; i < S * kHistCols; i += kThreads) hist[i] = 0.0;
     for (int i = tid; i < (Ps - 1) * 8; i += kThreads) spl[i] = tb.spl[i];
-    for (int i = tid; i < 4 * K; i += kThreads) kv[i] = tb.kv[i];
-    for (int i = tid; i < 2 * K; i += kThreads) al[i] = tb.alpha[i];
     __syncthreads();
 
     // ------------------------------------------------------------ pairs
-    const T hinv = (T)sy.hinv, rc = (T)sy.rc, pcut = (T)sy.pot_cutoff;
-    const T pa = (T)sy.pot_a, pb = (T)sy.pot_b;
+    const T hinv = (T)sy.hinv, hsc = (T)(sy.hinv / scale), rc = (T)(sy.rc * scale);
+    const T pcut = (T)(sy.pot_cutoff * scale);
+    const T pa = (T)(sy.pot_a * scale), pb = (T)sy.pot_b;
+    const T lapsc = (T)((sy.D - 1) * scale);   // (D-1)/r = (D-1) scale / r_box
     const int shalf = (N - 1) / 2;
     const int smax = (N % 2 == 0) ? N / 2 : shalf;    // s = N/2 only for n < N/2 (even N)
     const int chunk = (smax + kWaves - 1) / kWaves;
@@ -137,7 +139,8 @@ __global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, 
                     T r2 = (T)0;
 #pragma unroll
                     for (int a = 0; a < D; a++) {
-                        d[a] = min_image(rn[a] - pos[a * N + i], L, Linv, sy.periodic);
+                        d[a] = rn[a] - pos[a * N + i];
+                        if (PER) d[a] -= Prec<T>::rint_(d[a]);
                         r2 += d[a] * d[a];
                     }
                     const T r = Prec<T>::sqrt_(r2);
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, 
                             T x = r / pa;
                             v = pb * Prec<T>::exp_((T)-0.5 * x * x);
                         } else {
-                            T q = pa * pa / r2;
+                            T q = pa * pa * Prec<T>::rcp_(r2);
                             T q6 = q * q * q;
                             v = (T)4 * pb * q6 * (q6 - (T)1);
                         }
@@ -157,14 +160,20 @@ __global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, 
                     }
                     if (r <= rc) {
                         T t;
-                        const int j = knot_interval(r, hinv, Ps, &t);
+                        const int j = knot_interval(r, hsc, Ps, &t);
                         const T u = (T)1 - t;
                         const T t2 = t * t, t3 = t2 * t;
+                        // the four non-zero uniform cubic B-splines; the third
+                        // follows from the partition of unity
+                        const T b0 = u * u * u * (T)(1.0 / 6.0);
+                        const T b1 = ((T)3 * t3 - (T)6 * t2 + (T)4) * (T)(1.0 / 6.0);
+                        const T b3 = t3 * (T)(1.0 / 6.0);
+                        const T b2 = (T)1 - b0 - b1 - b3;
                         double* hb = hist + (size_t)j * kHistCols + (lane & (kHistCols - 1));
-                        lds_add(hb, (double)(u * u * u * (T)(1.0 / 6.0)));
-                        lds_add(hb + kHistCols, (double)(((T)3 * t3 - (T)6 * t2 + (T)4) * (T)(1.0 / 6.0)));
-                        lds_add(hb + 2 * kHistCols, (double)(((T)-3 * t3 + (T)3 * t2 + (T)3 * t + (T)1) * (T)(1.0 / 6.0)));
-                        lds_add(hb + 3 * kHistCols, (double)(t3 * (T)(1.0 / 6.0)));
+                        lds_add(hb, (double)b0);
+                        lds_add(hb + kHistCols, (double)b1);
+                        lds_add(hb + 2 * kHistCols, (double)b2);
+                        lds_add(hb + 3 * kHistCols, (double)b3);
                         const T* c = spl + j * 8;
                         // u'(r) = (A1 + 2 A2 t + 3 A3 t^2) / h ; u''(r) = (2 A2 + 6 A3 t) / h^2
                         const T upR = (c[1] + t * ((T)2 * c[2] + (T)3 * c[3] * t)) * hinv;
@@ -181,8 +190,8 @@ __global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, 
                             lds_add(gp + a, (double)(-fR * d[a]));
                             lds_add(gp + D + a, (double)(-fI * d[a]));
                         }
-                        const T lfR = uppR + (T)(D - 1) * fR;
-                        const T lfI = uppI + (T)(D - 1) * fI;
+                        const T lfR = uppR + lapsc * fR;
+                        const T lfI = uppI + lapsc * fI;
                         if (PP) {
                             lR += lfR;
                             lI += lfI;
@@ -212,20 +221,26 @@ __global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, 
     }
 
     // ------------------------------------------------------------ rho_k
+    const T ksc = (T)(1.0 / scale);   // phase in revolutions = (kv / scale) . s
     double ex_k = 0.0;
     for (int k = tid; k < K; k += kThreads) {
-        T q[D];
-#pragma unroll
-        for (int a = 0; a < D; a++) q[a] = kv[k * 4 + a];
         T re = (T)0, im = (T)0;
-        for (int n = 0; n < N; n++) {
-            T ph = (T)0;
+        if (rho_in) {
+            re = rho_in[((size_t)w * K + k) * 2];
+            im = rho_in[((size_t)w * K + k) * 2 + 1];
+        } else {
+            T q[D];
 #pragma unroll
-            for (int a = 0; a < D; a++) ph += q[a] * pos[a * N + n];
-            T sn, cs;
-            Prec<T>::sincos_rev(ph, &sn, &cs);
-            re += cs;
-            im += sn;
+            for (int a = 0; a < D; a++) q[a] = kv[k * 4 + a] * ksc;
+            for (int n = 0; n < N; n++) {
+                T ph = (T)0;
+#pragma unroll
+                for (int a = 0; a < D; a++) ph += q[a] * pos[a * N + n];
+                T sn, cs;
+                Prec<T>::sincos_rev(ph, &sn, &cs);
+                re += cs;
+                im += sn;
+            }
         }
         rho[2 * k] = re;
         rho[2 * k + 1] = im;
@@ -249,27 +264,37 @@ __global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, 
 #pragma unroll
                 for (int a = 0; a < D; a++) gR[a] = gI[a] = (T)0;
                 T lR = (T)0, lI = (T)0;
+                // grad |rho_k|^2 = -2 k Im(e^{ikr} conj rho), k = 2 pi kv;
+                // lap |rho_k|^2 = 2 |k|^2 (1 - Re(e^{ikr} conj rho)); the
+                // constant factors are applied after the loop
                 for (int k = k_lo; k < k_hi; k++) {
+                    const T* q = tb.kv + k * 4;
                     T ph = (T)0;
 #pragma unroll
-                    for (int a = 0; a < D; a++) ph += kv[k * 4 + a] * rn[a];
+                    for (int a = 0; a < D; a++) ph += (q[a] * ksc) * rn[a];
                     T sn, cs;
                     Prec<T>::sincos_rev(ph, &sn, &cs);
                     const T rr = rho[2 * k], ri = rho[2 * k + 1];
                     const T pre = cs * rr + sn * ri;    // Re(e^{ikr} conj(rho))
                     const T pim = sn * rr - cs * ri;    // Im(e^{ikr} conj(rho))
-                    const T aR = al[2 * k], aI = al[2 * k + 1];
-                    // grad |rho|^2 = -2 k Im(...), k = 2 pi kv
-                    const T gk = (T)(-2.0 * 6.283185307179586) * pim;
-                    const T lk = (T)2 * kv[k * 4 + 3] * ((T)1 - pre);
+                    const T aR = tb.alpha[2 * k], aI = tb.alpha[2 * k + 1];
+                    const T sR = aR * pim, sI = aI * pim;
 #pragma unroll
                     for (int a = 0; a < D; a++) {
-                        gR[a] += aR * gk * kv[k * 4 + a];
-                        gI[a] += aI * gk * kv[k * 4 + a];
+                        gR[a] += sR * q[a];
+                        gI[a] += sI * q[a];
                     }
+                    const T lk = q[3] * ((T)1 - pre);
                     lR += aR * lk;
                     lI += aI * lk;
                 }
+#pragma unroll
+                for (int a = 0; a < D; a++) {
+                    gR[a] *= (T)(-2.0 * 6.283185307179586);
+                    gI[a] *= (T)(-2.0 * 6.283185307179586);
+                }
+                lR *= (T)2;
+                lI *= (T)2;
                 double* gp = gsum + (size_t)n * 2 * D;
 #pragma unroll
                 for (int a = 0; a < D; a++) {
@@ -397,37 +422,47 @@ template <typename T>
 size_t eval_lds_bytes(const KSys& sy, bool per_particle)
 {
     const size_t nd = (size_t)sy.N * 2 * sy.D + (per_particle ? 2 * (size_t)sy.N : 0) + (size_t)(sy.P_s + 2) * kHistCols;
-    const size_t nt = (size_t)sy.N * sy.D + 2 * (size_t)sy.K + (size_t)(sy.P_s - 1) * 8 + 4 * (size_t)sy.K +
-                      2 * (size_t)sy.K;
+    const size_t nt = (size_t)sy.N * sy.D + 2 * (size_t)sy.K + (size_t)(sy.P_s - 1) * 8;
     return 64 + nd * sizeof(double) + nt * sizeof(T);
 }
 
-template <typename T, int D, bool PP, typename TO>
-static hipError_t launch_eval_d(const KSys& sy, const KTables<T>& tb, const double* R, int64_t W,
-                                const EvalOut<TO>& o, hipStream_t st)
+template <typename T, int D, bool PP, bool PER, typename TO>
+static hipError_t launch_eval_k(const KSys& sy, const KTables<T>& tb, const double* R, int64_t W,
+                                const EvalOut<TO>& o, const T* rho_in, hipStream_t st)
 {
     const size_t lds = eval_lds_bytes<T>(sy, PP);
-    auto kern = eval_kernel<T, D, PP, TO>;
+    auto kern = eval_kernel<T, D, PP, PER, TO>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kern, dim3((unsigned)W), dim3(kThreads), lds, st, sy, tb, R, W, o);
+    hipLaunchKernelGGL(kern, dim3((unsigned)W), dim3(kThreads), lds, st, sy, tb, R, W, o, rho_in);
     return hipGetLastError();
+}
+
+template <typename T, int D, typename TO>
+static hipError_t launch_eval_d(const KSys& sy, const KTables<T>& tb, const double* R, int64_t W,
+                                const EvalOut<TO>& o, const T* rho_in, bool pp, hipStream_t st)
+{
+    if (sy.periodic)
+        return pp ? launch_eval_k<T, D, true, true, TO>(sy, tb, R, W, o, rho_in, st)
+                  : launch_eval_k<T, D, false, true, TO>(sy, tb, R, W, o, rho_in, st);
+    return pp ? launch_eval_k<T, D, true, false, TO>(sy, tb, R, W, o, rho_in, st)
+              : launch_eval_k<T, D, false, false, TO>(sy, tb, R, W, o, rho_in, st);
 }
 
 template <typename T, typename TO>
 hipError_t launch_eval(const KSys& sy, const KTables<T>& tb, const double* R, int64_t W, double* expo, double* ER,
                        double* EI, TO* O, double* other, double* grad, double* lap, double* outer, double phiR,
-                       hipStream_t st)
+                       const T* rho_in, hipStream_t st)
 {
     if (W <= 0) return hipSuccess;
     EvalOut<TO> o{expo, ER, EI, O, other, grad, lap, outer, phiR};
     const bool pp = grad != nullptr;
     switch (sy.D) {
-    case 1: return pp ? launch_eval_d<T, 1, true, TO>(sy, tb, R, W, o, st) : launch_eval_d<T, 1, false, TO>(sy, tb, R, W, o, st);
-    case 2: return pp ? launch_eval_d<T, 2, true, TO>(sy, tb, R, W, o, st) : launch_eval_d<T, 2, false, TO>(sy, tb, R, W, o, st);
-    case 3: return pp ? launch_eval_d<T, 3, true, TO>(sy, tb, R, W, o, st) : launch_eval_d<T, 3, false, TO>(sy, tb, R, W, o, st);
+    case 1: return launch_eval_d<T, 1, TO>(sy, tb, R, W, o, rho_in, pp, st);
+    case 2: return launch_eval_d<T, 2, TO>(sy, tb, R, W, o, rho_in, pp, st);
+    case 3: return launch_eval_d<T, 3, TO>(sy, tb, R, W, o, rho_in, pp, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -435,10 +470,13 @@ hipError_t launch_eval(const KSys& sy, const KTables<T>& tb, const double* R, in
 template size_t eval_lds_bytes<float>(const KSys&, bool);
 template size_t eval_lds_bytes<double>(const KSys&, bool);
 template hipError_t launch_eval<double, double>(const KSys&, const KTables<double>&, const double*, int64_t, double*,
-                                                double*, double*, double*, double*, double*, double*, double*, double, hipStream_t);
+                                                double*, double*, double*, double*, double*, double*, double*, double,
+                                                const double*, hipStream_t);
 template hipError_t launch_eval<float, float>(const KSys&, const KTables<float>&, const double*, int64_t, double*,
-                                              double*, double*, float*, double*, double*, double*, double*, double, hipStream_t);
+                                              double*, double*, float*, double*, double*, double*, double*, double,
+                                              const float*, hipStream_t);
 template hipError_t launch_eval<float, double>(const KSys&, const KTables<float>&, const double*, int64_t, double*,
-                                               double*, double*, double*, double*, double*, double*, double*, double, hipStream_t);
+                                               double*, double*, double*, double*, double*, double*, double*, double,
+                                               const float*, hipStream_t);
 
 }  // namespace tdvmc

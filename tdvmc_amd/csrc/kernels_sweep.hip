@@ -11,13 +11,17 @@
 //
 // Mapping: a walker is a group of LPW lanes (LPW = 16, 32 or 64) inside a
 // wavefront; a 256-thread workgroup holds 256/LPW walkers.  The walker's
-// coordinates live in LDS (structure of arrays) for the whole launch; lane g of the group handles
-// partners i = g, g + LPW, ... and k-vectors k = g + m LPW (their rho_k stay
-// in that lane's registers).  The change of ln|psi| is summed with an xor
-// butterfly (bitwise identical in every lane), so all lanes take the same
-// accept decision without further communication.  Random numbers come from
-// Philox4x32-10 (counter = trial, call epoch, walker): each lane draws the
-// numbers of one future trial, and trial t of a batch reads them from lane t.
+// coordinates live in LDS (structure of arrays) for the whole launch; lane g
+// of the group handles partners i = g, g + LPW, ... and k-vectors
+// k = g + m LPW (their rho_k stay in that lane's registers).  The change of
+// ln|psi| is summed with DPP row steps whose result is bitwise identical in
+// every lane of the group, so all lanes take the same accept decision without
+// further communication.  Random numbers come from Philox4x32-10 (counter =
+// trial, call epoch, walker): each lane draws the numbers of one future
+// trial, and trial t of a batch reads them from lane t (v_readlane when the
+// group is the whole wavefront).  The pair terms are branch-free.  In a
+// periodic box the coordinates are held in box units (x / L), so the minimum
+// image is d - rint(d) and the phase in revolutions is n . s.
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -31,7 +35,33 @@ constexpr int kThreads = 256;
 template <typename T>
 struct Vec4 { T x, y, z, w; };
 
-template <typename T, int D, int LPW, int KPL>
+template <int LPW, typename T>
+__device__ __forceinline__ T bcast(T v, int src)
+{
+    if (LPW == 64) return readlane_f(v, src);
+    return __shfl(v, src, LPW);
+}
+template <int LPW>
+__device__ __forceinline__ int bcast_i(int v, int src)
+{
+    if (LPW == 64) return readlane_i(v, src);
+    return __shfl(v, src, LPW);
+}
+
+// u(r) on the spline table, or the outer-pair value beyond the cutoff
+template <typename T>
+__device__ __forceinline__ T pair_u(T r, T rc, T hinv, int Ps, const Vec4<T>* spl, T u_out)
+{
+    T x = r * hinv;
+    int j = (int)x;
+    j = j > Ps - 2 ? Ps - 2 : j;
+    const T t = x - (T)j;
+    const Vec4<T> cf = spl[j];
+    const T u = cf.x + t * (cf.y + t * (cf.z + t * cf.w));
+    return r <= rc ? u : u_out;
+}
+
+template <typename T, int D, int LPW, int KPL, bool PER>
 __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb, SweepArgs a)
 {
     constexpr int WPB = kThreads / LPW;   // walkers per block
@@ -60,14 +90,16 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
         kvl[i].w = tb.kv[i * 4 + 3];
         all[i] = tb.alpha[2 * i];
     }
+    // lengths in box units when periodic (scale = 1/L), unscaled otherwise
+    const double scale = PER ? sy.Linv : 1.0;
     const bool active = w < a.W;
     double* Rw = a.R + (active ? w : 0) * (size_t)N * D;
     if (active) {
         for (int i = gl; i < N; i += LPW) {
 #pragma unroll
             for (int c = 0; c < D; c++) {
-                double x = Rw[(size_t)i * D + c];
-                if (sy.periodic) x = x - sy.L * rint(x * sy.Linv);
+                double x = Rw[(size_t)i * D + c] * scale;
+                if (PER) x = x - rint(x);
                 pos[c * N + i] = (T)x;
             }
         }
@@ -75,10 +107,10 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
     __syncthreads();
     if (!active) return;
 
-    const T L = (T)sy.L, Linv = (T)sy.Linv, hinv = (T)sy.hinv, rc = (T)sy.rc, step = (T)sy.step;
+    const T hinv = (T)(sy.hinv / scale), rc = (T)(sy.rc * scale), step = (T)(sy.step * scale);
     const T u_out = (T)tb.uR[sy.outer_idx];
-    const T ob = sy.one_body ? (T)tb.uR[0] : (T)0;
-    const int periodic = sy.periodic;
+    const T ob = sy.one_body ? (T)(tb.uR[0] / (scale * scale)) : (T)0;
+    const T kscale = (T)(1.0 / scale);   // phase in revolutions = (kv / scale) . s
 
     // rho_k of this lane's k-vectors
     T rhoR[KPL > 0 ? KPL : 1], rhoI[KPL > 0 ? KPL : 1];
@@ -87,7 +119,10 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
         rhoR[m] = rhoI[m] = (T)0;
         const int k = gl + m * LPW;
         if (k < K) {
-            const Vec4<T> q = kvl[k];
+            Vec4<T> q = kvl[k];
+            q.x *= kscale;
+            q.y *= kscale;
+            q.z *= kscale;
             for (int n = 0; n < N; n++) {
                 T ph = q.x * pos[n];
                 if (D > 1) ph += q.y * pos[N + n];
@@ -104,16 +139,21 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
     const uint32_t ep = a.epoch ? *a.epoch : 0u;
     const int64_t ntr = a.n_moves;
     unsigned long long nacc = 0;
+    int seq = 0;   // sequential order: particle of trial base + gl
 
     for (int64_t base = 0; base < ntr; base += LPW) {
         // random numbers of trial base + gl
         const uint32_t tr = (uint32_t)(base + gl);
         Philox4 x = philox4x32_10(tr, ep, (uint32_t)w, 0u, k0, k1);
         int pm;
-        if (sy.move_order == 0)
+        if (sy.move_order == 0) {
             pm = (int)__umulhi(x.v[0], (uint32_t)N);
-        else
-            pm = (int)((base + gl) % N);
+        } else {
+            pm = seq + gl;
+            while (pm >= N) pm -= N;
+            seq += LPW;
+            while (seq >= N) seq -= N;
+        }
         T z[3];
         T logu;
         {
@@ -138,47 +178,38 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
         }
         const int nb = (int)((ntr - base) < LPW ? (ntr - base) : LPW);
         for (int tt = 0; tt < nb; tt++) {
-            const int p = __shfl(pm, tt, LPW);
+            const int p = bcast_i<LPW>(pm, tt);
             T dz[3];
 #pragma unroll
-            for (int c = 0; c < D; c++) dz[c] = __shfl(z[c], tt, LPW);
-            const T lu = __shfl(logu, tt, LPW);
+            for (int c = 0; c < D; c++) dz[c] = bcast<LPW>(z[c], tt);
+            const T lu = bcast<LPW>(logu, tt);
 
             T old[D], nw[D];
 #pragma unroll
             for (int c = 0; c < D; c++) {
                 old[c] = pos[c * N + p];
                 T v = old[c] + step * dz[c];
-                if (periodic) v = v - L * Prec<T>::rint_(v * Linv);
+                if (PER) v = v - Prec<T>::rint_(v);
                 nw[c] = v;
             }
             T delta = (T)0;
+#pragma unroll 2
             for (int i = gl; i < N; i += LPW) {
-                if (i == p) continue;
                 T ro2 = (T)0, rn2 = (T)0;
 #pragma unroll
                 for (int c = 0; c < D; c++) {
                     const T pi = pos[c * N + i];
-                    const T dOld = min_image(pi - old[c], L, Linv, periodic);
-                    const T dNew = min_image(pi - nw[c], L, Linv, periodic);
+                    T dOld = pi - old[c], dNew = pi - nw[c];
+                    if (PER) {
+                        dOld -= Prec<T>::rint_(dOld);
+                        dNew -= Prec<T>::rint_(dNew);
+                    }
                     ro2 += dOld * dOld;
                     rn2 += dNew * dNew;
                 }
-                const T ro = Prec<T>::sqrt_(ro2), rn = Prec<T>::sqrt_(rn2);
-                T uo = u_out, un = u_out;
-                if (ro <= rc) {
-                    T t;
-                    const int j = knot_interval(ro, hinv, Ps, &t);
-                    const Vec4<T> cf = spl[j];
-                    uo = cf.x + t * (cf.y + t * (cf.z + t * cf.w));
-                }
-                if (rn <= rc) {
-                    T t;
-                    const int j = knot_interval(rn, hinv, Ps, &t);
-                    const Vec4<T> cf = spl[j];
-                    un = cf.x + t * (cf.y + t * (cf.z + t * cf.w));
-                }
-                delta += un - uo;
+                const T uo = pair_u(Prec<T>::sqrt_(ro2), rc, hinv, Ps, spl, u_out);
+                const T un = pair_u(Prec<T>::sqrt_(rn2), rc, hinv, Ps, spl, u_out);
+                delta += (i != p) ? un - uo : (T)0;
             }
             T dR[KPL > 0 ? KPL : 1], dI[KPL > 0 ? KPL : 1];
 #pragma unroll
@@ -186,7 +217,10 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
                 dR[m] = dI[m] = (T)0;
                 const int k = gl + m * LPW;
                 if (k < K) {
-                    const Vec4<T> q = kvl[k];
+                    Vec4<T> q = kvl[k];
+                    q.x *= kscale;
+                    q.y *= kscale;
+                    q.z *= kscale;
                     T po = q.x * old[0], pn = q.x * nw[0];
                     if (D > 1) {
                         po += q.y * old[1];
@@ -214,8 +248,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
                 }
                 delta += -ob * (rn2 - ro2);
             }
-#pragma unroll
-            for (int o = LPW / 2; o > 0; o >>= 1) delta += __shfl_xor(delta, o, LPW);
+            delta = group_sum<LPW>(delta);
             const bool accept = ((T)2 * delta >= lu);
             if (accept) {
                 if (gl == 0) {
@@ -238,7 +271,18 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
 
     for (int i = gl; i < N; i += LPW) {
 #pragma unroll
-        for (int c = 0; c < D; c++) Rw[(size_t)i * D + c] = (double)pos[c * N + i];
+        for (int c = 0; c < D; c++) Rw[(size_t)i * D + c] = (double)pos[c * N + i] * (PER ? sy.L : 1.0);
+    }
+    if (a.rho_out) {
+        T* ro = static_cast<T*>(a.rho_out) + (size_t)w * K * 2;
+#pragma unroll
+        for (int m = 0; m < KPL; m++) {
+            const int k = gl + m * LPW;
+            if (k < K) {
+                ro[2 * k] = rhoR[m];
+                ro[2 * k + 1] = rhoI[m];
+            }
+        }
     }
     if (gl == 0 && a.acc) a.acc[w] += nacc;
 }
@@ -261,11 +305,11 @@ size_t sweep_lds_bytes(const KSys& sy, int lpw)
                         wpb * sy.N * sy.D);
 }
 
-template <typename T, int D, int LPW, int KPL>
+template <typename T, int D, int LPW, int KPL, bool PER>
 static hipError_t launch_sweep_k(const KSys& sy, const KTables<T>& tb, const SweepArgs& a, hipStream_t st)
 {
     const size_t lds = sweep_lds_bytes<T>(sy, LPW);
-    auto kern = sweep_kernel<T, D, LPW, KPL>;
+    auto kern = sweep_kernel<T, D, LPW, KPL, PER>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -276,25 +320,30 @@ static hipError_t launch_sweep_k(const KSys& sy, const KTables<T>& tb, const Swe
     return hipGetLastError();
 }
 
-template <typename T, int D, int LPW>
+template <typename T, int D, int LPW, bool PER>
 static hipError_t launch_sweep_l(const KSys& sy, const KTables<T>& tb, const SweepArgs& a, hipStream_t st)
 {
     const int kpl = (sy.K + LPW - 1) / LPW;
-    if (kpl == 0) return launch_sweep_k<T, D, LPW, 0>(sy, tb, a, st);
-    if (kpl <= 2) return launch_sweep_k<T, D, LPW, 2>(sy, tb, a, st);
-    if (kpl <= 4) return launch_sweep_k<T, D, LPW, 4>(sy, tb, a, st);
-    if (kpl <= 8) return launch_sweep_k<T, D, LPW, 8>(sy, tb, a, st);
-    return hipErrorInvalidValue;
+    if (kpl == 0) return launch_sweep_k<T, D, LPW, 0, PER>(sy, tb, a, st);
+    if constexpr (!PER) {
+        return hipErrorInvalidValue;   // k-space terms need the periodic box
+    } else {
+        if (kpl <= 2) return launch_sweep_k<T, D, LPW, 2, PER>(sy, tb, a, st);
+        if (kpl <= 4) return launch_sweep_k<T, D, LPW, 4, PER>(sy, tb, a, st);
+        if (kpl <= 8) return launch_sweep_k<T, D, LPW, 8, PER>(sy, tb, a, st);
+        return hipErrorInvalidValue;
+    }
 }
 
 template <typename T, int D>
 static hipError_t launch_sweep_d(const KSys& sy, const KTables<T>& tb, const SweepArgs& a, hipStream_t st)
 {
     const int lpw = a.lanes_per_walker > 0 ? a.lanes_per_walker : pick_lpw(sy.N, sy.K);
+    const bool per = sy.periodic != 0;
     switch (lpw) {
-    case 16: return launch_sweep_l<T, D, 16>(sy, tb, a, st);
-    case 32: return launch_sweep_l<T, D, 32>(sy, tb, a, st);
-    case 64: return launch_sweep_l<T, D, 64>(sy, tb, a, st);
+    case 16: return per ? launch_sweep_l<T, D, 16, true>(sy, tb, a, st) : launch_sweep_l<T, D, 16, false>(sy, tb, a, st);
+    case 32: return per ? launch_sweep_l<T, D, 32, true>(sy, tb, a, st) : launch_sweep_l<T, D, 32, false>(sy, tb, a, st);
+    case 64: return per ? launch_sweep_l<T, D, 64, true>(sy, tb, a, st) : launch_sweep_l<T, D, 64, false>(sy, tb, a, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -116,3 +116,22 @@ def test_sweep_is_deterministic_per_seed():
             ctx.sweep(50, seed=42, precision=0)
             outs.append(ctx.get_walkers())
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_sampling_stage_rho_handoff(precision):
+    """tdvmc_sample takes rho_k from the sweep instead of recomputing it; the
+    stage's moments must equal those of an evaluation of the final walkers."""
+    N, D, L, P, K = 27, 3, 4.0, 8, 26
+    cfg = make_cfg(N=N, D=D, L=L, P_s=P, pot="GAUSSIAN", sysparams=(0.5, 1.0), K=K, hbar2_2m=0.5, W=2048,
+                   step=0.3, therm=N)
+    rng = np.random.default_rng(4)
+    uR = np.concatenate([rng.normal(0, 0.15, P), rng.normal(0, 0.01, K)])
+    with tdvmc.Context(cfg) as ctx:
+        ctx.set_params(uR)
+        st = ctx.sample(seed=3, precision=precision)
+        R = ctx.get_walkers()
+        ev = ctx.eval_configs(R, precision=precision)
+    tol = 1e-12 if precision == 0 else 2e-5
+    assert np.allclose(st["localOperators"], ev["O"].mean(0), rtol=tol, atol=tol)
+    assert abs(st["localEnergyR"] - ev["ER"].mean()) < tol * max(1.0, abs(ev["ER"].mean())) * 10
