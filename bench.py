@@ -31,6 +31,17 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3     # MI355X published vector / matrix FP32 peak
 PEAK_FP64_TFLOPS = 78.6
 
+# HBM bytes per launch of the two hot kernels at 65536 walkers, from one
+# rocprofv3 pass each of FETCH_SIZE and WRITE_SIZE (KiB per dispatch), scaled
+# by what the same counters read for an elementwise kernel that reads and
+# writes 1 GiB (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).  Algorithmic bytes:
+# sweep = coordinates in + out (2 W N D 8) + rho_k out (W K 8) = 1.213 GB,
+# measurement = coordinates + rho_k in + O_k (FP32) and energies out = 0.753 GB.
+MEASURED_TRAFFIC = {
+    ("bulk3d_lj343_k256", 65536): {"sweep": 1.1918e9, "eval": 7.795e8,
+                                   "source": "profiles/traffic_bulk3d_lj343_k256.md"},
+}
+
 
 def sweep_flops_per_move(N, D, K):
     """Algorithmic flops of one trial move: 2 (N-1) pair evaluations of
@@ -42,37 +53,55 @@ def measure_flops(N, D, K):
     return N * (N - 1) / 2 * (8 * D + 29) + N * K * (2 * D + 12) + N * (4 * D + 2)
 
 
-def run_gpu(cfg, steps, warmup, precision, use_graph=True, profile_stages=1):
+def run_gpu(cfg, steps, warmup, precision, use_graph=True):
+    """Times `steps` RK4 TDVP steps.  Kernel durations come from HIP events
+    recorded on the context stream around every sweep / measurement /
+    moments / solve launch of the timed steps themselves (inside the captured
+    graph they are event-record nodes, re-recorded by every replay)."""
     import tdvmc_amd as t
 
     ctx = t.Context(cfg)
     info = ctx.get_info()
     W = info.n_walkers
-    # thermalise the lattice start a little (untimed)
-    ctx.sweep(info.N * 2, seed=11, precision=precision)
+    # thermalise the lattice start a little (untimed; launches of the same
+    # shape as the timed ones, N moves each)
+    for i in range(2):
+        ctx.sweep(info.N, seed=11 + i, precision=precision)
     dt = info.timestep
+    ctx.profile_enable(True)
     for _ in range(warmup):
         ctx.tdvp_step(dt, ode_type=2, precision=precision, use_graph=use_graph)
     ctx.synchronize()
+    ctx.profile_enable(True)          # clears the counters; the graph is kept
     t0 = time.perf_counter()
     for _ in range(steps):
         ctx.tdvp_step(dt, ode_type=2, precision=precision, use_graph=use_graph)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
+    prof = {k: ctx.profile_get(k) for k in ("sweep", "eval", "moments", "solve")}
+    ctx.profile_enable(False)
     stages = 4 * steps
     samples = stages * W * info.samples_per_walker
-    # per-kernel durations with HIP events on the context stream: the same
-    # TDVP step launched eagerly (events cannot be read back inside a graph)
-    ctx.profile_enable(True)
-    for _ in range(profile_stages):
-        ctx.tdvp_step(dt, ode_type=2, precision=precision, use_graph=False)
-    prof = {k: ctx.profile_get(k) for k in ("sweep", "eval", "moments", "solve")}
     acc, trials = ctx.acceptance()
     uR, _, _, _ = ctx.get_params()
     ok = bool(np.all(np.isfinite(uR)))
     ctx.close()
     return dict(elapsed=elapsed, samples=samples, steps=steps, W=W, prof=prof,
                 acceptance=acc / max(trials, 1), finite=ok, info=info)
+
+
+def copy_probe(mib):
+    """A plain elementwise kernel of known size (reads and writes `mib` MiB
+    each, 10 launches), for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(10):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "launches": 10, "bytes_read": n * 4, "bytes_written": n * 4}))
 
 
 def cpu_baseline(budget_s=15.0):
@@ -117,6 +146,12 @@ def cpu_baseline(budget_s=15.0):
                        f"single thread, {el:.1f} s")
 
 
+def kernel_roofline(name, avg_ms, flops, peak):
+    achieved = flops / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": None, "kernel": name, "flops_per_launch": flops, "avg_ms": avg_ms}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,11 +162,16 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain elementwise kernel of this size (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU path is design-only "
                                    "(walker shards + one all-reduce of the moments per stage)"}))
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     from tdvmc_amd.workloads import WORKLOADS
 
@@ -145,12 +185,26 @@ def main():
 
     info = mixed["info"]
     N, D, K = info.N, info.DIM, info.K
+    W = mixed["W"]
     value = mixed["samples"] / mixed["elapsed"]
     ms_step = mixed["elapsed"] / mixed["steps"] * 1e3
-    sw_ms, sw_n = mixed["prof"]["sweep"]
-    sweep_avg_s = sw_ms / max(sw_n, 1) / 1e3
-    flops_launch = mixed["W"] * info.therm_moves * sweep_flops_per_move(N, D, K)
-    achieved = flops_launch / sweep_avg_s / 1e12 if sweep_avg_s > 0 else 0.0
+    avg = {k: (v[0] / max(v[1], 1)) for k, v in mixed["prof"].items()}
+    launches = {k: v[1] for k, v in mixed["prof"].items()}
+    kpl = next(m for m in (0, 2, 4, 8) if m * 64 >= K)   # k-vectors per lane of the sweep template
+    # algorithmic flops per launch: one sweep = N trial moves of every walker,
+    # one measurement = every pair and every (particle, k) term of every walker
+    rl = {
+        "sweep": kernel_roofline(f"sweep_kernel<float, {D}, 64, {kpl}, true>", avg["sweep"],
+                                 W * info.therm_moves * sweep_flops_per_move(N, D, K), PEAK_FP32_TFLOPS),
+        "eval": kernel_roofline(f"eval_kernel<float, {D}, false, true, float>", avg["eval"],
+                                W * measure_flops(N, D, K), PEAK_FP32_TFLOPS),
+    }
+    dom = max(rl, key=lambda k: rl[k]["avg_ms"])
+    tr = MEASURED_TRAFFIC.get((args.workload, W))
+    if tr:
+        for k in rl:
+            rl[k]["traffic"] = tr[k]
+            rl[k]["traffic_source"] = tr["source"]
     line = {
         "metric": "walker-samples/s + TDVP steps/s, N=343 3D, mixed vs FP64 @ S,F rel err 1e-6",
         "value": value,
@@ -165,15 +219,14 @@ def main():
         "dtype": "fp32 pair/spline/phase + bf16x3 MFMA moments + fp64 solve",
         "data": "synthetic (McMillan-initialised spline, seeded k-space amplitudes, jittered-lattice walkers)",
         "config": {"workload": args.workload, "N": N, "DIM": D, "P": info.P, "K": K,
-                   "walkers": mixed["W"], "step": "RK4 TDVP step = 4 x (1 sweep of N moves + measure + "
+                   "walkers": W, "step": "RK4 TDVP step = 4 x (1 sweep of N moves + measure + "
                    "moments + FP64 solve)", "graph": not args.no_graph},
         "tdvp_steps_per_s": mixed["steps"] / mixed["elapsed"],
         "acceptance": mixed["acceptance"],
-        "kernel_ms": {k: (v[0] / max(v[1], 1)) for k, v in mixed["prof"].items()},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "sweep_kernel<float,3,64,4>",
-                     "flops_per_launch": flops_launch},
+        "kernel_ms": avg,
+        "kernel_launches": launches,
+        "roofline": rl[dom],
+        "roofline_other": {k: v for k, v in rl.items() if k != dom},
     }
     if fp64 is not None:
         v64 = fp64["samples"] / fp64["elapsed"]

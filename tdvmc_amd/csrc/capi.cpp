@@ -96,11 +96,17 @@ struct tdvmc_ctx {
     std::string graph_key;
 
     bool prof = false;
+    bool capturing = false;     // building a profiled graph from captured segments
+    bool cap_active = false;    // a stream capture is open
+    hipGraph_t seg_top = nullptr;
+    hipGraphNode_t seg_last = nullptr;
     struct Pending {
         std::string name;
         hipEvent_t a, b;
+        bool owned;   // false: the events belong to the captured graph
     };
     std::vector<Pending> pending;
+    std::vector<Pending> graph_ev;   // event pairs recorded by nodes of the captured graph
     std::map<std::string, Prof> prof_acc;
     int64_t trials_total = 0;
 };
@@ -377,12 +383,60 @@ int ensure_tables(tdvmc_ctx* ctx, DevTables<T>& t, bool force)
     return TDVMC_OK;
 }
 
+// Kernel timing with HIP events on the context stream.  A profiled TDVP step
+// is captured in segments: each timing point closes the open capture, appends
+// it to the step's graph as a child-graph node, appends an event-record node
+// and reopens the capture, so every replay of the graph re-records the events.
+int seg_begin(tdvmc_ctx* ctx)
+{
+    HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->cap_active = true;
+    return TDVMC_OK;
+}
+
+int seg_cut(tdvmc_ctx* ctx)
+{
+    hipGraph_t g = nullptr;
+    ctx->cap_active = false;
+    HIPC(hipStreamEndCapture(ctx->stream, &g));
+    size_t n = 0;
+    hipError_t e = hipGraphGetNodes(g, nullptr, &n);
+    if (e == hipSuccess && n > 0) {
+        hipGraphNode_t node;
+        e = hipGraphAddChildGraphNode(&node, ctx->seg_top, ctx->seg_last ? &ctx->seg_last : nullptr,
+                                      ctx->seg_last ? 1 : 0, g);
+        if (e == hipSuccess) ctx->seg_last = node;
+    }
+    hipGraphDestroy(g);
+    HIPC(e);
+    return TDVMC_OK;
+}
+
+int seg_event(tdvmc_ctx* ctx, hipEvent_t ev)
+{
+    int rc = seg_cut(ctx);
+    if (rc) return rc;
+    hipGraphNode_t node;
+    HIPC(hipGraphAddEventRecordNode(&node, ctx->seg_top, ctx->seg_last ? &ctx->seg_last : nullptr,
+                                    ctx->seg_last ? 1 : 0, ev));
+    ctx->seg_last = node;
+    return seg_begin(ctx);
+}
+
 int prof_begin(tdvmc_ctx* ctx, const char* name, hipEvent_t* a)
 {
     if (!ctx->prof) return TDVMC_OK;
     HIPC(hipEventCreate(a));
-    HIPC(hipEventRecord(*a, ctx->stream));
     (void)name;
+    if (ctx->capturing) {
+        int rc = seg_event(ctx, *a);
+        if (rc) {
+            hipEventDestroy(*a);
+            return rc;
+        }
+        return TDVMC_OK;
+    }
+    HIPC(hipEventRecord(*a, ctx->stream));
     return TDVMC_OK;
 }
 
@@ -391,8 +445,13 @@ int prof_end(tdvmc_ctx* ctx, const char* name, hipEvent_t a)
     if (!ctx->prof) return TDVMC_OK;
     hipEvent_t b;
     HIPC(hipEventCreate(&b));
+    if (ctx->capturing) {
+        // the pair is owned by the graph from here on
+        ctx->graph_ev.push_back({name, a, b, false});
+        return seg_event(ctx, b);
+    }
     HIPC(hipEventRecord(b, ctx->stream));
-    ctx->pending.push_back({name, a, b});
+    ctx->pending.push_back({name, a, b, true});
     return TDVMC_OK;
 }
 
@@ -406,11 +465,27 @@ int prof_flush(tdvmc_ctx* ctx)
         auto& acc = ctx->prof_acc[p.name];
         acc.ms += ms;
         acc.launches++;
-        hipEventDestroy(p.a);
-        hipEventDestroy(p.b);
+        if (p.owned) {
+            hipEventDestroy(p.a);
+            hipEventDestroy(p.b);
+        }
     }
     ctx->pending.clear();
     return TDVMC_OK;
+}
+
+void drop_graph(tdvmc_ctx* ctx)
+{
+    if (ctx->graph) hipGraphExecDestroy(ctx->graph);
+    if (ctx->graph_src) hipGraphDestroy(ctx->graph_src);
+    ctx->graph = nullptr;
+    ctx->graph_src = nullptr;
+    for (auto& p : ctx->graph_ev) {
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    ctx->graph_ev.clear();
+    ctx->graph_key.clear();
 }
 
 int ensure_samples(tdvmc_ctx* ctx, int64_t n)
@@ -687,11 +762,13 @@ void release(tdvmc_ctx* ctx)
 {
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     for (auto& p : ctx->pending) {
-        hipEventDestroy(p.a);
-        hipEventDestroy(p.b);
+        if (p.owned) {
+            hipEventDestroy(p.a);
+            hipEventDestroy(p.b);
+        }
     }
-    if (ctx->graph) hipGraphExecDestroy(ctx->graph);
-    if (ctx->graph_src) hipGraphDestroy(ctx->graph_src);
+    ctx->pending.clear();
+    drop_graph(ctx);
     dfree(ctx->dR);
     dfree(ctx->dRho);
     dfree(ctx->dAcc);
@@ -839,15 +916,9 @@ int tdvmc_set_walker_count(tdvmc_ctx* ctx, int64_t W)
     if (rc) return rc;
     rc = ensure_moments(ctx, W * ctx->spw);
     if (rc) return rc;
-    if (ctx->graph) {
-        hipGraphExecDestroy(ctx->graph);
-        ctx->graph = nullptr;
-    }
-    if (ctx->graph_src) {
-        hipGraphDestroy(ctx->graph_src);
-        ctx->graph_src = nullptr;
-    }
-    ctx->graph_key.clear();
+    rc = prof_flush(ctx);
+    if (rc) return rc;
+    drop_graph(ctx);
     return TDVMC_OK;
 }
 
@@ -1194,30 +1265,55 @@ int tdvmc_tdvp_step(tdvmc_ctx* ctx, double dt, int ode_type, int precision, int 
     CHECK_ARG(ode == 0 || ode == 2, "ODE type must be 0 (Euler) or 2 (RK4)");
     if (use_graph) {
         char key[256];
-        std::snprintf(key, sizeof(key), "%d/%d/%.17g/%lld/%.17g/%.17g", prec, ode, dt, (long long)ctx->W,
-                      ctx->sy.pot_a, ctx->sy.pot_b);
+        std::snprintf(key, sizeof(key), "%d/%d/%.17g/%lld/%.17g/%.17g/%d", prec, ode, dt, (long long)ctx->W,
+                      ctx->sy.pot_a, ctx->sy.pot_b, ctx->prof ? 1 : 0);
         if (!ctx->graph || ctx->graph_key != key) {
-            if (ctx->graph) hipGraphExecDestroy(ctx->graph);
-            if (ctx->graph_src) hipGraphDestroy(ctx->graph_src);
-            ctx->graph = nullptr;
-            ctx->graph_src = nullptr;
-            const bool prof = ctx->prof;
-            ctx->prof = false;
-            HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            rc = enqueue_tdvp_step(ctx, prec, ode, dt);
-            hipGraph_t g = nullptr;
-            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-            ctx->prof = prof;
-            if (rc) {
-                if (g) hipGraphDestroy(g);
-                return rc;
+            rc = prof_flush(ctx);
+            if (rc) return rc;
+            drop_graph(ctx);
+            if (ctx->prof) {
+                // segmented capture with event-record nodes between the segments
+                HIPC(hipGraphCreate(&ctx->seg_top, 0));
+                ctx->seg_last = nullptr;
+                ctx->capturing = true;
+                rc = seg_begin(ctx);
+                if (!rc) rc = enqueue_tdvp_step(ctx, prec, ode, dt);
+                if (!rc) rc = seg_cut(ctx);
+                ctx->capturing = false;
+                if (ctx->cap_active) {
+                    hipGraph_t g = nullptr;
+                    hipStreamEndCapture(ctx->stream, &g);
+                    if (g) hipGraphDestroy(g);
+                    ctx->cap_active = false;
+                }
+                hipGraph_t top = ctx->seg_top;
+                ctx->seg_top = nullptr;
+                ctx->seg_last = nullptr;
+                if (rc) {
+                    hipGraphDestroy(top);
+                    drop_graph(ctx);
+                    return rc;
+                }
+                ctx->graph_src = top;
+            } else {
+                HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                rc = enqueue_tdvp_step(ctx, prec, ode, dt);
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+                if (rc) {
+                    if (g) hipGraphDestroy(g);
+                    drop_graph(ctx);
+                    return rc;
+                }
+                HIPC(e);
+                ctx->graph_src = g;
             }
-            HIPC(e);
-            ctx->graph_src = g;
-            HIPC(hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0));
+            HIPC(hipGraphInstantiate(&ctx->graph, ctx->graph_src, nullptr, nullptr, 0));
             ctx->graph_key = key;
         }
         HIPC(hipGraphLaunch(ctx->graph, ctx->stream));
+        // the graph's event nodes were re-recorded by this replay
+        for (const auto& p : ctx->graph_ev) ctx->pending.push_back(p);
         ctx->trials_total += (ode == 2 ? 4 : 1) * (ctx->init_moves + ctx->spw * ctx->therm_moves) * ctx->W;
     } else {
         rc = enqueue_tdvp_step(ctx, prec, ode, dt);

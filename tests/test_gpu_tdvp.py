@@ -100,3 +100,27 @@ def test_graph_replays_match_eager(therm):
     # last bit of FP64 sums) varies between launches
     assert np.allclose(res[0][0], res[1][0], rtol=1e-11, atol=1e-14)
     assert np.allclose(res[0][1], res[1][1], rtol=1e-12, atol=1e-12)
+
+
+def test_profiled_graph_replays():
+    """Event-record nodes inside the captured step time every replay: one
+    sweep / measurement / moments / solve per RK stage, positive durations,
+    and the same parameters as the unprofiled graph."""
+    cfg, s, R, uR, uI = _setup(1)
+    cfg["MC_NTHERMSTEPS"] = 9
+    cfg["MC_STEP"] = 0.3
+    res = []
+    for prof in (False, True):
+        with tdvmc.Context(cfg) as ctx:
+            ctx.set_params(uR, uI)
+            ctx.set_walkers(R)
+            ctx.profile_enable(prof)
+            for _ in range(3):
+                ctx.tdvp_step(0.01, ode_type=2, precision=tdvmc.FP64, use_graph=True)
+            if prof:
+                for name in ("sweep", "eval", "moments", "solve"):
+                    ms, n = ctx.profile_get(name)
+                    print(name, ms, n, flush=True)
+                    assert n == 12 and ms > 0.0
+            res.append(np.concatenate(ctx.get_params()[:2]))
+    assert np.allclose(res[0], res[1], rtol=1e-11, atol=1e-14)
