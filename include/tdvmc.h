@@ -42,6 +42,7 @@ extern "C" {
 #define TDVMC_ERR_NOT_PD 4
 #define TDVMC_ERR_NONFINITE 5
 #define TDVMC_ERR_STATE 6
+#define TDVMC_ERR_REJECTED 7
 
 #define TDVMC_PRECISION_DEFAULT (-1)
 #define TDVMC_FP64 0
@@ -156,18 +157,30 @@ int tdvmc_eval_configs_dev(tdvmc_ctx* ctx, const double* R_dev, int64_t W, int p
 int tdvmc_moments(tdvmc_ctx* ctx, const double* O, const double* ER, const double* EI, int64_t W, int P,
                   int precision, double* mean, double* S, double* F);
 
-/* SolveForParametersDot, solver type 0 (src/TDVMC.cpp:1713-1762):
- * xdot [2P+2] = (alpha-dot R, alpha-dot I, phi-dot R, phi-dot I);
+/* SolveForParametersDot (src/TDVMC.cpp:1713-1828): solver type 0 (scaling,
+ * S += eps I, Cholesky, :1733-1762) or type 1 (:1763-1822: S += 0.002 I only
+ * with the scaling, solve, subtract the mean of each solution; eps is
+ * ignored).  xdot [2P+2] = (alpha-dot R, alpha-dot I, phi-dot R, phi-dot I);
  * *n_bad = count of non-positive Cholesky pivots (TDVMC_ERR_NOT_PD if > 0). */
 int tdvmc_solve(tdvmc_ctx* ctx, const double* mean, const double* S, const double* F, int P, int imag_time,
-                int precondition, double eps, double* xdot, int* n_bad);
+                int precondition, double eps, int solver_type, double* xdot, int* n_bad);
 
 /* One TDVP time step on the device: CalculateNextParametersEuler (type 0,
  * src/TDVMC.cpp:1834-1853) or CalculateNextParametersRK4 (type 2,
  * src/TDVMC.cpp:1969-2035, fresh samples per stage, chain continued).
  * ode_type < 0 uses the configured ODE_SOLVER_TYPE.  With use_graph != 0 the
- * step is captured once into a hipGraph and replayed. */
+ * step is captured once into a hipGraph and replayed.  After the step
+ * USE_NORMALIZE_WF sets phi R = -exponent of walker 0's last sample and
+ * USE_ADJUST_PARAMETERS subtracts alpha[P-1] from every alpha
+ * (src/TDVMC.cpp:3757-3775, :2525-2567). */
 int tdvmc_tdvp_step(tdvmc_ctx* ctx, double dt, int ode_type, int precision, int use_graph);
+
+/* Non-positive Cholesky pivots met by the solves of the last TDVP step (the
+ * doNotAcceptStep flag of src/TDVMC.cpp:1580-1590).  With
+ * USE_PARAMETER_ACCEPTANCE_CHECK = 1 tdvmc_tdvp_step itself applies
+ * AcceptNewParams (src/TDVMC.cpp:2585-2640) and on rejection restores the
+ * parameters, leaves the time unchanged and returns TDVMC_ERR_REJECTED. */
+int tdvmc_get_step_status(tdvmc_ctx* ctx, int* n_bad);
 
 /* Adjoint of Re ln psi over fed configurations: given gbar [W],
  * dalpha [P] = sum_w gbar_w d(Re ln psi_w)/d(alpha)  (= O_w, plus the count of

@@ -118,6 +118,11 @@ def lib():
         L.tdo_solve_param_dot.restype = C.c_int
         L.tdo_solve_param_dot.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp,
                                           _dp, _dp, _dp, _dp]
+        L.tdo_solve_param_dot_qr.restype = C.c_int
+        L.tdo_solve_param_dot_qr.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp,
+                                             _dp, _dp, _dp, _dp]
+        L.tdo_fpqr_solve.restype = C.c_int
+        L.tdo_fpqr_solve.argtypes = [C.c_int, _dp, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -226,15 +231,32 @@ def cholesky_ref(A):
     return np.tril(A), bad
 
 
-def solve_param_dot(m, imag_time=1, precondition=0, eps=1e-3):
+def solve_param_dot(m, imag_time=1, precondition=0, eps=1e-3, solver_type=0):
+    """Solver type 0 (Cholesky with S += eps I) or type 1 (full-pivot QR,
+    mean-subtracted solution; eps unused).  'bad' = non-positive pivots
+    (type 0) or the rank deficit (type 1)."""
     P = m["mO"].size
     S = _f64(m["S"]).copy()
     xR, xI = np.zeros(P), np.zeros(P)
     pR, pI = C.c_double(0), C.c_double(0)
-    bad = lib().tdo_solve_param_dot(P, int(imag_time), int(precondition), float(eps), _ptr(S), _ptr(_f64(m["mO"])),
-                                    _ptr(_f64(m["mE"])), _ptr(_f64(m["FR"])), _ptr(_f64(m["FI"])),
-                                    _ptr(xR), _ptr(xI), C.byref(pR), C.byref(pI))
+    # the converted arrays are held in locals for the duration of the call
+    # (a pointer into a temporary copy would dangle)
+    mO, mE, FR, FI = _f64(m["mO"]), _f64(m["mE"]), _f64(m["FR"]), _f64(m["FI"])
+    args = (_ptr(S), _ptr(mO), _ptr(mE), _ptr(FR), _ptr(FI), _ptr(xR), _ptr(xI), C.byref(pR), C.byref(pI))
+    if solver_type == 1:
+        bad = lib().tdo_solve_param_dot_qr(P, int(imag_time), int(precondition), *args)
+    else:
+        bad = lib().tdo_solve_param_dot(P, int(imag_time), int(precondition), float(eps), *args)
     return dict(xR=xR, xI=xI, phiDotR=pR.value, phiDotI=pI.value, bad=bad)
+
+
+def fpqr_solve(A, b):
+    """Full-pivot Householder QR solve of A x = b; returns (x, rank)."""
+    A = _f64(A).copy()
+    n = A.shape[0]
+    x = np.zeros(n)
+    rank = lib().tdo_fpqr_solve(n, _ptr(A), _ptr(_f64(b).copy()), _ptr(x))
+    return x, rank
 
 
 # ---------------------------------------------------------------- helpers

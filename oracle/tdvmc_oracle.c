@@ -651,3 +651,155 @@ int tdo_solve_param_dot(int P, int imag_time, int precondition, double eps, doub
     free(bR); free(bI); free(sc);
     return bad;
 }
+
+/* Householder QR with full (row and column) pivoting, the decomposition
+ * Eigen::FullPivHouseholderQR (Eigen 3.3, vendored under resources/) applies
+ * in solver type 1 (src/TDVMC.cpp:1786-1796); algorithm of Golub & Van Loan,
+ * "Matrix Computations", 4th ed., sec. 5.4.2, with Eigen's conventions:
+ *  - pivot = entry of largest magnitude of the trailing block, the first one
+ *    in column-major order on ties;
+ *  - the factorisation stops when that entry is <= n * eps * (first pivot);
+ *  - the row swap exchanges the trailing part of the two rows, the column
+ *    swap whole columns;
+ *  - reflector H = I - tau v v^T, v = (1, essential), beta = -sign(c0) |x|
+ *    (c0 >= 0 -> negative), essential = tail / (c0 - beta),
+ *    tau = (beta - c0) / beta; a zero tail gives tau = 0;
+ *  - rank = number of |R_kk| above n * eps * max_k |R_kk|; the solve applies
+ *    the first rank row swaps and reflectors to b, back-substitutes on the
+ *    leading rank x rank block of R, sets the rest to zero and undoes the
+ *    column swaps.
+ * A is n x n row-major (unchanged).  Returns the rank. */
+int tdo_fpqr_solve(int n, const double* A_in, const double* b_in, double* x)
+{
+    const double eps = 2.220446049250313e-16, prec = n * eps;
+    double* A = malloc(sizeof(double) * n * n);
+    double* c = malloc(sizeof(double) * n);
+    double* tau = malloc(sizeof(double) * n);
+    int* rt = malloc(sizeof(int) * n);
+    int* ct = malloc(sizeof(int) * n);
+    memcpy(A, A_in, sizeof(double) * n * n);
+    double biggest = 0.0, maxpivot = 0.0;
+    int nz = n;
+    for (int k = 0; k < n; k++) {
+        int bi = k, bj = k;
+        double big = -1.0;
+        for (int j = k; j < n; j++)          /* column-major scan, strict > */
+            for (int i = k; i < n; i++)
+                if (fabs(A[i * n + j]) > big) {
+                    big = fabs(A[i * n + j]);
+                    bi = i;
+                    bj = j;
+                }
+        if (k == 0) biggest = big;
+        if (big <= biggest * prec) {
+            nz = k;
+            for (int i = k; i < n; i++) {
+                rt[i] = i;
+                ct[i] = i;
+                tau[i] = 0.0;
+            }
+            break;
+        }
+        rt[k] = bi;
+        ct[k] = bj;
+        if (bi != k)
+            for (int j = k; j < n; j++) {
+                double t = A[k * n + j]; A[k * n + j] = A[bi * n + j]; A[bi * n + j] = t;
+            }
+        if (bj != k)
+            for (int i = 0; i < n; i++) {
+                double t = A[i * n + k]; A[i * n + k] = A[i * n + bj]; A[i * n + bj] = t;
+            }
+        double tail = 0.0;
+        for (int i = k + 1; i < n; i++) tail += A[i * n + k] * A[i * n + k];
+        const double c0 = A[k * n + k];
+        double beta, tk;
+        if (tail <= 2.2250738585072014e-308) {
+            tk = 0.0;
+            beta = c0;
+            for (int i = k + 1; i < n; i++) A[i * n + k] = 0.0;
+        } else {
+            beta = sqrt(c0 * c0 + tail);
+            if (c0 >= 0.0) beta = -beta;
+            for (int i = k + 1; i < n; i++) A[i * n + k] /= (c0 - beta);
+            tk = (beta - c0) / beta;
+        }
+        A[k * n + k] = beta;
+        tau[k] = tk;
+        if (fabs(beta) > maxpivot) maxpivot = fabs(beta);
+        /* apply H to the trailing columns: w = row k + essential^T bottom */
+        for (int j = k + 1; j < n; j++) {
+            double w = A[k * n + j];
+            for (int i = k + 1; i < n; i++) w += A[i * n + k] * A[i * n + j];
+            A[k * n + j] -= tk * w;
+            for (int i = k + 1; i < n; i++) A[i * n + j] -= (tk * A[i * n + k]) * w;
+        }
+    }
+    int rank = 0;
+    for (int i = 0; i < nz; i++) rank += fabs(A[i * n + i]) > maxpivot * prec;
+    memcpy(c, b_in, sizeof(double) * n);
+    for (int k = 0; k < rank; k++) {
+        double t = c[k]; c[k] = c[rt[k]]; c[rt[k]] = t;
+        double w = c[k];
+        for (int i = k + 1; i < n; i++) w += A[i * n + k] * c[i];
+        c[k] -= tau[k] * w;
+        for (int i = k + 1; i < n; i++) c[i] -= (tau[k] * A[i * n + k]) * w;
+    }
+    for (int i = rank - 1; i >= 0; i--) {
+        double s = c[i];
+        for (int j = i + 1; j < rank; j++) s -= A[i * n + j] * c[j];
+        c[i] = s / A[i * n + i];
+    }
+    for (int i = rank; i < n; i++) c[i] = 0.0;
+    /* x = Pi y with Pi = t_0 t_1 ... t_{n-1}: apply the transpositions last first */
+    for (int k = n - 1; k >= 0; k--) {
+        double t = c[k]; c[k] = c[ct[k]]; c[ct[k]] = t;
+    }
+    memcpy(x, c, sizeof(double) * n);
+    free(A); free(c); free(tau); free(rt); free(ct);
+    return rank;
+}
+
+/* Solver type 1 of src/TDVMC.cpp:1763-1822: optional diagonal scaling with
+ * S += 0.002 I (only then), full-pivot QR solves, subtraction of the mean of
+ * each solution vector (:1800-1809), phi-dot on that vector, then the scaling
+ * is undone.  S is overwritten.  Returns P - rank. */
+int tdo_solve_param_dot_qr(int P, int imag_time, int precondition, double* S, const double* mO, const double* mE,
+                           const double* FR, const double* FI, double* xR, double* xI, double* phiDotR,
+                           double* phiDotI)
+{
+    double* bR = malloc(sizeof(double) * P);
+    double* bI = malloc(sizeof(double) * P);
+    double* sc = malloc(sizeof(double) * P);
+    tdo_build_rhs(P, imag_time, mO, mE, FR, FI, bR, bI);
+    if (precondition) {
+        for (int i = 0; i < P; i++) sc[i] = sqrt(S[i * P + i]);
+        for (int i = 0; i < P; i++) {
+            for (int j = 0; j < P; j++) S[i * P + j] /= (sc[i] * sc[j]);
+            bR[i] /= sc[i];
+            bI[i] /= sc[i];
+        }
+        for (int i = 0; i < P; i++) S[i * P + i] += 0.002;
+    }
+    int rank = tdo_fpqr_solve(P, S, bR, xR);
+    tdo_fpqr_solve(P, S, bI, xI);
+    double mr = 0.0, mi = 0.0;
+    for (int i = 0; i < P; i++) {
+        mr += xR[i];
+        mi += xI[i];
+    }
+    mr /= P;
+    mi /= P;
+    for (int i = 0; i < P; i++) {
+        xR[i] -= mr;
+        xI[i] -= mi;
+    }
+    tdo_phi_dot(P, imag_time, mO, mE[0], xR, xI, phiDotR, phiDotI);
+    if (precondition)
+        for (int i = 0; i < P; i++) {
+            xR[i] /= sc[i];
+            xI[i] /= sc[i];
+        }
+    free(bR); free(bI); free(sc);
+    return P - rank;
+}

@@ -28,7 +28,7 @@ FP64 = 0
 MIXED = 1
 DEFAULT = -1
 
-OK, ERR_BAD_ARG, ERR_BAD_CONFIG, ERR_HIP, ERR_NOT_PD, ERR_NONFINITE, ERR_STATE = range(7)
+OK, ERR_BAD_ARG, ERR_BAD_CONFIG, ERR_HIP, ERR_NOT_PD, ERR_NONFINITE, ERR_STATE, ERR_REJECTED = range(8)
 
 _dp = C.POINTER(C.c_double)
 _lib = None
@@ -82,8 +82,9 @@ SYMBOLS = {
     "tdvmc_eval_configs_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "tdvmc_moments": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int64, C.c_int, C.c_int, _dp, _dp, _dp]),
-    "tdvmc_solve": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double, _dp,
+    "tdvmc_solve": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp,
                               C.POINTER(C.c_int)]),
+    "tdvmc_get_step_status": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "tdvmc_tdvp_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int]),
     "tdvmc_adjoint": (C.c_int, [C.c_void_p, _dp, C.c_int64, _dp, C.c_int, _dp, _dp]),
     "tdvmc_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
@@ -276,20 +277,26 @@ class Context:
                                         _ptr(mean), _ptr(S), _ptr(F)))
         return dict(mO=mean[:P], mE=mean[P:], S=S, FR=F[0], FI=F[1])
 
-    def solve(self, mO, mE, S, FR, FI, imag_time=1, precondition=0, eps=1e-3, check=True):
+    def solve(self, mO, mE, S, FR, FI, imag_time=1, precondition=0, eps=1e-3, check=True, solver_type=0):
         P = len(mO)
         mean = _f64(np.concatenate([mO, mE]))
         F = _f64(np.stack([FR, FI]))
         xdot = np.zeros(2 * P + 2)
         bad = C.c_int()
         rc = lib().tdvmc_solve(self._h, _ptr(mean), _ptr(_f64(S)), _ptr(F), P, int(imag_time), int(precondition),
-                               float(eps), _ptr(xdot), C.byref(bad))
+                               float(eps), int(solver_type), _ptr(xdot), C.byref(bad))
         if rc not in (OK, ERR_NOT_PD) or (check and rc != OK):
             self._check(rc)
         return dict(xR=xdot[:P], xI=xdot[P:2 * P], phiDotR=xdot[2 * P], phiDotI=xdot[2 * P + 1], bad=bad.value)
 
     def tdvp_step(self, dt: float, ode_type: int = -1, precision: int = DEFAULT, use_graph: bool = False):
         self._check(lib().tdvmc_tdvp_step(self._h, float(dt), int(ode_type), int(precision), int(use_graph)))
+
+    def step_status(self) -> int:
+        """Non-positive Cholesky pivots met by the last TDVP step."""
+        bad = C.c_int()
+        self._check(lib().tdvmc_get_step_status(self._h, C.byref(bad)))
+        return bad.value
 
     def adjoint(self, R, gbar, precision: int = DEFAULT):
         R = _f64(R).reshape(-1, self.N, self.D)

@@ -47,7 +47,11 @@ struct tdvmc_ctx {
     int step = 0;
     int precision = TDVMC_FP64;
     int ode_type = 0, imag_time = 0, precondition = 0, spw = 1;
+    int solver_type = 0;              // LINEAR_EQUATION_SOLVER_TYPE (0 Cholesky, 1 + mean subtraction)
+    int normalize_wf = 0, adjust_params = 0;
+    int accept_check = 0, accept_type = 0;
     double reg_eps = 1e-3, dt = 0.0;
+    double* dXprev = nullptr;         // parameters before the step (acceptance check)
     int64_t therm_moves = 0, init_moves = 0;
     uint64_t seed = 1;
     int lanes = 0;
@@ -224,6 +228,15 @@ int build_system(tdvmc_ctx* ctx)
         ctx->err = "USE_PARAM_START/END: only the full parameter set is implemented";
         return TDVMC_ERR_BAD_CONFIG;
     }
+    if (c.LINEAR_EQUATION_SOLVER_TYPE != 0 && c.LINEAR_EQUATION_SOLVER_TYPE != 1) {
+        ctx->err = "LINEAR_EQUATION_SOLVER_TYPE must be 0 or 1";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    if (c.USE_PARAMETER_ACCEPTANCE_CHECK == 1 &&
+        (c.PARAMETER_ACCEPTANCE_CHECK_TYPE < 0 || c.PARAMETER_ACCEPTANCE_CHECK_TYPE > 2)) {
+        ctx->err = "PARAMETER_ACCEPTANCE_CHECK_TYPE must be 0, 1 or 2";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
     s.N = c.N;
     s.D = c.DIM;
     s.P_s = c.N_PARAM;
@@ -285,6 +298,11 @@ int build_system(tdvmc_ctx* ctx)
     ctx->imag_time = c.IMAGINARY_TIME;
     ctx->precondition = c.USE_PRECONDITIONING;
     ctx->reg_eps = c.REG_EPS > 0 ? c.REG_EPS : 1e-3;
+    ctx->solver_type = c.LINEAR_EQUATION_SOLVER_TYPE;
+    ctx->normalize_wf = c.USE_NORMALIZE_WF == 1;
+    ctx->adjust_params = c.USE_ADJUST_PARAMETERS == 1;
+    ctx->accept_check = c.USE_PARAMETER_ACCEPTANCE_CHECK == 1;
+    ctx->accept_type = c.PARAMETER_ACCEPTANCE_CHECK_TYPE;
     ctx->dt = c.TIMESTEP;
     ctx->therm_moves = c.MC_NTHERMSTEPS > 0 ? c.MC_NTHERMSTEPS : 0;
     ctx->init_moves = c.MC_NINITIALIZATIONSTEPS > 0 ? c.MC_NINITIALIZATIONSTEPS : 0;
@@ -606,7 +624,10 @@ int enqueue_solve(tdvmc_ctx* ctx, double* xdot)
     a.P = P_of(ctx);
     a.imag_time = ctx->imag_time;
     a.precondition = ctx->precondition;
-    a.eps = ctx->reg_eps;
+    // type 0: S += eps I (0.001 in src/TDVMC.cpp:1737, configurable here);
+    // type 1: 0.002 only together with the scaling (:1768-1772)
+    a.eps = ctx->solver_type == 1 ? (ctx->precondition ? 0.002 : 0.0) : ctx->reg_eps;
+    a.mean_sub = ctx->solver_type == 1;
     a.mean = ctx->dMean;
     a.S = ctx->dS;
     a.F = ctx->dF;
@@ -657,6 +678,8 @@ int enqueue_tdvp_step(tdvmc_ctx* ctx, int prec, int ode, double dt)
 {
     const int P = P_of(ctx);
     int rc;
+    // the solves of this step add their non-positive pivots to dStatus
+    HIPC(launch_fill_u32(reinterpret_cast<uint32_t*>(ctx->dStatus), 0u, 1, ctx->stream));
     if (ode == 0) {
         rc = enqueue_rhs(ctx, prec, ctx->dX, ctx->dK[0]);
         if (rc) return rc;
@@ -678,8 +701,29 @@ int enqueue_tdvp_step(tdvmc_ctx* ctx, int prec, int ode, double dt)
         ctx->err = "ODE_SOLVER_TYPE " + std::to_string(ode) + " is not implemented (0 Euler, 2 RK4)";
         return TDVMC_ERR_BAD_ARG;
     }
+    // src/TDVMC.cpp:3757-3775: phi R := -(exponent of the root rank's walker,
+    // here walker 0 at the last sample), then the gauge shift
+    if (ctx->normalize_wf || ctx->adjust_params)
+        HIPC(launch_post_step(P, ctx->dX, ctx->dExpo + (int64_t)(ctx->spw - 1) * ctx->W, ctx->normalize_wf,
+                              ctx->adjust_params, ctx->stream));
     HIPC(launch_unpack_params(P, ctx->dX, ctx->dUR, ctx->dUI, ctx->stream));
     return TDVMC_OK;
+}
+
+// AcceptNewParams (src/TDVMC.cpp:2585-2640): type 0 rejects only a step whose
+// Cholesky met a non-positive pivot; type 1 also non-finite parameters;
+// type 2 also relative changes above 0.1.  The checks on the parameters apply
+// from the second step on (:2599, :2613).
+bool accept_step(const tdvmc_ctx* ctx, int n_bad, const std::vector<double>& x, const std::vector<double>& x0)
+{
+    if (n_bad > 0) return false;
+    if (ctx->accept_type == 0 || ctx->step < 1) return true;
+    const int P = ctx->sy.P;
+    for (int p = 0; p < P; p++) {
+        if (!std::isfinite(x[p])) return false;
+        if (ctx->accept_type == 2 && std::fabs(1.0 - x[p] / x0[p]) > 0.1) return false;
+    }
+    return true;
 }
 
 int fill_step_data(tdvmc_ctx* ctx, int64_t n, tdvmc_step_data* out)
@@ -733,6 +777,7 @@ int alloc_ctx(tdvmc_ctx* ctx)
     HIPC(dalloc(&ctx->dUI, P));
     HIPC(dalloc(&ctx->dX, 2 * P + 2));
     HIPC(dalloc(&ctx->dXtmp, 2 * P + 2));
+    HIPC(dalloc(&ctx->dXprev, 2 * P + 2));
     for (int i = 0; i < 4; i++) HIPC(dalloc(&ctx->dK[i], 2 * P + 2));
     HIPC(dalloc(&ctx->dEpoch, 1));
     HIPC(launch_fill_u32(ctx->dEpoch, 0u, 1, ctx->stream));
@@ -777,6 +822,7 @@ void release(tdvmc_ctx* ctx)
     dfree(ctx->dUI);
     dfree(ctx->dX);
     dfree(ctx->dXtmp);
+    dfree(ctx->dXprev);
     for (auto& k : ctx->dK) dfree(k);
     dfree(ctx->tf.spl);
     dfree(ctx->tf.kv);
@@ -1216,10 +1262,11 @@ int tdvmc_moments(tdvmc_ctx* ctx, const double* O, const double* ER, const doubl
 }
 
 int tdvmc_solve(tdvmc_ctx* ctx, const double* mean, const double* S, const double* F, int P, int imag_time,
-                int precondition, double eps, double* xdot, int* n_bad)
+                int precondition, double eps, int solver_type, double* xdot, int* n_bad)
 {
     if (!ctx) return TDVMC_ERR_BAD_ARG;
     CHECK_ARG(mean && S && F && xdot && P >= 1, "bad solve arguments");
+    CHECK_ARG(solver_type == 0 || solver_type == 1, "solver type must be 0 or 1");
     double *dM = nullptr, *dS = nullptr, *dF = nullptr, *dW = nullptr, *dX = nullptr;
     int* dSt = nullptr;
     hipError_t e = hipSuccess;
@@ -1236,8 +1283,11 @@ int tdvmc_solve(tdvmc_ctx* ctx, const double* mean, const double* S, const doubl
     step(hipMemcpyAsync(dS, S, (size_t)P * P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     step(hipMemcpyAsync(dF, F, 2 * (size_t)P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     int bad = 0;
+    if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(dSt), 0u, 1, ctx->stream);
     if (e == hipSuccess) {
-        SolveArgs a{P, imag_time, precondition, eps, dM, dS, dF, dW, dX, dSt};
+        // type 1 regularises only together with the scaling (src/TDVMC.cpp:1766-1771)
+        const double e_eff = solver_type == 1 ? (precondition ? 0.002 : 0.0) : eps;
+        SolveArgs a{P, imag_time, precondition, e_eff, dM, dS, dF, dW, dX, dSt, solver_type == 1 ? 1 : 0};
         e = launch_solve(a, ctx->stream);
     }
     step(hipMemcpyAsync(xdot, dX, (2 * (size_t)P + 2) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -1311,20 +1361,56 @@ int tdvmc_tdvp_step(tdvmc_ctx* ctx, double dt, int ode_type, int precision, int 
             HIPC(hipGraphInstantiate(&ctx->graph, ctx->graph_src, nullptr, nullptr, 0));
             ctx->graph_key = key;
         }
+        if (ctx->accept_check)
+            HIPC(hipMemcpyAsync(ctx->dXprev, ctx->dX, (2 * (size_t)P_of(ctx) + 2) * sizeof(double),
+                                hipMemcpyDeviceToDevice, ctx->stream));
         HIPC(hipGraphLaunch(ctx->graph, ctx->stream));
         // the graph's event nodes were re-recorded by this replay
         for (const auto& p : ctx->graph_ev) ctx->pending.push_back(p);
         ctx->trials_total += (ode == 2 ? 4 : 1) * (ctx->init_moves + ctx->spw * ctx->therm_moves) * ctx->W;
     } else {
+        if (ctx->accept_check)
+            HIPC(hipMemcpyAsync(ctx->dXprev, ctx->dX, (2 * (size_t)P_of(ctx) + 2) * sizeof(double),
+                                hipMemcpyDeviceToDevice, ctx->stream));
         rc = enqueue_tdvp_step(ctx, prec, ode, dt);
         if (rc) return rc;
     }
     ctx->tf.dirty = ctx->td.dirty = true;
     ctx->host_params_stale = true;
+    rc = prof_flush(ctx);
+    if (rc) return rc;
+    if (ctx->accept_check) {
+        const size_t n = 2 * (size_t)P_of(ctx) + 2;
+        std::vector<double> x(n), x0(n);
+        int bad = 0;
+        HIPC(hipMemcpyAsync(x.data(), ctx->dX, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(hipMemcpyAsync(x0.data(), ctx->dXprev, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(hipMemcpyAsync(&bad, ctx->dStatus, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPC(hipStreamSynchronize(ctx->stream));
+        if (!accept_step(ctx, bad, x, x0)) {
+            // roll back to the parameters before the step; time does not advance
+            HIPC(hipMemcpyAsync(ctx->dX, ctx->dXprev, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            HIPC(launch_unpack_params(P_of(ctx), ctx->dX, ctx->dUR, ctx->dUI, ctx->stream));
+            HIPC(hipStreamSynchronize(ctx->stream));
+            ctx->err = bad > 0 ? "step rejected: non-positive pivot in the Cholesky factorisation"
+                               : "step rejected by the parameter acceptance check";
+            return TDVMC_ERR_REJECTED;
+        }
+    }
     ctx->time += dt;
     ctx->step += 1;
     resolve_potential(ctx);
-    return prof_flush(ctx);
+    return TDVMC_OK;
+}
+
+int tdvmc_get_step_status(tdvmc_ctx* ctx, int* n_bad)
+{
+    if (!ctx || !n_bad) return TDVMC_ERR_BAD_ARG;
+    int bad = 0;
+    HIPC(hipMemcpyAsync(&bad, ctx->dStatus, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    *n_bad = bad;
+    return TDVMC_OK;
 }
 
 int tdvmc_adjoint(tdvmc_ctx* ctx, const double* R, int64_t W, const double* gbar, int precision, double* dR,

@@ -65,10 +65,13 @@ struct SolveArgs {
     const double* F;           // [2][P]
     double* work;              // [P][P] scratch (factor), + 4P
     double* xdot;              // [2P+2] out: alpha-dot R, alpha-dot I, phi-dot R, phi-dot I
-    int* status;               // [1] count of non-positive pivots
+    int* status;               // [1] count of non-positive pivots (added to)
+    int mean_sub;              // 1: subtract the mean of each solution vector (solver type 1)
 };
 hipError_t launch_solve(const SolveArgs& a, hipStream_t st);
 
+// after a step: alpha -= alpha[P-1] (adjust), phi R = -exponent (normalize)
+hipError_t launch_post_step(int P, double* x, const double* expo, int normalize, int adjust, hipStream_t st);
 // y = x0 + c * xdot  over the 2P+2 entries (alpha R, alpha I, phi R, phi I)
 hipError_t launch_axpy_params(int P, const double* x0, const double* xdot, double c, double* y, hipStream_t st);
 // x += dt/6 (k1 + 2 k2 + 2 k3 + k4)

@@ -66,6 +66,22 @@ __global__ void unpack_params_kernel(int P, const double* packed, double* uR, do
     }
 }
 
+// AdjustParameters (src/TDVMC.cpp:2531-2567: u_i -= u_{P-1}, both parts) and
+// NormalizeWavefunction (:2525-2529 with the exponent of :3763: phi R = -exponent)
+__global__ void post_step_kernel(int P, double* x, const double* expo, int normalize, int adjust)
+{
+    const double lastR = x[P - 1], lastI = x[2 * P - 1];
+    const double e = normalize ? expo[0] : 0.0;
+    __syncthreads();
+    if (adjust) {
+        for (int i = threadIdx.x; i < P; i += blockDim.x) {
+            x[i] -= lastR;
+            x[P + i] -= lastI;
+        }
+    }
+    if (normalize && threadIdx.x == 0) x[2 * P] = -e;
+}
+
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int64_t n)
 {
     for (int64_t i = threadIdx.x + (int64_t)blockIdx.x * blockDim.x; i < n; i += (int64_t)blockDim.x * gridDim.x)
@@ -154,6 +170,12 @@ hipError_t launch_rk4_combine(int P, double* x, const double* k1, const double* 
 hipError_t launch_unpack_params(int P, const double* packed, double* uR, double* uI, hipStream_t st)
 {
     hipLaunchKernelGGL(unpack_params_kernel, dim3((P + 255) / 256), dim3(256), 0, st, P, packed, uR, uI);
+    return hipGetLastError();
+}
+
+hipError_t launch_post_step(int P, double* x, const double* expo, int normalize, int adjust, hipStream_t st)
+{
+    hipLaunchKernelGGL(post_step_kernel, dim3(1), dim3(256), 0, st, P, x, expo, normalize, adjust);
     return hipGetLastError();
 }
 

@@ -7,7 +7,7 @@
 // non-positive pivot is counted and leaves the diagonal entry at its
 // unreduced value), :1594-1622 (forward / back substitution), :1658-1682
 // (phi-dot, computed on the scaled solution before the scaling is undone, as
-// in :1749-1758).
+// in :1749-1758).  Solver type 1 is the full-pivot QR kernel further down.
 //
 // Blocked right-looking Cholesky: panels of NB columns are staged in LDS
 // (transposed, so consecutive threads read consecutive rows), factored there
@@ -247,7 +247,258 @@ __global__ __launch_bounds__(kThreads) void solve_kernel(SolveArgs a)
             pr -= eR;
         a.xdot[2 * P] = pr;
         a.xdot[2 * P + 1] = pi;
-        if (a.status) a.status[0] = nbad;
+        if (a.status) a.status[0] += nbad;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Solver type 1 (src/TDVMC.cpp:1763-1822): Householder QR with full pivoting,
+// the decomposition of Eigen::FullPivHouseholderQR, with its conventions
+// (pivot = largest magnitude of the trailing block, first in column-major
+// order on ties; stop when it falls to n eps of the first pivot; trailing-row
+// and whole-column swaps; v = (1, tail / (c0 - beta)), tau = (beta - c0) /
+// beta, beta = -sign(c0) |x|; rank from n eps max |R_kk|), then the mean of
+// each solution vector is subtracted.  For an S made singular by the exactly
+// equal O of k and -k this picks the same solution as the reference's
+// decomposition; Cholesky would not.  The matrix is held column-major in the
+// work buffer, one thread per column for the reflector updates.
+
+__device__ void argmax_reduce(double& v, int& e, double* rv, int* re)
+{
+    // larger value wins, equal values: smaller column-major index
+    for (int o = 32; o > 0; o >>= 1) {
+        const double v2 = __shfl_xor(v, o, 64);
+        const int e2 = __shfl_xor(e, o, 64);
+        if (v2 > v || (v2 == v && e2 < e)) {
+            v = v2;
+            e = e2;
+        }
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        rv[wv] = v;
+        re[wv] = e;
+    }
+    __syncthreads();
+    v = rv[0];
+    e = re[0];
+    for (int w = 1; w < kThreads / 64; w++)
+        if (rv[w] > v || (rv[w] == v && re[w] < e)) {
+            v = rv[w];
+            e = re[w];
+        }
+}
+
+__global__ __launch_bounds__(kThreads) void solve_qr_kernel(SolveArgs a)
+{
+    const int P = a.P, tid = threadIdx.x;
+    double* Q = a.work;                  // column-major: Q[j * P + i] = A[i][j]
+    double* bR = a.work + (size_t)P * P;
+    double* bI = bR + P;
+    double* sc = bI + P;
+    double* tau = sc + P;
+    extern __shared__ __align__(16) unsigned char smem[];
+    int* rt = reinterpret_cast<int*>(smem);   // row transpositions [P]
+    int* ct = rt + P;                          // column transpositions [P]
+    __shared__ double red[kThreads / 64];
+    __shared__ double rv[kThreads / 64];
+    __shared__ int re[kThreads / 64];
+    const double* mO = a.mean;
+    const double eR = a.mean[P], eI = a.mean[P + 1];
+    const double* FR = a.F;
+    const double* FI = a.F + P;
+    const double prec = P * 2.220446049250313e-16;
+
+    for (int i = tid; i < P; i += kThreads) {
+        if (a.imag_time == 0) {
+            bR[i] = FI[i];
+            bI[i] = -FR[i];
+        } else {
+            bR[i] = -FR[i];
+            bI[i] = -(FI[i] + mO[i] * eI);
+        }
+        sc[i] = a.precondition ? sqrt(a.S[(size_t)i * P + i]) : 1.0;
+    }
+    __syncthreads();
+    for (size_t e = tid; e < (size_t)P * P; e += kThreads) {
+        const int i = (int)(e % P), j = (int)(e / P);
+        double v = a.S[(size_t)i * P + j];
+        if (a.precondition) v /= (sc[i] * sc[j]);
+        if (i == j) v += a.eps;
+        Q[e] = v;
+    }
+    for (int i = tid; i < P; i += kThreads) {
+        if (a.precondition) {
+            bR[i] /= sc[i];
+            bI[i] /= sc[i];
+        }
+    }
+    __syncthreads();
+
+    double biggest = 0.0, maxpivot = 0.0;
+    int nz = P;
+    for (int k = 0; k < P; k++) {
+        const int m = P - k;
+        double bv = -1.0;
+        int be = 0x7fffffff;
+        for (int e = tid; e < m * m; e += kThreads) {
+            const int jj = e / m, ii = e - jj * m;   // e increases in column-major order
+            const double v = fabs(Q[(size_t)(k + jj) * P + k + ii]);
+            if (v > bv) {
+                bv = v;
+                be = e;
+            }
+        }
+        argmax_reduce(bv, be, rv, re);
+        if (k == 0) biggest = bv;
+        if (bv <= biggest * prec) {
+            nz = k;
+            break;
+        }
+        const int bj = k + be / m, bi = k + be % m;
+        if (tid == 0) {
+            rt[k] = bi;
+            ct[k] = bj;
+        }
+        if (bi != k)
+            for (int j = k + tid; j < P; j += kThreads) {
+                const double t = Q[(size_t)j * P + k];
+                Q[(size_t)j * P + k] = Q[(size_t)j * P + bi];
+                Q[(size_t)j * P + bi] = t;
+            }
+        __syncthreads();
+        if (bj != k)
+            for (int i = tid; i < P; i += kThreads) {
+                const double t = Q[(size_t)k * P + i];
+                Q[(size_t)k * P + i] = Q[(size_t)bj * P + i];
+                Q[(size_t)bj * P + i] = t;
+            }
+        __syncthreads();
+        double* col = Q + (size_t)k * P;
+        double tl = 0.0;
+        for (int i = k + 1 + tid; i < P; i += kThreads) tl += col[i] * col[i];
+        tl = block_reduce(tl, red);
+        const double c0 = col[k];
+        double beta, tk;
+        if (tl <= 2.2250738585072014e-308) {
+            tk = 0.0;
+            beta = c0;
+            for (int i = k + 1 + tid; i < P; i += kThreads) col[i] = 0.0;
+        } else {
+            beta = sqrt(c0 * c0 + tl);
+            if (c0 >= 0.0) beta = -beta;
+            const double den = c0 - beta;
+            for (int i = k + 1 + tid; i < P; i += kThreads) col[i] /= den;
+            tk = (beta - c0) / beta;
+        }
+        maxpivot = fmax(maxpivot, fabs(beta));
+        __syncthreads();
+        if (tid == 0) {
+            col[k] = beta;
+            tau[k] = tk;
+        }
+        // H applied to the trailing columns, one thread per column
+        for (int j = k + 1 + tid; j < P; j += kThreads) {
+            double* cj = Q + (size_t)j * P;
+            double w = cj[k];
+            for (int i = k + 1; i < P; i++) w += col[i] * cj[i];
+            cj[k] -= tk * w;
+            for (int i = k + 1; i < P; i++) cj[i] -= (tk * col[i]) * w;
+        }
+        __syncthreads();
+    }
+    for (int k = nz + tid; k < P; k += kThreads) {
+        rt[k] = k;
+        ct[k] = k;
+    }
+    __syncthreads();
+    int rank = 0;
+    for (int i = 0; i < nz; i++) rank += fabs(Q[(size_t)i * P + i]) > maxpivot * prec;
+
+    // c = Q^T (row-swapped) b for the first rank reflectors
+    for (int k = 0; k < rank; k++) {
+        if (tid == 0 && rt[k] != k) {
+            double t = bR[k]; bR[k] = bR[rt[k]]; bR[rt[k]] = t;
+            t = bI[k]; bI[k] = bI[rt[k]]; bI[rt[k]] = t;
+        }
+        __syncthreads();
+        const double* col = Q + (size_t)k * P;
+        const double ck_r = bR[k], ck_i = bI[k];   // read before the barriers of the reductions
+        double wr = 0.0, wi = 0.0;
+        for (int i = k + 1 + tid; i < P; i += kThreads) {
+            wr += col[i] * bR[i];
+            wi += col[i] * bI[i];
+        }
+        wr = ck_r + block_reduce(wr, red);
+        wi = ck_i + block_reduce(wi, red);
+        const double tk = tau[k];
+        for (int i = k + 1 + tid; i < P; i += kThreads) {
+            bR[i] -= (tk * col[i]) * wr;
+            bI[i] -= (tk * col[i]) * wi;
+        }
+        if (tid == 0) {
+            bR[k] -= tk * wr;
+            bI[k] -= tk * wi;
+        }
+        __syncthreads();
+    }
+    // R z = c on the leading rank x rank block (column-oriented), zero beyond
+    for (int i = rank - 1; i >= 0; i--) {
+        const double rii = Q[(size_t)i * P + i];
+        const double zr = bR[i] / rii, zi = bI[i] / rii;
+        __syncthreads();
+        for (int l = tid; l < i; l += kThreads) {
+            const double r = Q[(size_t)i * P + l];
+            bR[l] -= r * zr;
+            bI[l] -= r * zi;
+        }
+        if (tid == 0) {
+            bR[i] = zr;
+            bI[i] = zi;
+        }
+        __syncthreads();
+    }
+    for (int i = rank + tid; i < P; i += kThreads) {
+        bR[i] = 0.0;
+        bI[i] = 0.0;
+    }
+    __syncthreads();
+    if (tid == 0)
+        for (int k = P - 1; k >= 0; k--) {
+            const int c = ct[k];
+            if (c != k) {
+                double t = bR[k]; bR[k] = bR[c]; bR[c] = t;
+                t = bI[k]; bI[k] = bI[c]; bI[c] = t;
+            }
+        }
+    __syncthreads();
+
+    // mean subtraction, phi-dot on the scaled solution, unscaling
+    double sr = 0.0, si = 0.0;
+    for (int i = tid; i < P; i += kThreads) {
+        sr += bR[i];
+        si += bI[i];
+    }
+    const double mr = block_reduce(sr, red) / P;
+    const double mi = block_reduce(si, red) / P;
+    double pr = 0.0, pi = 0.0;
+    for (int i = tid; i < P; i += kThreads) {
+        const double xr = bR[i] - mr, xi = bI[i] - mi;
+        pr -= mO[i] * xr;
+        pi -= mO[i] * xi;
+        a.xdot[i] = xr / sc[i];
+        a.xdot[P + i] = xi / sc[i];
+    }
+    pr = block_reduce(pr, red);
+    pi = block_reduce(pi, red);
+    if (tid == 0) {
+        if (a.imag_time == 0)
+            pi -= eR;
+        else
+            pr -= eR;
+        a.xdot[2 * P] = pr;
+        a.xdot[2 * P + 1] = pi;
     }
 }
 
@@ -269,6 +520,10 @@ hipError_t launch_nb(const SolveArgs& a, hipStream_t st)
 hipError_t launch_solve(const SolveArgs& a, hipStream_t st)
 {
     if (a.P <= 0) return hipErrorInvalidValue;
+    if (a.mean_sub) {
+        hipLaunchKernelGGL(solve_qr_kernel, dim3(1), dim3(kThreads), 2 * (size_t)a.P * sizeof(int), st, a);
+        return hipGetLastError();
+    }
     if ((size_t)32 * a.P * sizeof(double) <= 150 * 1024) return launch_nb<32>(a, st);
     if ((size_t)16 * a.P * sizeof(double) <= 150 * 1024) return launch_nb<16>(a, st);
     return hipErrorInvalidValue;

@@ -109,3 +109,50 @@ def test_solve_nonpositive_pivot_flagged():
     ref = o.solve_param_dot(m, imag_time=1, eps=1e-3)
     assert r["bad"] == ref["bad"] == 1
     assert np.allclose(r["xR"], ref["xR"], rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("imag,pre", [(0, 0), (1, 1)])
+@pytest.mark.parametrize("P", [12, 296])
+def test_solve_type1_vs_oracle(imag, pre, P):
+    """Solver type 1: the GPU's Cholesky solve of the same SPD system against
+    the oracle's full-pivot QR, then the mean subtraction."""
+    rng = np.random.default_rng(3 * P + 10 * imag + pre)
+    A = rng.normal(size=(P, P))
+    S = A @ A.T / P + 0.05 * np.eye(P)
+    m = dict(mO=rng.normal(size=P), mE=np.array([0.9, 0.3]), S=S, FR=rng.normal(size=P), FI=rng.normal(size=P))
+    print("solve type 1", P, imag, pre, flush=True)
+    with _ctx() as ctx:
+        r = ctx.solve(m["mO"], m["mE"], S, m["FR"], m["FI"], imag_time=imag, precondition=pre, solver_type=1)
+    ref = o.solve_param_dot(m, imag_time=imag, precondition=pre, solver_type=1)
+    assert r["bad"] == 0 and ref["bad"] == 0
+    assert fro(r["xR"], ref["xR"]) < 1e-10
+    assert fro(r["xI"], ref["xI"]) < 1e-10
+    assert abs(r["phiDotR"] - ref["phiDotR"]) < 1e-10 * max(1, abs(ref["phiDotR"]))
+    assert abs(r["phiDotI"] - ref["phiDotI"]) < 1e-10 * max(1, abs(ref["phiDotI"]))
+
+
+@pytest.mark.parametrize("P,ndup", [(24, 4), (296, 128)])
+def test_solve_type1_singular_pm_k(P, ndup):
+    """Solver type 1 on an S made singular by identical O columns (k and -k
+    share |rho_k|^2): the GPU's full-pivot QR and the oracle's pick the same
+    solution (rank P - ndup)."""
+    rng = np.random.default_rng(P)
+    W = 3000
+    base = rng.normal(size=(W, P - ndup)) * rng.uniform(0.5, 3.0, P - ndup)
+    cols = list(range(P - ndup))
+    twins = rng.choice(P - ndup, ndup, replace=False)
+    O = np.empty((W, P))
+    order = rng.permutation(P)          # scatter the twins among the columns
+    src = cols + list(twins)
+    for dst, c in zip(order, src):
+        O[:, dst] = base[:, c]
+    E = O @ rng.normal(size=P) * 0.1 + rng.normal(size=W)
+    m = o.moments(O, E, 0.3 * E)
+    print("solve type 1 singular", P, ndup, flush=True)
+    with _ctx() as ctx:
+        r = ctx.solve(m["mO"], m["mE"], m["S"], m["FR"], m["FI"], imag_time=1, precondition=0, solver_type=1)
+    ref = o.solve_param_dot(m, imag_time=1, precondition=0, solver_type=1)
+    assert ref["bad"] == ndup
+    assert fro(r["xR"], ref["xR"]) < 1e-8
+    assert fro(r["xI"], ref["xI"]) < 1e-8
+    assert abs(r["phiDotR"] - ref["phiDotR"]) < 1e-8 * max(1, abs(ref["phiDotR"]))

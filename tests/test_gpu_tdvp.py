@@ -124,3 +124,58 @@ def test_profiled_graph_replays():
                     assert n == 12 and ms > 0.0
             res.append(np.concatenate(ctx.get_params()[:2]))
     assert np.allclose(res[0], res[1], rtol=1e-11, atol=1e-14)
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_euler_step_normalize_adjust_vs_oracle(solver):
+    """USE_NORMALIZE_WF (phi R := -exponent of walker 0 at the last sample)
+    and USE_ADJUST_PARAMETERS (alpha -= alpha[P-1]) after an Euler step, with
+    either linear solver (src/TDVMC.cpp:3757-3775)."""
+    imag = 1
+    cfg, s, R, uR, uI = _setup(imag)
+    cfg.update(USE_NORMALIZE_WF=1, USE_ADJUST_PARAMETERS=1, LINEAR_EQUATION_SOLVER_TYPE=solver)
+    dt = 0.01
+    P = s.P
+    out = o.eval_configs(s, R, uR, uI)
+    mo = o.moments(out["O"], out["ER"], out["EI"])
+    r = o.solve_param_dot(mo, imag_time=imag, precondition=0, eps=1e-3, solver_type=solver)
+    xR = uR + dt * r["xR"]
+    xI = uI + dt * r["xI"]
+    xR, xI = xR - xR[-1], xI - xI[-1]
+    phiR = -out["expo"][0]
+    phiI = -0.2 + dt * r["phiDotI"]
+    print("normalize/adjust", solver, flush=True)
+    with tdvmc.Context(cfg) as ctx:
+        ctx.set_params(uR, uI, 0.1, -0.2)
+        ctx.set_walkers(R)
+        ctx.tdvp_step(dt, ode_type=0, precision=tdvmc.FP64)
+        assert ctx.step_status() == 0
+        gR, gI, pR, pI = ctx.get_params()
+    scale = dt * max(np.max(np.abs(r["xR"])), np.max(np.abs(r["xI"])))
+    assert np.max(np.abs(gR - xR)) < 1e-9 * scale + 1e-13
+    assert np.max(np.abs(gI - xI)) < 1e-9 * scale + 1e-13
+    assert gR[-1] == 0.0 and gI[-1] == 0.0
+    assert abs(pR - phiR) < 1e-10 * max(1.0, abs(phiR))
+    assert abs(pI - phiI) < 1e-10
+
+
+def test_parameter_acceptance_check_rolls_back():
+    """PARAMETER_ACCEPTANCE_CHECK_TYPE 2 (src/TDVMC.cpp:2610-2630): from the
+    second step on, a relative parameter change above 0.1 rejects the step;
+    the parameters and the time stay as they were."""
+    cfg, s, R, uR, uI = _setup(1)
+    cfg.update(USE_PARAMETER_ACCEPTANCE_CHECK=1, PARAMETER_ACCEPTANCE_CHECK_TYPE=2)
+    print("acceptance check", flush=True)
+    with tdvmc.Context(cfg) as ctx:
+        ctx.set_params(uR, uI)
+        ctx.set_walkers(R)
+        ctx.tdvp_step(1e-4, ode_type=0, precision=tdvmc.FP64)     # first step: accepted
+        before = np.concatenate(ctx.get_params()[:2])
+        t0 = ctx.get_info().time
+        with pytest.raises(tdvmc.TdvmcError) as e:
+            ctx.tdvp_step(1e3, ode_type=0, precision=tdvmc.FP64)  # huge step: >10 % change
+        assert e.value.code == tdvmc.ERR_REJECTED
+        after = np.concatenate(ctx.get_params()[:2])
+        assert np.array_equal(before, after)
+        assert ctx.get_info().time == t0
+        ctx.tdvp_step(1e-6, ode_type=0, precision=tdvmc.FP64)     # a small step is accepted again

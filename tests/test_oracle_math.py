@@ -288,3 +288,69 @@ def test_metropolis_two_particles_ring():
     chi2 = np.sum((hist - expect) ** 2 / expect)
     assert chi2 < 60, (hist, expect)
     assert 0.2 < acc / (40000 * 5) < 0.95
+
+
+def test_full_pivot_qr_vs_numpy():
+    rng = np.random.default_rng(11)
+    for n in (1, 5, 17):
+        A = rng.normal(size=(n, n))        # non-symmetric: a wrong row/column un-permutation shows
+        b = rng.normal(size=n)
+        x, rank = o.fpqr_solve(A, b)
+        assert rank == n
+        assert np.allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+    # rank-deficient but consistent: the returned x solves the system
+    n = 9
+    B = rng.normal(size=(n, n - 2))
+    A = B @ rng.normal(size=(n - 2, n))
+    b = A @ rng.normal(size=n)
+    x, rank = o.fpqr_solve(A, b)
+    assert rank == n - 2
+    assert np.allclose(A @ x, b, rtol=1e-9, atol=1e-9)
+
+
+def test_solver_type1_vs_numpy():
+    """Solver type 1 (src/TDVMC.cpp:1763-1822): solve, subtract the mean of
+    each solution, phi-dot from that vector, then undo the scaling; S gets
+    +0.002 I only together with the scaling."""
+    rng = np.random.default_rng(5)
+    P = 10
+    A = rng.normal(size=(P, P))
+    S = A @ A.T + 0.1 * np.eye(P)
+    m = dict(mO=rng.normal(size=P), mE=np.array([0.7, 0.2]), S=S, FR=rng.normal(size=P), FI=rng.normal(size=P))
+    for imag in (0, 1):
+        for pre in (0, 1):
+            res = o.solve_param_dot(m, imag_time=imag, precondition=pre, solver_type=1)
+            if imag == 0:
+                bR, bI = m["FI"], -m["FR"]
+            else:
+                bR, bI = -m["FR"], -(m["FI"] + m["mO"] * m["mE"][1])
+            d = np.sqrt(np.diag(S)) if pre else np.ones(P)
+            Sp = S / np.outer(d, d) + (0.002 * np.eye(P) if pre else 0.0)
+            yR, yI = np.linalg.solve(Sp, bR / d), np.linalg.solve(Sp, bI / d)
+            yR, yI = yR - yR.mean(), yI - yI.mean()
+            phR, phI = -m["mO"] @ yR, -m["mO"] @ yI
+            if imag == 0:
+                phI -= m["mE"][0]
+            else:
+                phR -= m["mE"][0]
+            assert res["bad"] == 0
+            assert np.allclose(res["xR"], yR / d, rtol=1e-10, atol=1e-12)
+            assert np.allclose(res["xI"], yI / d, rtol=1e-10, atol=1e-12)
+            assert abs(res["phiDotR"] - phR) < 1e-10 * max(1, abs(phR))
+            assert abs(res["phiDotI"] - phI) < 1e-10 * max(1, abs(phI))
+
+
+def test_full_pivot_qr_duplicate_columns():
+    """An S with two identical rows/columns (the O of k and -k): rank
+    deficit 1, the system is still solved, and the pivot order (first
+    maximum in column-major order) puts the whole weight on one of the pair."""
+    rng = np.random.default_rng(2)
+    n = 8
+    B = rng.normal(size=(n - 1, 200))
+    O = np.vstack([B, B[-1:]])                 # rows n-2 and n-1 identical
+    S = O @ O.T / 200
+    b = S @ rng.normal(size=n)
+    x, rank = o.fpqr_solve(S, b)
+    assert rank == n - 1
+    assert np.allclose(S @ x, b, rtol=1e-9, atol=1e-9)
+    assert x[n - 1] == 0.0 or x[n - 2] == 0.0
