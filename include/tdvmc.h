@@ -95,7 +95,7 @@ typedef struct {
 
 /* Replaces ReadConfig + InitializePhysicalSystem + IPhysicalSystem::InitSystem
  * (src/TDVMC.cpp:351-375, :2786-2998; src/PhysicalSystems/BosonsBulk.cpp:51-156).
- * config: the text of a config/*.config JSON document or a path to one. */
+ * config: the text of a config/NAME.config JSON document or a path to one. */
 int tdvmc_create(const char* config, tdvmc_ctx** out);
 int tdvmc_destroy(tdvmc_ctx* ctx);
 /* Host-only: parse and validate a config without touching the GPU (the

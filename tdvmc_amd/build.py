@@ -64,7 +64,26 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_cpp_check(verbose)
     return LIB
+
+
+CPP_CHECK_SRC = os.path.join(ROOT, "tests", "cpp", "adapter_check.cpp")
+CPP_CHECK_BIN = os.path.join(ROOT, "tests", "cpp", "build", "adapter_check")
+
+
+def build_cpp_check(verbose: bool = True) -> str:
+    """Host-only C++ program over include/tdvmc.hpp (g++, no device code),
+    linked against the in-tree libtdvmc.so with a relative run path."""
+    os.makedirs(os.path.dirname(CPP_CHECK_BIN), exist_ok=True)
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), CPP_CHECK_SRC,
+           "-L" + HERE, "-ltdvmc", "-Wl,-rpath,$ORIGIN/../../../tdvmc_amd", "-o", CPP_CHECK_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ check build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print("built", CPP_CHECK_BIN)
+    return CPP_CHECK_BIN
 
 
 if __name__ == "__main__":

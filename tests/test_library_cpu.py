@@ -77,3 +77,19 @@ def test_workloads_are_valid_configs():
     assert (info.N, info.DIM, info.P, info.K, info.n_walkers) == (343, 3, 296, 256, 65536)
     info = tdvmc.parse_config(workloads.bulk2d_gauss400_k512())
     assert (info.N, info.DIM, info.P, info.K) == (400, 2, 560, 512)
+
+
+def test_cpp_header_compiles_and_check_program_built():
+    """include/tdvmc.hpp compiles on its own as C++17, and the C++ check
+    program is built (it only prints its usage without arguments)."""
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = "#include \"tdvmc.hpp\"\nint main() { return 0; }\n"
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-x", "c++", "-I",
+                        os.path.join(root, "include"), "-"], input=src, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    binp = os.path.join(root, "tests", "cpp", "build", "adapter_check")
+    assert os.path.exists(binp), "run the build first"
+    r = subprocess.run([binp], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
