@@ -190,7 +190,7 @@ def main():
     ms_step = mixed["elapsed"] / mixed["steps"] * 1e3
     avg = {k: (v[0] / max(v[1], 1)) for k, v in mixed["prof"].items()}
     launches = {k: v[1] for k, v in mixed["prof"].items()}
-    kpl = next(m for m in (0, 2, 4, 8) if m * 64 >= K)   # k-vectors per lane of the sweep template
+    kpl = next(m for m in (0, 2, 4, 8) if m * 64 >= info.K_terms)   # k terms per lane of the sweep template
     # algorithmic flops per launch: one sweep = N trial moves of every walker,
     # one measurement = every pair and every (particle, k) term of every walker
     rl = {

@@ -72,6 +72,7 @@ typedef struct {
     double time;
     double reg_eps;
     int precondition;
+    int K_terms;           /* k-space terms the kernels evaluate: one per +-k pair, plus unpaired k */
 } tdvmc_info;
 
 /* Moments of one sampling stage; fields mirror SimulationStepData

@@ -47,7 +47,7 @@ class Info(C.Structure):
         ("hbar2_2m", C.c_double), ("imaginary_time", C.c_int), ("ode_solver_type", C.c_int),
         ("precision", C.c_int), ("samples_per_walker", C.c_int), ("timestep", C.c_double),
         ("mc_step", C.c_double), ("therm_moves", C.c_int64), ("init_moves", C.c_int64), ("time", C.c_double),
-        ("reg_eps", C.c_double), ("precondition", C.c_int),
+        ("reg_eps", C.c_double), ("precondition", C.c_int), ("K_terms", C.c_int),
     ]
 
 

@@ -31,6 +31,7 @@ struct Prof {
 template <typename T>
 struct DevTables {
     T* spl = nullptr;
+    T* dspl = nullptr;
     T* kv = nullptr;
     T* alpha = nullptr;
     bool dirty = true;
@@ -65,7 +66,7 @@ struct tdvmc_ctx {
 
     int64_t W = 0;
     double* dR = nullptr;
-    double* dRho = nullptr;     // [W][K][2] rho_k left by the last sweep (float or double)
+    double* dRho = nullptr;     // [W][Kh][2] rho_k of the k representatives left by the last sweep (float or double)
     unsigned long long* dAcc = nullptr;
     uint32_t* dEpoch = nullptr;
     double* dUR = nullptr;
@@ -75,8 +76,10 @@ struct tdvmc_ctx {
     double* dK[4] = {nullptr, nullptr, nullptr, nullptr};
     DevTables<float> tf;
     DevTables<double> td;
-    std::vector<float> kv_f;
+    std::vector<float> kv_f;          // [Kh][4] representatives
     std::vector<double> kv_d;
+    std::vector<int> kmap;            // [Kh][2] parameter slots of k and -k
+    int* dKmap = nullptr;
 
     int64_t cap_samples = 0;
     double* dO64 = nullptr;
@@ -325,20 +328,41 @@ int build_system(tdvmc_ctx* ctx)
     ctx->phiR = c.PARAM_PHIR;
     ctx->phiI = c.PARAM_PHII;
 
-    // k-vectors: stored as n/L (phase in revolutions) and |k|^2
+    // k-vectors in parameter order; each k whose -k is also in the set is
+    // paired with it (the first of the two is the representative).  Stored
+    // per representative: n/L (phase in revolutions) and |k|^2.
     auto ns = shortest_lattice_vectors(s.D, s.K);
-    ctx->kv_d.assign(4 * (size_t)std::max(s.K, 1), 0.0);
-    ctx->kv_f.assign(4 * (size_t)std::max(s.K, 1), 0.0f);
-    for (int k = 0; k < s.K; k++) {
+    std::vector<int> partner(s.K, -1);
+    for (int i = 0; i < s.K; i++)
+        for (int j = i + 1; j < s.K && partner[i] < 0; j++) {
+            bool neg = true;
+            for (int a = 0; a < s.D; a++) neg &= ns[j][a] == -ns[i][a];
+            if (neg && partner[j] < 0) {
+                partner[i] = j;
+                partner[j] = i;
+            }
+        }
+    ctx->kmap.clear();
+    for (int i = 0; i < s.K; i++) {
+        if (partner[i] >= 0 && partner[i] < i) continue;   // represented by its partner
+        ctx->kmap.push_back(i);
+        ctx->kmap.push_back(partner[i]);
+    }
+    s.Kh = (int)ctx->kmap.size() / 2;
+    ctx->kv_d.assign(4 * (size_t)std::max(s.Kh, 1), 0.0);
+    ctx->kv_f.assign(4 * (size_t)std::max(s.Kh, 1), 0.0f);
+    for (int k = 0; k < s.Kh; k++) {
+        const auto& n = ns[ctx->kmap[2 * k]];
         double n2 = 0;
         for (int a = 0; a < s.D; a++) {
-            ctx->kv_d[4 * k + a] = ns[k][a] * s.Linv;
-            n2 += (double)ns[k][a] * ns[k][a];
+            ctx->kv_d[4 * k + a] = n[a] * s.Linv;
+            n2 += (double)n[a] * n[a];
         }
         const double tp = 2.0 * M_PI * s.Linv;
         ctx->kv_d[4 * k + 3] = tp * tp * n2;
     }
     for (size_t i = 0; i < ctx->kv_d.size(); i++) ctx->kv_f[i] = (float)ctx->kv_d[i];
+    if (ctx->kmap.empty()) ctx->kmap.assign(2, -1);
 
     // resource limits of the kernels
     const size_t lds_max = 160 * 1024;
@@ -385,8 +409,10 @@ KTables<T> tables_of(tdvmc_ctx* ctx, DevTables<T>& t)
 {
     KTables<T> k;
     k.spl = t.spl;
+    k.dspl = t.dspl;
     k.kv = t.kv;
     k.alpha = t.alpha;
+    k.kmap = ctx->dKmap;
     k.uR = ctx->dUR;
     k.uI = ctx->dUI;
     return k;
@@ -396,7 +422,7 @@ template <typename T>
 int ensure_tables(tdvmc_ctx* ctx, DevTables<T>& t, bool force)
 {
     if (!t.dirty && !force) return TDVMC_OK;
-    HIPC(launch_prep_tables<T>(ctx->sy, ctx->dUR, ctx->dUI, t.spl, t.alpha, ctx->stream));
+    HIPC(launch_prep_tables<T>(ctx->sy, ctx->dUR, ctx->dUI, ctx->dKmap, t.spl, t.dspl, t.alpha, ctx->stream));
     t.dirty = false;
     return TDVMC_OK;
 }
@@ -556,7 +582,7 @@ int enqueue_sweep(tdvmc_ctx* ctx, int prec, int64_t n_moves, uint64_t seed, bool
 {
     if (n_moves <= 0) return TDVMC_OK;
     SweepArgs a;
-    a.rho_out = (want_rho && ctx->sy.K > 0) ? static_cast<void*>(ctx->dRho) : nullptr;
+    a.rho_out = (want_rho && ctx->sy.Kh > 0) ? static_cast<void*>(ctx->dRho) : nullptr;
     a.R = ctx->dR;
     a.W = ctx->W;
     a.n_moves = n_moves;
@@ -583,7 +609,7 @@ int enqueue_sweep(tdvmc_ctx* ctx, int prec, int64_t n_moves, uint64_t seed, bool
 // take rho_k from the sweep that just ran instead of recomputing it
 int enqueue_eval(tdvmc_ctx* ctx, int prec, int64_t row0, bool use_rho = false)
 {
-    const bool rho = use_rho && ctx->sy.K > 0;
+    const bool rho = use_rho && ctx->sy.Kh > 0;
     const int P = P_of(ctx);
     hipEvent_t ev{};
     int rc = prof_begin(ctx, "eval", &ev);
@@ -781,8 +807,13 @@ int alloc_ctx(tdvmc_ctx* ctx)
     for (int i = 0; i < 4; i++) HIPC(dalloc(&ctx->dK[i], 2 * P + 2));
     HIPC(dalloc(&ctx->dEpoch, 1));
     HIPC(launch_fill_u32(ctx->dEpoch, 0u, 1, ctx->stream));
-    const int Ps = ctx->sy.P_s, K = std::max(ctx->sy.K, 1);
+    const int Ps = ctx->sy.P_s, K = std::max(ctx->sy.Kh, 1);
+    HIPC(dalloc(&ctx->dKmap, ctx->kmap.size()));
+    HIPC(hipMemcpyAsync(ctx->dKmap, ctx->kmap.data(), ctx->kmap.size() * sizeof(int), hipMemcpyHostToDevice,
+                        ctx->stream));
     HIPC(dalloc(&ctx->tf.spl, (size_t)(Ps - 1) * 8));
+    HIPC(dalloc(&ctx->tf.dspl, (size_t)(Ps - 1) * 8));
+    HIPC(dalloc(&ctx->td.dspl, (size_t)(Ps - 1) * 8));
     HIPC(dalloc(&ctx->tf.kv, 4 * (size_t)K));
     HIPC(dalloc(&ctx->tf.alpha, 2 * (size_t)K));
     HIPC(dalloc(&ctx->td.spl, (size_t)(Ps - 1) * 8));
@@ -825,6 +856,9 @@ void release(tdvmc_ctx* ctx)
     dfree(ctx->dXprev);
     for (auto& k : ctx->dK) dfree(k);
     dfree(ctx->tf.spl);
+    dfree(ctx->dKmap);
+    dfree(ctx->tf.dspl);
+    dfree(ctx->td.dspl);
     dfree(ctx->tf.kv);
     dfree(ctx->tf.alpha);
     dfree(ctx->td.spl);
@@ -943,6 +977,7 @@ int tdvmc_get_info(tdvmc_ctx* ctx, tdvmc_info* info)
     info->time = ctx->time;
     info->reg_eps = ctx->reg_eps;
     info->precondition = ctx->precondition;
+    info->K_terms = ctx->sy.Kh;
     return TDVMC_OK;
 }
 
@@ -954,7 +989,7 @@ int tdvmc_set_walker_count(tdvmc_ctx* ctx, int64_t W)
     HIPC(hipStreamSynchronize(ctx->stream));
     HIPC(dalloc(&ctx->dR, (size_t)W * ctx->sy.N * ctx->sy.D));
     HIPC(dalloc(&ctx->dAcc, (size_t)W));
-    HIPC(dalloc(&ctx->dRho, (size_t)W * 2 * std::max(ctx->sy.K, 1)));
+    HIPC(dalloc(&ctx->dRho, (size_t)W * 2 * std::max(ctx->sy.Kh, 1)));
     HIPC(hipMemsetAsync(ctx->dAcc, 0, (size_t)W * sizeof(unsigned long long), ctx->stream));
     ctx->W = W;
     ctx->trials_total = 0;
@@ -1497,7 +1532,7 @@ int tdvmc_set_sweep_lanes(tdvmc_ctx* ctx, int lanes_per_walker)
     CHECK_ARG(lanes_per_walker == 0 || lanes_per_walker == 16 || lanes_per_walker == 32 || lanes_per_walker == 64,
               "lanes per walker must be 0, 16, 32 or 64");
     if (lanes_per_walker) {
-        const int kpl = (ctx->sy.K + lanes_per_walker - 1) / lanes_per_walker;
+        const int kpl = (ctx->sy.Kh + lanes_per_walker - 1) / lanes_per_walker;
         CHECK_ARG(kpl <= 8, "too many k-vectors for this lane count");
     }
     ctx->lanes = lanes_per_walker;

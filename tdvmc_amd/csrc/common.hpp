@@ -11,6 +11,7 @@ namespace tdvmc {
 struct KSys {
     int N, D;
     int P_s, K, P;
+    int Kh;                // k-space terms after pairing k with -k (see KTables)
     int off_pair, off_k, one_body;
     int periodic;          // 1: minimum image in a box of length L
     int pot_type;          // 0 none, 1 square well, 2 Gaussian, 3 Lennard-Jones
@@ -26,11 +27,19 @@ struct KSys {
 
 // Per-precision device tables derived from the parameters (written by
 // prep_tables_kernel, so parameter updates stay on the device).
+// k-space terms come in pairs k, -k whenever both are in the set.  For real
+// coordinates rho_{-k} = conj(rho_k), so both share |rho_k|^2, its gradient
+// and Laplacian; the kernels work on one representative per pair (Kh of them)
+// with the amplitude sum alpha_k + alpha_{-k}, and write O = |rho_k|^2 to both
+// parameter slots.
 template <typename T>
 struct KTables {
     const T* spl;     // [(P_s-1)][8]: cubic-in-t coefficients A0..A3 of Re u, then of Im u
-    const T* kv;      // [K][4]: n/L (phase in revolutions = kv . r), |k|^2
-    const T* alpha;   // [K][2]: (Re, Im) of the k-space amplitudes
+    const T* dspl;    // [(P_s-1)][8]: u' = d1 + t (d2 + t d3), u'' = hinv d2 + t e3 per unit length,
+                      //   (d1, d2, d3, e3) of Re u, then of Im u
+    const T* kv;      // [Kh][4]: n/L of the representative (phase in revolutions = kv . r), |k|^2
+    const T* alpha;   // [Kh][2]: (Re, Im) of alpha_k (+ alpha_{-k} when paired)
+    const int* kmap;  // [Kh][2]: parameter slots (relative to off_k) of k and of -k (-1: unpaired)
     const double* uR; // [P] parameters (FP64 master copy)
     const double* uI;
 };
@@ -90,6 +99,13 @@ template <> struct Prec<float> {
     __device__ static __forceinline__ float rint_(float x) { return __builtin_rintf(x); }
     __device__ static __forceinline__ float floor_(float x) { return __builtin_floorf(x); }
     __device__ static __forceinline__ float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
+    // r = sqrt(r2) and 1/r from one reciprocal square root
+    __device__ static __forceinline__ void r_and_rinv(float r2, float* r, float* rinv)
+    {
+        const float q = __builtin_amdgcn_rsqf(r2);
+        *rinv = q;
+        *r = r2 * q;
+    }
     // sin / cos of 2*pi*t, t in revolutions, |t| <= 0.5 after reduction
     __device__ static __forceinline__ void sincos_rev(float t, float* s, float* c)
     {
@@ -106,6 +122,11 @@ template <> struct Prec<double> {
     __device__ static __forceinline__ double rint_(double x) { return __builtin_rint(x); }
     __device__ static __forceinline__ double floor_(double x) { return __builtin_floor(x); }
     __device__ static __forceinline__ double rcp_(double x) { return 1.0 / x; }
+    __device__ static __forceinline__ void r_and_rinv(double r2, double* r, double* rinv)
+    {
+        *r = __builtin_sqrt(r2);
+        *rinv = 1.0 / *r;
+    }
     __device__ static __forceinline__ void sincos_rev(double t, double* s, double* c)
     {
         t = t - __builtin_rint(t);

@@ -10,7 +10,8 @@ namespace tdvmc {
 
 // ---- parameter-derived tables (kernels_misc.hip)
 template <typename T>
-hipError_t launch_prep_tables(const KSys& sy, const double* uR, const double* uI, T* spl, T* alpha, hipStream_t st);
+hipError_t launch_prep_tables(const KSys& sy, const double* uR, const double* uI, const int* kmap, T* spl, T* dspl,
+                              T* alpha, hipStream_t st);
 
 // ---- evaluation (kernels_eval.hip); grad/lap non-null selects the per-particle variant
 template <typename T>

@@ -5,21 +5,25 @@
 // Restates src/PhysicalSystems/BosonsBulk.cpp:179-218 (basis sums O_k and the
 // outer-pair count), :220-336 (pair potential, gradient and Laplacian pieces),
 // :349-458 (E_L and the nine 'other' values) and :522-539 (exponent), adds the
-// k-space Jastrow sum_k alpha_k |rho_k|^2 and a one-body Gaussian factor.
+// k-space Jastrow sum_k alpha_k |rho_k|^2 (k and -k share |rho_k|^2 and are
+// handled once with the summed amplitude) and a one-body Gaussian factor.
 //
 // Work split (N particles, K k-vectors, 256 threads):
 //   pairs    : every unordered pair once, on a half ring: lane = row n of a
 //              64-row block, partner i = (n + s) mod N, the s range split over
-//              the 4 waves; own-row sums stay in registers, partner sums go
-//              to LDS with conflict-free atomics (distinct i per lane);
+//              the 4 waves; own-row sums stay in registers, partner sums ride
+//              a DPP register ring and reach LDS once per particle and step;
 //   rho_k    : thread = k, loop over particles (LDS broadcast reads);
 //   k terms  : lane = row n, the k range split over the 4 waves;
 //   E_L      : thread = particle, FP64 block reduction.
-// The spline basis sums go to an LDS histogram [bin][16] (4 lanes per column).
-// Coordinates are staged in LDS as structure-of-arrays (bank-conflict-free
-// reads of consecutive particles); every LDS accumulator is FP64 in both
-// precisions (FP64 LDS atomics run at full rate, FP32 ones stall the LDS
-// pipeline on this part, and FP64 sums keep O_k accurate in the mixed form).
+// The pair's position t in its knot interval goes to an LDS histogram of the
+// moments 1, t, t^2, t^3 per interval ([interval][4][8], 8 lanes per column);
+// the B-spline basis sums are rebuilt from these moments at the end.
+// Coordinates and per-particle sums are staged in LDS as structure-of-arrays
+// (bank-conflict-free access to consecutive particles).  Every LDS
+// accumulator is FP64 in both precisions: FP64 LDS atomics run several
+// times faster than FP32 ones on this part, and FP64 sums keep O_k and the
+// gradients accurate in the mixed form.
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -30,7 +34,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
-constexpr int kHistCols = 16;
+constexpr int kHistCols = 8;    // histogram columns (LDS footprint vs bank conflicts: occupancy wins)
 
 template <typename TO>
 struct EvalOut {
@@ -46,9 +50,27 @@ struct EvalOut {
     __device__ __forceinline__ void O_store(int64_t w, int P, int k, double v) const { O[w * P + k] = (TO)v; }
 };
 
-__device__ __forceinline__ void lds_add(double* p, double v)
+template <typename V>
+__device__ __forceinline__ void lds_add(V* p, V v)
 {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Whole-wavefront rotation by one lane towards lane 0 (lane l receives lane
+// l + 1, lane 63 receives lane 0): DPP wave_rol:1 (GFX9 DPP control 0x134),
+// one in-place v_mov_b32_dpp per dword.
+constexpr int kDppWaveRol1 = 0x134;
+__device__ __forceinline__ float wave_rotate(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kDppWaveRol1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double wave_rotate(double v)
+{
+    const int2 x = *reinterpret_cast<int2*>(&v);
+    int2 y;
+    y.x = __builtin_amdgcn_mov_dpp(x.x, kDppWaveRol1, 0xF, 0xF, false);
+    y.y = __builtin_amdgcn_mov_dpp(x.y, kDppWaveRol1, 0xF, 0xF, false);
+    return *reinterpret_cast<double*>(&y);
 }
 
 __device__ __forceinline__ double block_sum(double v, double* red)
@@ -66,24 +88,27 @@ __device__ __forceinline__ double block_sum(double v, double* red)
 }
 
 template <typename T, int D, bool PP, bool PER, typename TO>
-__global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, const double* __restrict__ Rg,
+__global__ __launch_bounds__(kThreads, 4) void eval_kernel(KSys sy, KTables<T> tb, const double* __restrict__ Rg,
                                                          int64_t W, EvalOut<TO> out, const T* __restrict__ rho_in)
 {
     const int64_t w = blockIdx.x;
     if (w >= W) return;
-    const int N = sy.N, Ps = sy.P_s, K = sy.K, S = Ps + 2;
+    const int N = sy.N, Ps = sy.P_s, K = sy.Kh;   // K: +-k representatives
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (scalar registers)
 
     extern __shared__ __align__(16) unsigned char smem[];
-    // LDS carve-up (sizes mirrored by eval_lds_bytes)
+    // LDS carve-up (sizes mirrored by eval_lds_bytes); per-particle arrays are
+    // structure-of-arrays so consecutive particles sit in consecutive banks
     double* red = reinterpret_cast<double*>(smem);             // 8 doubles
-    double* gsum = red + 8;                                    // N * 2D : (Re g, Im g)
-    double* lsum = gsum + (size_t)N * 2 * D;                   // N * 2 (PP only)
-    double* hist = lsum + (PP ? (size_t)N * 2 : 0);            // S * 16
-    T* pos = reinterpret_cast<T*>(hist + (size_t)S * kHistCols);   // D * N (structure of arrays)
+    double* hist = red + 8;                                    // (Ps-1) * 4 * kHistCols
+    double* gsum = hist + (size_t)(Ps - 1) * 4 * kHistCols;    // [2D][N]: Re g, Im g
+    double* lsum = gsum + (size_t)N * 2 * D;                   // [2][N] (PP only)
+    T* pos = reinterpret_cast<T*>(lsum + (PP ? (size_t)N * 2 : 0));   // [D][N]
     T* rho = pos + (size_t)N * D;                              // 2K
-    T* spl = rho + 2 * (size_t)K;                              // (Ps-1) * 8
+    // derivative table (Ps-1) * 8, 16-byte aligned for 128-bit reads
+    T* dsp = reinterpret_cast<T*>(smem + ((reinterpret_cast<unsigned char*>(rho + 2 * (size_t)K) - smem + 15) &
+                                          ~(size_t)15));
     const T* kv = tb.kv;                                       // K * 4 (global, read with uniform k)
 
     // periodic box: coordinates in box units (x / L, wrapped into the primary
@@ -99,51 +124,74 @@ __global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, 
             pos[a * N + i] = (T)x;
         }
 #pragma unroll
-        for (int c = 0; c < 2 * D; c++) gsum[i * 2 * D + c] = 0.0;
+        for (int c = 0; c < 2 * D; c++) gsum[c * N + i] = 0.0;
         if (PP) {
-            lsum[2 * i] = 0.0;
-            lsum[2 * i + 1] = 0.0;
+            lsum[i] = 0.0;
+            lsum[N + i] = 0.0;
         }
     }
-    for (int i = tid; i < S * kHistCols; i += kThreads) hist[i] = 0.0;
-    for (int i = tid; i < (Ps - 1) * 8; i += kThreads) spl[i] = tb.spl[i];
+    for (int i = tid; i < (Ps - 1) * 4 * kHistCols; i += kThreads) hist[i] = 0.0;
+    for (int i = tid; i < (Ps - 1) * 8; i += kThreads) dsp[i] = tb.dspl[i];
     __syncthreads();
 
     // ------------------------------------------------------------ pairs
-    const T hinv = (T)sy.hinv, hsc = (T)(sy.hinv / scale), rc = (T)(sy.rc * scale);
+    const T hsc = (T)(sy.hinv / scale), rc = (T)(sy.rc * scale);
     const T pcut = (T)(sy.pot_cutoff * scale);
     const T pa = (T)(sy.pot_a * scale), pb = (T)sy.pot_b;
     const T lapsc = (T)((sy.D - 1) * scale);   // (D-1)/r = (D-1) scale / r_box
-    const int shalf = (N - 1) / 2;
-    const int smax = (N % 2 == 0) ? N / 2 : shalf;    // s = N/2 only for n < N/2 (even N)
-    const int chunk = (smax + kWaves - 1) / kWaves;
-    const int s_lo = 1 + wave * chunk;
-    const int s_hi = min(smax, (wave + 1) * chunk);
     double vpot = 0.0, outer = 0.0, lap_tot_R = 0.0, lap_tot_I = 0.0;
+    const T hinvT = (T)sy.hinv;
 
-    for (int rb = 0; rb < N; rb += 64) {
-        const int n = rb + lane;
-        if (n < N) {
-            T rn[D];
+    // Pairs in 64 x 64 tiles (A, B), A <= B, of particle blocks.  Lane l owns
+    // row particle 64 A + l; column particles ride a register ring that
+    // rotates one lane per step (wave_rol), so at step s lane l meets column
+    // 64 B + ((l + s) mod 64) and after 64 steps every row has met every
+    // column.  Diagonal tiles take s = 1..32 (s = 32 by lanes < 32 only), so
+    // every unordered pair occurs once.  Row sums stay in registers, column
+    // sums rotate with the column coordinates; both reach LDS once per tile
+    // segment.  The steps of all tiles are split evenly over the 4 waves.
+    const int nblk = (N + 63) / 64;
+    const int total = nblk * (nblk - 1) / 2 * 64 + nblk * 32;
+    const int g0 = total * wave / kWaves, g1 = total * (wave + 1) / kWaves;
+    int off = 0;
+    for (int A = 0; A < nblk; A++) {
+        for (int Bk = A; Bk < nblk; Bk++) {
+            const bool diag = A == Bk;
+            const int cnt = diag ? 32 : 64;
+            const int lo = max(g0, off), hi = min(g1, off + cnt);
+            off += cnt;
+            if (lo >= hi) continue;
+            const int s_first = lo - (off - cnt) + (diag ? 1 : 0);
+            const int nsteps = hi - lo;
+            const int ra = A * 64 + lane;
+            const bool rval = ra < N;
+            T rn[D], cp[D];
+            int cidx = Bk * 64 + ((lane + s_first) & 63);
 #pragma unroll
-            for (int a = 0; a < D; a++) rn[a] = pos[a * N + n];
-            T gR[D], gI[D];
+            for (int a = 0; a < D; a++) {
+                rn[a] = rval ? pos[a * N + ra] : (T)0;
+                cp[a] = cidx < N ? pos[a * N + cidx] : (T)0;
+            }
+            T gR[D], gI[D], pR[D], pI[D];
 #pragma unroll
-            for (int a = 0; a < D; a++) gR[a] = gI[a] = (T)0;
-            T lR = (T)0, lI = (T)0;
-            int i = (n + s_lo) % N;
-            for (int s = s_lo; s <= s_hi; s++) {
-                const bool live = !(2 * s == N && n >= N / 2);
+            for (int a = 0; a < D; a++) gR[a] = gI[a] = pR[a] = pI[a] = (T)0;
+            T lR = (T)0, lI = (T)0, plR = (T)0, plI = (T)0;
+            for (int u = 0; u < nsteps; u++) {
+                const int s = s_first + u;
+                cidx = Bk * 64 + ((lane + s) & 63);
+                const bool live = rval && cidx < N && (!diag || s < 32 || lane < 32);
                 if (live) {
                     T d[D];
                     T r2 = (T)0;
 #pragma unroll
                     for (int a = 0; a < D; a++) {
-                        d[a] = rn[a] - pos[a * N + i];
+                        d[a] = rn[a] - cp[a];
                         if (PER) d[a] -= Prec<T>::rint_(d[a]);
                         r2 += d[a] * d[a];
                     }
-                    const T r = Prec<T>::sqrt_(r2);
+                    // FP32: one v_rsq gives 1/r, and r = r^2 / r; FP64: sqrt and a divide
+                    T r, rinv;
+                    Prec<T>::r_and_rinv(r2, &r, &rinv);
                     if (sy.pot_type != 0 && r <= pcut) {
                         T v;
                         if (sy.pot_type == 1) {
@@ -161,61 +209,81 @@ __global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, 
                     if (r <= rc) {
                         T t;
                         const int j = knot_interval(r, hsc, Ps, &t);
-                        const T u = (T)1 - t;
-                        const T t2 = t * t, t3 = t2 * t;
-                        // the four non-zero uniform cubic B-splines; the third
-                        // follows from the partition of unity
-                        const T b0 = u * u * u * (T)(1.0 / 6.0);
-                        const T b1 = ((T)3 * t3 - (T)6 * t2 + (T)4) * (T)(1.0 / 6.0);
-                        const T b3 = t3 * (T)(1.0 / 6.0);
-                        const T b2 = (T)1 - b0 - b1 - b3;
-                        double* hb = hist + (size_t)j * kHistCols + (lane & (kHistCols - 1));
-                        lds_add(hb, (double)b0);
-                        lds_add(hb + kHistCols, (double)b1);
-                        lds_add(hb + 2 * kHistCols, (double)b2);
-                        lds_add(hb + 3 * kHistCols, (double)b3);
-                        const T* c = spl + j * 8;
-                        // u'(r) = (A1 + 2 A2 t + 3 A3 t^2) / h ; u''(r) = (2 A2 + 6 A3 t) / h^2
-                        const T upR = (c[1] + t * ((T)2 * c[2] + (T)3 * c[3] * t)) * hinv;
-                        const T upI = (c[5] + t * ((T)2 * c[6] + (T)3 * c[7] * t)) * hinv;
-                        const T uppR = ((T)2 * c[2] + (T)6 * c[3] * t) * hinv * hinv;
-                        const T uppI = ((T)2 * c[6] + (T)6 * c[7] * t) * hinv * hinv;
-                        const T rinv = Prec<T>::rcp_(r);
+                        // moments 1, t, t^2, t^3 of the pair's position in knot
+                        // interval j (FP64); the basis sums B_s are rebuilt from
+                        // them at the end
+                        const double td = (double)t, t2 = td * td, t3 = t2 * td;
+                        double* hb = hist + j * 4 * kHistCols + (lane & (kHistCols - 1));
+                        lds_add(hb, 1.0);
+                        lds_add(hb + kHistCols, td);
+                        lds_add(hb + 2 * kHistCols, t2);
+                        lds_add(hb + 3 * kHistCols, t3);
+                        // u'(r), u''(r) per unit length on interval j; the
+                        // 24-bit multiply keeps the LDS offset a full-rate op
+                        const T* c = reinterpret_cast<const T*>(reinterpret_cast<const unsigned char*>(dsp) +
+                                                                __umul24((unsigned)j, 8 * sizeof(T)));
+                        const T upR = c[0] + t * (c[1] + t * c[2]);
+                        const T uppR = hinvT * c[1] + t * c[3];
+                        const T upI = c[4] + t * (c[5] + t * c[6]);
+                        const T uppI = hinvT * c[5] + t * c[7];
                         const T fR = upR * rinv, fI = upI * rinv;
-                        double* gp = gsum + (size_t)i * 2 * D;
 #pragma unroll
                         for (int a = 0; a < D; a++) {
                             gR[a] += fR * d[a];
                             gI[a] += fI * d[a];
-                            lds_add(gp + a, (double)(-fR * d[a]));
-                            lds_add(gp + D + a, (double)(-fI * d[a]));
+                            pR[a] -= fR * d[a];
+                            pI[a] -= fI * d[a];
                         }
                         const T lfR = uppR + lapsc * fR;
                         const T lfI = uppI + lapsc * fI;
+                        lR += lfR;
+                        lI += lfI;
                         if (PP) {
-                            lR += lfR;
-                            lI += lfI;
-                            lds_add(lsum + 2 * i, (double)lfR);
-                            lds_add(lsum + 2 * i + 1, (double)lfI);
-                        } else {
-                            lap_tot_R += 2.0 * (double)lfR;
-                            lap_tot_I += 2.0 * (double)lfI;
+                            plR += lfR;
+                            plI += lfI;
                         }
                     } else {
                         outer += 1.0;
                     }
                 }
-                i = (i + 1 == N) ? 0 : i + 1;
-            }
-            double* gp = gsum + (size_t)n * 2 * D;
 #pragma unroll
-            for (int a = 0; a < D; a++) {
-                lds_add(gp + a, (double)gR[a]);
-                lds_add(gp + D + a, (double)gI[a]);
+                for (int a = 0; a < D; a++) {
+                    cp[a] = wave_rotate(cp[a]);
+                    pR[a] = wave_rotate(pR[a]);
+                    pI[a] = wave_rotate(pI[a]);
+                }
+                if (PP) {
+                    plR = wave_rotate(plR);
+                    plI = wave_rotate(plI);
+                }
             }
-            if (PP) {
-                lds_add(lsum + 2 * n, (double)lR);
-                lds_add(lsum + 2 * n + 1, (double)lI);
+            // column sums: lane l now holds column 64 B + ((l + s_first + nsteps) mod 64)
+            cidx = Bk * 64 + ((lane + s_first + nsteps) & 63);
+            if (cidx < N) {
+#pragma unroll
+                for (int a = 0; a < D; a++) {
+                    lds_add(gsum + a * N + cidx, (double)pR[a]);
+                    lds_add(gsum + (D + a) * N + cidx, (double)pI[a]);
+                }
+                if (PP) {
+                    lds_add(lsum + cidx, (double)plR);
+                    lds_add(lsum + N + cidx, (double)plI);
+                }
+            }
+            if (rval) {
+#pragma unroll
+                for (int a = 0; a < D; a++) {
+                    lds_add(gsum + a * N + ra, (double)gR[a]);
+                    lds_add(gsum + (D + a) * N + ra, (double)gI[a]);
+                }
+                if (PP) {
+                    lds_add(lsum + ra, (double)lR);
+                    lds_add(lsum + N + ra, (double)lI);
+                } else {
+                    // every pair term enters the Laplacians of both particles
+                    lap_tot_R += 2.0 * (double)lR;
+                    lap_tot_I += 2.0 * (double)lI;
+                }
             }
         }
     }
@@ -244,9 +312,16 @@ __global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, 
         }
         rho[2 * k] = re;
         rho[2 * k + 1] = im;
+        // O = |rho_k|^2 for k and for -k (rho_{-k} = conj rho_k)
         const double ok = (double)re * (double)re + (double)im * (double)im;
-        ex_k += tb.uR[sy.off_k + k] * ok;
-        out.O_store(w, sy.P, sy.off_k + k, ok);
+        const int m0 = tb.kmap[2 * k], m1 = tb.kmap[2 * k + 1];
+        double a = tb.uR[sy.off_k + m0];
+        out.O_store(w, sy.P, sy.off_k + m0, ok);
+        if (m1 >= 0) {
+            a += tb.uR[sy.off_k + m1];
+            out.O_store(w, sy.P, sy.off_k + m1, ok);
+        }
+        ex_k += a * ok;
     }
     __syncthreads();
 
@@ -295,21 +370,28 @@ __global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, 
                 }
                 lR *= (T)2;
                 lI *= (T)2;
-                double* gp = gsum + (size_t)n * 2 * D;
 #pragma unroll
                 for (int a = 0; a < D; a++) {
-                    lds_add(gp + a, (double)gR[a]);
-                    lds_add(gp + D + a, (double)gI[a]);
+                    lds_add(gsum + a * N + n, (double)gR[a]);
+                    lds_add(gsum + (D + a) * N + n, (double)gI[a]);
                 }
                 if (PP) {
-                    lds_add(lsum + 2 * n, (double)lR);
-                    lds_add(lsum + 2 * n + 1, (double)lI);
+                    lds_add(lsum + n, (double)lR);
+                    lds_add(lsum + N + n, (double)lI);
                 } else {
                     lap_tot_R += (double)lR;
                     lap_tot_I += (double)lI;
                 }
             }
         }
+    }
+    __syncthreads();
+
+    // histogram columns -> column 0 (interval moments M[j][p])
+    for (int e = tid; e < (Ps - 1) * 4; e += kThreads) {
+        double m = 0.0;
+        for (int c = 0; c < kHistCols; c++) m += hist[e * kHistCols + c];
+        hist[e * kHistCols] = m;
     }
     __syncthreads();
 
@@ -323,15 +405,15 @@ __global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, 
         for (int a = 0; a < D; a++) {
             // one-body factor uses the raw (unwrapped, open-space) coordinate
             const double x = Rw[(size_t)n * D + a];
-            gr[a] = gsum[n * 2 * D + a] - 2.0 * obR * x;
-            gi[a] = gsum[n * 2 * D + D + a] - 2.0 * obI * x;
+            gr[a] = (double)gsum[a * N + n] - 2.0 * obR * x;
+            gi[a] = (double)gsum[(D + a) * N + n] - 2.0 * obI * x;
             r2n += x * x;
         }
         sr2 += r2n;
         double lr = 0.0, li = 0.0;
         if (PP) {
-            lr = lsum[2 * n] - 2.0 * D * obR;
-            li = lsum[2 * n + 1] - 2.0 * D * obI;
+            lr = (double)lsum[n] - 2.0 * D * obR;
+            li = (double)lsum[N + n] - 2.0 * D * obI;
             sR2 += lr;
             sI2 += li;
         } else {
@@ -360,13 +442,24 @@ __global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, 
     }
     if (sy.trap_omega != 0.0) vext = 0.5 * sy.trap_omega * sy.trap_omega * sr2;
 
-    // spline basis sums -> O_pair (BC map), exponent contribution
+    // spline basis sums -> O_pair (BC map), exponent contribution.  Knot
+    // interval j carries the pieces of B_j .. B_{j+3}; piece m is the cubic
+    // sum_p beta[m][p] t^p, so B_s = sum_m sum_p beta[m][p] M[s-m][p] with the
+    // interval moments M[j][p] = sum over pairs in j of t^p.
     double ex_p = 0.0;
     for (int k = tid; k < Ps; k += kThreads) {
+        constexpr double beta[4][4] = {{1.0 / 6, -3.0 / 6, 3.0 / 6, -1.0 / 6},
+                                       {4.0 / 6, 0.0, -6.0 / 6, 3.0 / 6},
+                                       {1.0 / 6, 3.0 / 6, 3.0 / 6, -3.0 / 6},
+                                       {0.0, 0.0, 0.0, 1.0 / 6}};
         auto B = [&](int s) {
             double v = 0.0;
+            for (int m = 0; m < 4; m++) {
+                const int j = s - m;
+                if (j < 0 || j > Ps - 2) continue;
 #pragma unroll
-            for (int c = 0; c < kHistCols; c++) v += hist[s * kHistCols + c];
+                for (int p = 0; p < 4; p++) v += beta[m][p] * hist[(j * 4 + p) * kHistCols];
+            }
             return v;
         };
         double o;
@@ -421,9 +514,12 @@ __global__ __launch_bounds__(kThreads) void eval_kernel(KSys sy, KTables<T> tb, 
 template <typename T>
 size_t eval_lds_bytes(const KSys& sy, bool per_particle)
 {
-    const size_t nd = (size_t)sy.N * 2 * sy.D + (per_particle ? 2 * (size_t)sy.N : 0) + (size_t)(sy.P_s + 2) * kHistCols;
-    const size_t nt = (size_t)sy.N * sy.D + 2 * (size_t)sy.K + (size_t)(sy.P_s - 1) * 8;
-    return 64 + nd * sizeof(double) + nt * sizeof(T);
+    const size_t nd = (size_t)(sy.P_s - 1) * 4 * kHistCols + (size_t)sy.N * 2 * sy.D +
+                      (per_particle ? 2 * (size_t)sy.N : 0);
+    const size_t nt = (size_t)sy.N * sy.D + 2 * (size_t)sy.Kh;
+    // the derivative table (8 entries per interval) starts 16-byte aligned
+    const size_t head = (64 + nd * sizeof(double) + nt * sizeof(T) + 15) & ~(size_t)15;
+    return head + (size_t)(sy.P_s - 1) * 8 * sizeof(T);
 }
 
 template <typename T, int D, bool PP, bool PER, typename TO>

@@ -22,25 +22,42 @@ __device__ double spline_coeff(const double* u, int Ps, int s)
 
 template <typename T>
 __global__ void prep_tables_kernel(KSys sy, const double* __restrict__ uR, const double* __restrict__ uI,
-                                   T* __restrict__ spl, T* __restrict__ alpha)
+                                   const int* __restrict__ kmap, T* __restrict__ spl, T* __restrict__ dspl,
+                                   T* __restrict__ alpha)
 {
     const int Ps = sy.P_s;
-    for (int j = threadIdx.x + blockIdx.x * blockDim.x; j < Ps - 1 + sy.K; j += blockDim.x * gridDim.x) {
+    for (int j = threadIdx.x + blockIdx.x * blockDim.x; j < Ps - 1 + sy.Kh; j += blockDim.x * gridDim.x) {
         if (j < Ps - 1) {
             for (int part = 0; part < 2; part++) {
                 const double* u = (part == 0 ? uR : uI) + sy.off_pair;
                 const double c0 = spline_coeff(u, Ps, j), c1 = spline_coeff(u, Ps, j + 1);
                 const double c2 = spline_coeff(u, Ps, j + 2), c3 = spline_coeff(u, Ps, j + 3);
                 // u(t) on knot interval j for the four uniform cubic B-spline pieces
+                const double A1 = (c2 - c0) / 2.0, A2 = (c0 - 2.0 * c1 + c2) / 2.0;
+                const double A3 = (-c0 + 3.0 * c1 - 3.0 * c2 + c3) / 6.0;
                 spl[j * 8 + part * 4 + 0] = (T)((c0 + 4.0 * c1 + c2) / 6.0);
-                spl[j * 8 + part * 4 + 1] = (T)((c2 - c0) / 2.0);
-                spl[j * 8 + part * 4 + 2] = (T)((c0 - 2.0 * c1 + c2) / 2.0);
-                spl[j * 8 + part * 4 + 3] = (T)((-c0 + 3.0 * c1 - 3.0 * c2 + c3) / 6.0);
+                spl[j * 8 + part * 4 + 1] = (T)A1;
+                spl[j * 8 + part * 4 + 2] = (T)A2;
+                spl[j * 8 + part * 4 + 3] = (T)A3;
+                // derivatives per unit length, t = (r - knot_j) / h:
+                // u' = d1 + t (d2 + t d3), u'' = hinv d2 + t e3
+                const double hi = sy.hinv;
+                dspl[j * 8 + part * 4 + 0] = (T)(A1 * hi);
+                dspl[j * 8 + part * 4 + 1] = (T)(2.0 * A2 * hi);
+                dspl[j * 8 + part * 4 + 2] = (T)(3.0 * A3 * hi);
+                dspl[j * 8 + part * 4 + 3] = (T)(6.0 * A3 * hi * hi);
             }
         } else {
+            // amplitude of |rho_k|^2: alpha_k, plus alpha_{-k} when -k is in the set
             const int k = j - (Ps - 1);
-            alpha[2 * k] = (T)uR[sy.off_k + k];
-            alpha[2 * k + 1] = (T)uI[sy.off_k + k];
+            const int m0 = kmap[2 * k], m1 = kmap[2 * k + 1];
+            double aR = uR[sy.off_k + m0], aI = uI[sy.off_k + m0];
+            if (m1 >= 0) {
+                aR += uR[sy.off_k + m1];
+                aI += uI[sy.off_k + m1];
+            }
+            alpha[2 * k] = (T)aR;
+            alpha[2 * k + 1] = (T)aI;
         }
     }
 }
@@ -142,15 +159,18 @@ hipError_t launch_adjoint_outer(const double* outer, const double* gbar, int64_t
 }
 
 template <typename T>
-hipError_t launch_prep_tables(const KSys& sy, const double* uR, const double* uI, T* spl, T* alpha, hipStream_t st)
+hipError_t launch_prep_tables(const KSys& sy, const double* uR, const double* uI, const int* kmap, T* spl, T* dspl,
+                              T* alpha, hipStream_t st)
 {
-    const int n = sy.P_s - 1 + sy.K;
+    const int n = sy.P_s - 1 + sy.Kh;
     const int blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(prep_tables_kernel<T>, dim3(blocks), dim3(256), 0, st, sy, uR, uI, spl, alpha);
+    hipLaunchKernelGGL(prep_tables_kernel<T>, dim3(blocks), dim3(256), 0, st, sy, uR, uI, kmap, spl, dspl, alpha);
     return hipGetLastError();
 }
-template hipError_t launch_prep_tables<float>(const KSys&, const double*, const double*, float*, float*, hipStream_t);
-template hipError_t launch_prep_tables<double>(const KSys&, const double*, const double*, double*, double*, hipStream_t);
+template hipError_t launch_prep_tables<float>(const KSys&, const double*, const double*, const int*, float*, float*,
+                                              float*, hipStream_t);
+template hipError_t launch_prep_tables<double>(const KSys&, const double*, const double*, const int*, double*,
+                                               double*, double*, hipStream_t);
 
 hipError_t launch_axpy_params(int P, const double* x0, const double* xdot, double c, double* y, hipStream_t st)
 {

@@ -7,7 +7,8 @@
 // incremental ratio of src/PhysicalSystems/BosonsBulk.cpp:553-657
 // (CalculateWFChange: old and new pair terms of the moved particle, pairs
 // beyond the cutoff contribute the last pair parameter), plus the k-space
-// term (incremental rho_k update) and the one-body factor.
+// term (incremental rho_k update, one representative per +-k pair with the
+// summed amplitude) and the one-body factor.
 //
 // Mapping: a walker is a group of LPW lanes (LPW = 16, 32 or 64) inside a
 // wavefront; a 256-thread workgroup holds 256/LPW walkers.  The walker's
@@ -65,7 +66,7 @@ template <typename T, int D, int LPW, int KPL, bool PER>
 __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb, SweepArgs a)
 {
     constexpr int WPB = kThreads / LPW;   // walkers per block
-    const int N = sy.N, Ps = sy.P_s, K = sy.K;
+    const int N = sy.N, Ps = sy.P_s, K = sy.Kh;   // k-space: one representative per +-k pair
     const int tid = threadIdx.x;
     const int grp = tid / LPW, gl = tid % LPW;
     const int64_t w = (int64_t)blockIdx.x * WPB + grp;
@@ -299,9 +300,9 @@ int pick_lpw(int N, int K)
 template <typename T>
 size_t sweep_lds_bytes(const KSys& sy, int lpw)
 {
-    if (lpw <= 0) lpw = pick_lpw(sy.N, sy.K);
+    if (lpw <= 0) lpw = pick_lpw(sy.N, sy.Kh);
     const size_t wpb = kThreads / lpw;
-    return sizeof(T) * (4 * (size_t)(sy.P_s - 1) + 4 * (size_t)sy.K + (((size_t)sy.K + 3) & ~(size_t)3) +
+    return sizeof(T) * (4 * (size_t)(sy.P_s - 1) + 4 * (size_t)sy.Kh + (((size_t)sy.Kh + 3) & ~(size_t)3) +
                         wpb * sy.N * sy.D);
 }
 
@@ -323,7 +324,7 @@ static hipError_t launch_sweep_k(const KSys& sy, const KTables<T>& tb, const Swe
 template <typename T, int D, int LPW, bool PER>
 static hipError_t launch_sweep_l(const KSys& sy, const KTables<T>& tb, const SweepArgs& a, hipStream_t st)
 {
-    const int kpl = (sy.K + LPW - 1) / LPW;
+    const int kpl = (sy.Kh + LPW - 1) / LPW;
     if (kpl == 0) return launch_sweep_k<T, D, LPW, 0, PER>(sy, tb, a, st);
     if constexpr (!PER) {
         return hipErrorInvalidValue;   // k-space terms need the periodic box
@@ -338,7 +339,7 @@ static hipError_t launch_sweep_l(const KSys& sy, const KTables<T>& tb, const Swe
 template <typename T, int D>
 static hipError_t launch_sweep_d(const KSys& sy, const KTables<T>& tb, const SweepArgs& a, hipStream_t st)
 {
-    const int lpw = a.lanes_per_walker > 0 ? a.lanes_per_walker : pick_lpw(sy.N, sy.K);
+    const int lpw = a.lanes_per_walker > 0 ? a.lanes_per_walker : pick_lpw(sy.N, sy.Kh);
     const bool per = sy.periodic != 0;
     switch (lpw) {
     case 16: return per ? launch_sweep_l<T, D, 16, true>(sy, tb, a, st) : launch_sweep_l<T, D, 16, false>(sy, tb, a, st);
