@@ -370,6 +370,10 @@ int build_system(tdvmc_ctx* ctx)
         ctx->err = "system too large for the evaluation kernel's LDS budget";
         return TDVMC_ERR_BAD_CONFIG;
     }
+    if (sweep_lds_bytes<double>(s, 16) > lds_max) {
+        ctx->err = "system too large for the sweep kernel's LDS budget (spline table or particle count)";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
     return TDVMC_OK;
 }
 

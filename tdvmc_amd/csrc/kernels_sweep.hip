@@ -21,8 +21,11 @@
 // trial, call epoch, walker): each lane draws the numbers of one future
 // trial, and trial t of a batch reads them from lane t (v_readlane when the
 // group is the whole wavefront).  The pair terms are branch-free.  In a
-// periodic box the coordinates are held in box units (x / L), so the minimum
-// image is d - rint(d) and the phase in revolutions is n . s.
+// periodic box the coordinates are held in box units (x / L): as 32-bit
+// fractions of the box in the FP32 form (the wrapping integer difference is
+// the minimum image), as d - rint(d) in FP64; the phase in revolutions is
+// n . s, and the spline table runs out to the largest minimum-image distance
+// with the outer-pair value past the cutoff.
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -35,6 +38,51 @@ constexpr int kThreads = 256;
 
 template <typename T>
 struct Vec4 { T x, y, z, w; };
+
+// Storage of one coordinate in LDS.  FP32 in a periodic box: the position as
+// an unsigned 32-bit fraction of the box, so a wrapping integer difference is
+// the minimum image (one subtract and one convert per component, exact);
+// distances come out in units of 2^-32 box lengths.  Otherwise the compute
+// type in box units (periodic, minimum image by rint) or in real units.
+template <typename T, bool PER>
+struct Coord {
+    using S = T;
+    static constexpr double kUnit = 1.0;                 // length unit of diff() in box (or real) units
+    __device__ static S load(double x) { return (T)(PER ? x - rint(x) : x); }
+    __device__ static double store(S p) { return (double)p; }
+    __device__ static T diff(S a, S b)
+    {
+        T d = a - b;
+        if (PER) d -= Prec<T>::rint_(d);
+        return d;
+    }
+    __device__ static S move(S p, T dz)
+    {
+        T v = p + dz;
+        if (PER) v -= Prec<T>::rint_(v);
+        return v;
+    }
+    __device__ static T unit_coord(S p) { return p; }   // in box (or real) units
+};
+template <>
+struct Coord<float, true> {
+    using S = uint32_t;
+    static constexpr double kUnit = 2.3283064365386963e-10;   // 2^-32
+    __device__ static S load(double x)
+    {
+        const double f = x - floor(x);   // [0, 1)
+        return (uint32_t)(uint64_t)(f * 4294967296.0);
+    }
+    __device__ static double store(S p) { return (double)(int32_t)p * kUnit; }   // [-1/2, 1/2)
+    __device__ static float diff(S a, S b) { return (float)(int32_t)(a - b); }
+    __device__ static S move(S p, float dz)
+    {
+        // dz in box units; only its image in [-1/2, 1/2) matters
+        const float w = dz - __builtin_rintf(dz);
+        return p + (uint32_t)__float2int_rn(w * 4294967296.0f);
+    }
+    __device__ static float unit_coord(S p) { return (float)(int32_t)p * (float)kUnit; }
+};
 
 template <int LPW, typename T>
 __device__ __forceinline__ T bcast(T v, int src)
@@ -49,11 +97,20 @@ __device__ __forceinline__ int bcast_i(int v, int src)
     return __shfl(v, src, LPW);
 }
 
-// u(r) on the spline table, or the outer-pair value beyond the cutoff
-template <typename T>
+// u(r) from the spline table.  Periodic box: the table extends over every
+// possible minimum-image distance, entries past the cutoff hold the
+// outer-pair value, so the lookup needs no clamp or select.  Open space:
+// clamp to the last interval and select the outer value past the cutoff.
+template <typename T, bool PER>
 __device__ __forceinline__ T pair_u(T r, T rc, T hinv, int Ps, const Vec4<T>* spl, T u_out)
 {
-    T x = r * hinv;
+    const T x = r * hinv;
+    if (PER) {
+        const int j = (int)x;
+        const T t = x - (T)j;
+        const Vec4<T> cf = spl[j];
+        return cf.x + t * (cf.y + t * (cf.z + t * cf.w));
+    }
     int j = (int)x;
     j = j > Ps - 2 ? Ps - 2 : j;
     const T t = x - (T)j;
@@ -62,27 +119,46 @@ __device__ __forceinline__ T pair_u(T r, T rc, T hinv, int Ps, const Vec4<T>* sp
     return r <= rc ? u : u_out;
 }
 
+// Intervals of the (extended) spline table in LDS.
+__host__ __device__ inline int sweep_table_len(const KSys& sy)
+{
+    if (!sy.periodic) return sy.P_s - 1;
+    // largest minimum-image distance sqrt(D)/2 L, in knot intervals, + margin
+    const double xmax = 0.5 * sqrt((double)sy.D) * sy.L * sy.hinv;
+    const int j = (int)xmax + 2;
+    return j > sy.P_s - 1 ? j : sy.P_s - 1;
+}
+
 template <typename T, int D, int LPW, int KPL, bool PER>
 __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb, SweepArgs a)
 {
+    using C = Coord<T, PER>;
+    using S = typename C::S;
     constexpr int WPB = kThreads / LPW;   // walkers per block
     const int N = sy.N, Ps = sy.P_s, K = sy.Kh;   // k-space: one representative per +-k pair
+    const int NT = sweep_table_len(sy);
     const int tid = threadIdx.x;
     const int grp = tid / LPW, gl = tid % LPW;
     const int64_t w = (int64_t)blockIdx.x * WPB + grp;
 
     extern __shared__ __align__(16) unsigned char smem[];
-    Vec4<T>* spl = reinterpret_cast<Vec4<T>*>(smem);                      // (Ps-1) x (A0..A3) of Re u
-    Vec4<T>* kvl = spl + (Ps - 1);                                         // K x (n/L, |k|^2)
+    Vec4<T>* spl = reinterpret_cast<Vec4<T>*>(smem);                      // NT x (A0..A3) of Re u
+    Vec4<T>* kvl = spl + NT;                                               // K x (n/L, |k|^2)
     T* all = reinterpret_cast<T*>(kvl + K);                                // K x Re alpha
-    T* posb = all + ((K + 3) & ~3);                                        // WPB x D x N (structure of arrays)
-    T* pos = posb + (size_t)grp * N * D;
+    S* posb = reinterpret_cast<S*>(all + ((K + 3) & ~3));                  // WPB x D x N (structure of arrays)
+    S* pos = posb + (size_t)grp * N * D;
 
-    for (int i = tid; i < Ps - 1; i += kThreads) {
-        spl[i].x = tb.spl[i * 8 + 0];
-        spl[i].y = tb.spl[i * 8 + 1];
-        spl[i].z = tb.spl[i * 8 + 2];
-        spl[i].w = tb.spl[i * 8 + 3];
+    const T u_out = (T)tb.uR[sy.outer_idx];
+    for (int i = tid; i < NT; i += kThreads) {
+        if (i < Ps - 1) {
+            spl[i].x = tb.spl[i * 8 + 0];
+            spl[i].y = tb.spl[i * 8 + 1];
+            spl[i].z = tb.spl[i * 8 + 2];
+            spl[i].w = tb.spl[i * 8 + 3];
+        } else {
+            spl[i].x = u_out;
+            spl[i].y = spl[i].z = spl[i].w = (T)0;
+        }
     }
     for (int i = tid; i < K; i += kThreads) {
         kvl[i].x = tb.kv[i * 4 + 0];
@@ -91,25 +167,22 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
         kvl[i].w = tb.kv[i * 4 + 3];
         all[i] = tb.alpha[2 * i];
     }
-    // lengths in box units when periodic (scale = 1/L), unscaled otherwise
+    // lengths in box units when periodic (scale = 1/L), unscaled otherwise;
+    // diff() returns multiples of C::kUnit of that unit
     const double scale = PER ? sy.Linv : 1.0;
     const bool active = w < a.W;
     double* Rw = a.R + (active ? w : 0) * (size_t)N * D;
     if (active) {
         for (int i = gl; i < N; i += LPW) {
 #pragma unroll
-            for (int c = 0; c < D; c++) {
-                double x = Rw[(size_t)i * D + c] * scale;
-                if (PER) x = x - rint(x);
-                pos[c * N + i] = (T)x;
-            }
+            for (int c = 0; c < D; c++) pos[c * N + i] = C::load(Rw[(size_t)i * D + c] * scale);
         }
     }
     __syncthreads();
     if (!active) return;
 
-    const T hinv = (T)(sy.hinv / scale), rc = (T)(sy.rc * scale), step = (T)(sy.step * scale);
-    const T u_out = (T)tb.uR[sy.outer_idx];
+    const T hinv = (T)(sy.hinv / scale * C::kUnit), rc = (T)(sy.rc * scale / C::kUnit);
+    const T step = (T)(sy.step * scale);
     const T ob = sy.one_body ? (T)(tb.uR[0] / (scale * scale)) : (T)0;
     const T kscale = (T)(1.0 / scale);   // phase in revolutions = (kv / scale) . s
 
@@ -125,9 +198,9 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
             q.y *= kscale;
             q.z *= kscale;
             for (int n = 0; n < N; n++) {
-                T ph = q.x * pos[n];
-                if (D > 1) ph += q.y * pos[N + n];
-                if (D > 2) ph += q.z * pos[2 * N + n];
+                T ph = q.x * C::unit_coord(pos[n]);
+                if (D > 1) ph += q.y * C::unit_coord(pos[N + n]);
+                if (D > 2) ph += q.z * C::unit_coord(pos[2 * N + n]);
                 T s, c;
                 Prec<T>::sincos_rev(ph, &s, &c);
                 rhoR[m] += c;
@@ -185,13 +258,11 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
             for (int c = 0; c < D; c++) dz[c] = bcast<LPW>(z[c], tt);
             const T lu = bcast<LPW>(logu, tt);
 
-            T old[D], nw[D];
+            S old[D], nw[D];
 #pragma unroll
             for (int c = 0; c < D; c++) {
                 old[c] = pos[c * N + p];
-                T v = old[c] + step * dz[c];
-                if (PER) v = v - Prec<T>::rint_(v);
-                nw[c] = v;
+                nw[c] = C::move(old[c], step * dz[c]);
             }
             T delta = (T)0;
 #pragma unroll 2
@@ -199,18 +270,20 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
                 T ro2 = (T)0, rn2 = (T)0;
 #pragma unroll
                 for (int c = 0; c < D; c++) {
-                    const T pi = pos[c * N + i];
-                    T dOld = pi - old[c], dNew = pi - nw[c];
-                    if (PER) {
-                        dOld -= Prec<T>::rint_(dOld);
-                        dNew -= Prec<T>::rint_(dNew);
-                    }
+                    const S pi = pos[c * N + i];
+                    const T dOld = C::diff(pi, old[c]), dNew = C::diff(pi, nw[c]);
                     ro2 += dOld * dOld;
                     rn2 += dNew * dNew;
                 }
-                const T uo = pair_u(Prec<T>::sqrt_(ro2), rc, hinv, Ps, spl, u_out);
-                const T un = pair_u(Prec<T>::sqrt_(rn2), rc, hinv, Ps, spl, u_out);
+                const T uo = pair_u<T, PER>(Prec<T>::sqrt_(ro2), rc, hinv, Ps, spl, u_out);
+                const T un = pair_u<T, PER>(Prec<T>::sqrt_(rn2), rc, hinv, Ps, spl, u_out);
                 delta += (i != p) ? un - uo : (T)0;
+            }
+            T oc[D], ncd[D];
+#pragma unroll
+            for (int c = 0; c < D; c++) {
+                oc[c] = C::unit_coord(old[c]);
+                ncd[c] = C::unit_coord(nw[c]);
             }
             T dR[KPL > 0 ? KPL : 1], dI[KPL > 0 ? KPL : 1];
 #pragma unroll
@@ -222,14 +295,14 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
                     q.x *= kscale;
                     q.y *= kscale;
                     q.z *= kscale;
-                    T po = q.x * old[0], pn = q.x * nw[0];
+                    T po = q.x * oc[0], pn = q.x * ncd[0];
                     if (D > 1) {
-                        po += q.y * old[1];
-                        pn += q.y * nw[1];
+                        po += q.y * oc[1];
+                        pn += q.y * ncd[1];
                     }
                     if (D > 2) {
-                        po += q.z * old[2];
-                        pn += q.z * nw[2];
+                        po += q.z * oc[2];
+                        pn += q.z * ncd[2];
                     }
                     T so, co, sn, cn;
                     Prec<T>::sincos_rev(po, &so, &co);
@@ -244,8 +317,8 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
                 T ro2 = (T)0, rn2 = (T)0;
 #pragma unroll
                 for (int c = 0; c < D; c++) {
-                    ro2 += old[c] * old[c];
-                    rn2 += nw[c] * nw[c];
+                    ro2 += oc[c] * oc[c];
+                    rn2 += ncd[c] * ncd[c];
                 }
                 delta += -ob * (rn2 - ro2);
             }
@@ -272,7 +345,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
 
     for (int i = gl; i < N; i += LPW) {
 #pragma unroll
-        for (int c = 0; c < D; c++) Rw[(size_t)i * D + c] = (double)pos[c * N + i] * (PER ? sy.L : 1.0);
+        for (int c = 0; c < D; c++) Rw[(size_t)i * D + c] = C::store(pos[c * N + i]) * (PER ? sy.L : 1.0);
     }
     if (a.rho_out) {
         T* ro = static_cast<T*>(a.rho_out) + (size_t)w * K * 2;
@@ -302,8 +375,10 @@ size_t sweep_lds_bytes(const KSys& sy, int lpw)
 {
     if (lpw <= 0) lpw = pick_lpw(sy.N, sy.Kh);
     const size_t wpb = kThreads / lpw;
-    return sizeof(T) * (4 * (size_t)(sy.P_s - 1) + 4 * (size_t)sy.Kh + (((size_t)sy.Kh + 3) & ~(size_t)3) +
-                        wpb * sy.N * sy.D);
+    // coordinates: uint32 fractions (FP32, periodic) or T
+    const size_t cs = (sizeof(T) == 4 && sy.periodic) ? sizeof(uint32_t) : sizeof(T);
+    return sizeof(T) * (4 * (size_t)sweep_table_len(sy) + 4 * (size_t)sy.Kh + (((size_t)sy.Kh + 3) & ~(size_t)3)) +
+           cs * wpb * sy.N * sy.D;
 }
 
 template <typename T, int D, int LPW, int KPL, bool PER>
