@@ -64,9 +64,11 @@ def run_gpu(cfg, steps, warmup, precision, use_graph=True):
     info = ctx.get_info()
     W = info.n_walkers
     # thermalise the lattice start a little (untimed; launches of the same
-    # shape as the timed ones, N moves each)
+    # shape as the timed ones, N moves each), then let the autotuner pick the
+    # sweep's lanes per walker on these walkers
     for i in range(2):
         ctx.sweep(info.N, seed=11 + i, precision=precision)
+    lanes = ctx.autotune(precision)
     dt = info.timestep
     ctx.profile_enable(True)
     for _ in range(warmup):
@@ -87,7 +89,7 @@ def run_gpu(cfg, steps, warmup, precision, use_graph=True):
     ok = bool(np.all(np.isfinite(uR)))
     ctx.close()
     return dict(elapsed=elapsed, samples=samples, steps=steps, W=W, prof=prof,
-                acceptance=acc / max(trials, 1), finite=ok, info=info)
+                acceptance=acc / max(trials, 1), finite=ok, info=info, lanes=lanes)
 
 
 def copy_probe(mib):
@@ -190,11 +192,12 @@ def main():
     ms_step = mixed["elapsed"] / mixed["steps"] * 1e3
     avg = {k: (v[0] / max(v[1], 1)) for k, v in mixed["prof"].items()}
     launches = {k: v[1] for k, v in mixed["prof"].items()}
-    kpl = next(m for m in (0, 2, 4, 8) if m * 64 >= info.K_terms)   # k terms per lane of the sweep template
+    lpw = mixed["lanes"]
+    kpl = next(m for m in (0, 2, 4, 8) if m * lpw >= info.K_terms)   # k terms per lane of the sweep template
     # algorithmic flops per launch: one sweep = N trial moves of every walker,
     # one measurement = every pair and every (particle, k) term of every walker
     rl = {
-        "sweep": kernel_roofline(f"sweep_kernel<float, {D}, 64, {kpl}, true>", avg["sweep"],
+        "sweep": kernel_roofline(f"sweep_kernel<float, {D}, {lpw}, {kpl}, true>", avg["sweep"],
                                  W * info.therm_moves * sweep_flops_per_move(N, D, K), PEAK_FP32_TFLOPS),
         "eval": kernel_roofline(f"eval_kernel<float, {D}, false, true, float>", avg["eval"],
                                 W * measure_flops(N, D, K), PEAK_FP32_TFLOPS),
@@ -220,7 +223,8 @@ def main():
         "data": "synthetic (McMillan-initialised spline, seeded k-space amplitudes, jittered-lattice walkers)",
         "config": {"workload": args.workload, "N": N, "DIM": D, "P": info.P, "K": K,
                    "walkers": W, "step": "RK4 TDVP step = 4 x (1 sweep of N moves + measure + "
-                   "moments + FP64 solve)", "graph": not args.no_graph},
+                   "moments + FP64 solve)", "graph": not args.no_graph,
+                   "sweep_lanes_per_walker": {"mixed": mixed["lanes"], "fp64": fp64["lanes"] if fp64 else None}},
         "tdvp_steps_per_s": mixed["steps"] / mixed["elapsed"],
         "acceptance": mixed["acceptance"],
         "kernel_ms": avg,

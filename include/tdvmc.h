@@ -203,6 +203,14 @@ int tdvmc_set_sweep_lanes(tdvmc_ctx* ctx, int lanes_per_walker);
 
 int tdvmc_synchronize(tdvmc_ctx* ctx);
 
+/* Autotuner of the sweep's walker-block shape: times one sweep of a scratch
+ * copy of up to 16384 walkers for 16, 32 and 64 lanes per walker (4, 8 or 16
+ * walkers per 256-thread workgroup) and keeps the fastest for this precision
+ * (the context's walkers and random stream are untouched).  A lane count set
+ * with tdvmc_set_sweep_lanes takes precedence.  No counterpart in the
+ * reference (its sampler is one walker per MPI rank). */
+int tdvmc_autotune(tdvmc_ctx* ctx, int precision, int* lanes_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,6 +91,7 @@ SYMBOLS = {
     "tdvmc_profile_get": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "tdvmc_set_sweep_lanes": (C.c_int, [C.c_void_p, C.c_int]),
     "tdvmc_synchronize": (C.c_int, [C.c_void_p]),
+    "tdvmc_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
 }
 
 
@@ -304,6 +305,12 @@ class Context:
         dR, da = np.zeros_like(R), np.zeros(self.P)
         self._check(lib().tdvmc_adjoint(self._h, _ptr(R), W, _ptr(_f64(gbar)), int(precision), _ptr(dR), _ptr(da)))
         return dR, da
+
+    def autotune(self, precision: int = DEFAULT) -> int:
+        """Times the sweep's lanes-per-walker choices and keeps the fastest."""
+        lanes = C.c_int()
+        self._check(lib().tdvmc_autotune(self._h, int(precision), C.byref(lanes)))
+        return lanes.value
 
     def profile_enable(self, on: bool = True):
         self._check(lib().tdvmc_profile_enable(self._h, int(on)))

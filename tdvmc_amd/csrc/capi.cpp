@@ -55,7 +55,8 @@ struct tdvmc_ctx {
     double* dXprev = nullptr;         // parameters before the step (acceptance check)
     int64_t therm_moves = 0, init_moves = 0;
     uint64_t seed = 1;
-    int lanes = 0;
+    int lanes = 0;                    // lanes per walker set by the caller (0: tuned or default)
+    int lanes_tuned[2] = {0, 0};      // chosen by tdvmc_autotune per precision (0: not tuned)
     std::string err;
     hipStream_t stream = nullptr;
     int device = 0;
@@ -593,7 +594,7 @@ int enqueue_sweep(tdvmc_ctx* ctx, int prec, int64_t n_moves, uint64_t seed, bool
     a.seed = seed;
     a.epoch = ctx->dEpoch;
     a.acc = ctx->dAcc;
-    a.lanes_per_walker = ctx->lanes;
+    a.lanes_per_walker = ctx->lanes ? ctx->lanes : ctx->lanes_tuned[prec == TDVMC_MIXED ? 1 : 0];
     hipEvent_t ev{};
     int rc = prof_begin(ctx, "sweep", &ev);
     if (rc) return rc;
@@ -1354,8 +1355,9 @@ int tdvmc_tdvp_step(tdvmc_ctx* ctx, double dt, int ode_type, int precision, int 
     CHECK_ARG(ode == 0 || ode == 2, "ODE type must be 0 (Euler) or 2 (RK4)");
     if (use_graph) {
         char key[256];
-        std::snprintf(key, sizeof(key), "%d/%d/%.17g/%lld/%.17g/%.17g/%d", prec, ode, dt, (long long)ctx->W,
-                      ctx->sy.pot_a, ctx->sy.pot_b, ctx->prof ? 1 : 0);
+        std::snprintf(key, sizeof(key), "%d/%d/%.17g/%lld/%.17g/%.17g/%d/%d/%d/%d", prec, ode, dt, (long long)ctx->W,
+                      ctx->sy.pot_a, ctx->sy.pot_b, ctx->prof ? 1 : 0, ctx->lanes, ctx->lanes_tuned[0],
+                      ctx->lanes_tuned[1]);
         if (!ctx->graph || ctx->graph_key != key) {
             rc = prof_flush(ctx);
             if (rc) return rc;
@@ -1540,6 +1542,74 @@ int tdvmc_set_sweep_lanes(tdvmc_ctx* ctx, int lanes_per_walker)
         CHECK_ARG(kpl <= 8, "too many k-vectors for this lane count");
     }
     ctx->lanes = lanes_per_walker;
+    return TDVMC_OK;
+}
+
+// Times one sweep (N moves) of a scratch copy of up to 16384 walkers for each
+// admissible lane count and keeps the fastest for this precision.  The
+// walkers, acceptance counts and random stream of the context are untouched.
+int tdvmc_autotune(tdvmc_ctx* ctx, int precision, int* lanes_out)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    int prec, rc = resolve_precision(ctx, precision, &prec);
+    if (rc) return rc;
+    rc = prof_flush(ctx);
+    if (rc) return rc;
+    if (prec == TDVMC_MIXED) {
+        rc = ensure_tables(ctx, ctx->tf, false);
+    } else {
+        rc = ensure_tables(ctx, ctx->td, false);
+    }
+    if (rc) return rc;
+    const int N = ctx->sy.N, D = ctx->sy.D;
+    const int64_t Wt = std::min<int64_t>(ctx->W, 16384);
+    double* scratch = nullptr;
+    HIPC(dalloc(&scratch, (size_t)Wt * N * D));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    int best = 0;
+    float best_ms = 0.f;
+    for (int lanes : {16, 32, 64}) {
+        if (e != hipSuccess) break;
+        if ((ctx->sy.Kh + lanes - 1) / lanes > 8) continue;
+        SweepArgs a;
+        a.R = scratch;
+        a.W = Wt;
+        a.n_moves = N;
+        a.seed = 0x9e3779b97f4a7c15ull;
+        a.epoch = ctx->dEpoch;
+        a.acc = nullptr;
+        a.lanes_per_walker = lanes;
+        a.rho_out = nullptr;
+        float ms = 0.f;
+        for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
+            e = hipMemcpyAsync(scratch, ctx->dR, (size_t)Wt * N * D * sizeof(double), hipMemcpyDeviceToDevice,
+                               ctx->stream);
+            if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
+            if (e == hipSuccess)
+                e = prec == TDVMC_MIXED ? launch_sweep<float>(ctx->sy, tables_of(ctx, ctx->tf), a, ctx->stream)
+                                        : launch_sweep<double>(ctx->sy, tables_of(ctx, ctx->td), a, ctx->stream);
+            if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float t = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+            if (rep > 0) ms += t;   // the first launch warms up
+        }
+        if (e == hipSuccess && (best == 0 || ms < best_ms)) {
+            best = lanes;
+            best_ms = ms;
+        }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    dfree(scratch);
+    if (e != hipSuccess) {
+        ctx->err = std::string("autotune: ") + hipGetErrorString(e);
+        return TDVMC_ERR_HIP;
+    }
+    ctx->lanes_tuned[prec == TDVMC_MIXED ? 1 : 0] = best;
+    if (lanes_out) *lanes_out = best;
     return TDVMC_OK;
 }
 

@@ -361,9 +361,13 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
     if (gl == 0 && a.acc) a.acc[w] += nacc;
 }
 
+// Default lanes per walker without tuning: 32 from N = 24 on (two walkers
+// per wavefront share the per-trial overhead and waste fewer partner slots
+// than one walker per wavefront), 16 below; more when the k-space terms would
+// need more than 8 per lane.
 int pick_lpw(int N, int K)
 {
-    int l = N >= 96 ? 64 : (N >= 24 ? 32 : 16);
+    int l = N >= 24 ? 32 : 16;
     while (l < 64 && K > 8 * l) l *= 2;
     return l;
 }

@@ -135,3 +135,25 @@ def test_sampling_stage_rho_handoff(precision):
     tol = 1e-12 if precision == 0 else 2e-5
     assert np.allclose(st["localOperators"], ev["O"].mean(0), rtol=tol, atol=tol)
     assert abs(st["localEnergyR"] - ev["ER"].mean()) < tol * max(1.0, abs(ev["ER"].mean())) * 10
+
+
+def test_autotune_keeps_walkers_and_stream():
+    """The autotuner times sweeps of a scratch copy: the context's walkers,
+    acceptance counts and random stream are the same as without tuning, and
+    the chosen lane count is one of 16 / 32 / 64."""
+    cfg = make_cfg(N=40, D=3, L=5.0, P_s=8, pot="GAUSSIAN", sysparams=(0.5, 1.0), K=10, W=700, step=0.3)
+    res = []
+    for tune in (False, True):
+        with tdvmc.Context(cfg) as ctx:
+            ctx.init_walkers(5)
+            R0 = ctx.get_walkers()
+            if tune:
+                print("autotune", flush=True)
+                lanes = ctx.autotune(tdvmc.FP64)
+                assert lanes in (16, 32, 64)
+                assert np.array_equal(ctx.get_walkers(), R0)
+                assert ctx.acceptance() == (0, 0)
+                ctx.set_sweep_lanes(32)           # same lane count as the default: identical chain
+            ctx.sweep(80, seed=3, precision=tdvmc.FP64)
+            res.append(ctx.get_walkers())
+    assert np.array_equal(res[0], res[1])
