@@ -113,6 +113,14 @@ template <> struct Prec<float> {
         *s = __builtin_amdgcn_sinf(t);
         *c = __builtin_amdgcn_cosf(t);
     }
+    // sin / cos of 2*pi*t for |t| <= 256 revolutions (v_sin / v_cos reduce
+    // the argument themselves; within 1 ulp of the reduced form)
+    __device__ static __forceinline__ void sincos_small(float t, float* s, float* c)
+    {
+        *s = __builtin_amdgcn_sinf(t);
+        *c = __builtin_amdgcn_cosf(t);
+    }
+    __device__ static __forceinline__ float fract_(float x) { return __builtin_amdgcn_fractf(x); }
     __device__ static __forceinline__ float exp_(float x) { return __expf(x); }
     __device__ static __forceinline__ float log_(float x) { return __logf(x); }
 };
@@ -132,6 +140,8 @@ template <> struct Prec<double> {
         t = t - __builtin_rint(t);
         sincospi(2.0 * t, s, c);
     }
+    __device__ static __forceinline__ void sincos_small(double t, double* s, double* c) { sincos_rev(t, s, c); }
+    __device__ static __forceinline__ double fract_(double x) { return x - __builtin_floor(x); }
     __device__ static __forceinline__ double exp_(double x) { return exp(x); }
     __device__ static __forceinline__ double log_(double x) { return log(x); }
 };

@@ -107,7 +107,7 @@ __device__ __forceinline__ T pair_u(T r, T rc, T hinv, int Ps, const Vec4<T>* sp
     const T x = r * hinv;
     if (PER) {
         const int j = (int)x;
-        const T t = x - (T)j;
+        const T t = Prec<T>::fract_(x);
         const Vec4<T> cf = spl[j];
         return cf.x + t * (cf.y + t * (cf.z + t * cf.w));
     }
@@ -186,17 +186,23 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
     const T ob = sy.one_body ? (T)(tb.uR[0] / (scale * scale)) : (T)0;
     const T kscale = (T)(1.0 / scale);   // phase in revolutions = (kv / scale) . s
 
-    // rho_k of this lane's k-vectors
+    // this lane's k-vectors (n, in box units) and amplitudes, and their rho_k
     T rhoR[KPL > 0 ? KPL : 1], rhoI[KPL > 0 ? KPL : 1];
+    T qx[KPL > 0 ? KPL : 1], qy[KPL > 0 ? KPL : 1], qz[KPL > 0 ? KPL : 1], qa[KPL > 0 ? KPL : 1];
 #pragma unroll
     for (int m = 0; m < KPL; m++) {
         rhoR[m] = rhoI[m] = (T)0;
         const int k = gl + m * LPW;
+        qx[m] = qy[m] = qz[m] = qa[m] = (T)0;   // lanes past K: zero amplitude, no effect
         if (k < K) {
             Vec4<T> q = kvl[k];
             q.x *= kscale;
             q.y *= kscale;
             q.z *= kscale;
+            qx[m] = q.x;
+            qy[m] = q.y;
+            qz[m] = q.z;
+            qa[m] = all[k];
             for (int n = 0; n < N; n++) {
                 T ph = q.x * C::unit_coord(pos[n]);
                 if (D > 1) ph += q.y * C::unit_coord(pos[N + n]);
@@ -288,30 +294,23 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
             T dR[KPL > 0 ? KPL : 1], dI[KPL > 0 ? KPL : 1];
 #pragma unroll
             for (int m = 0; m < KPL; m++) {
-                dR[m] = dI[m] = (T)0;
-                const int k = gl + m * LPW;
-                if (k < K) {
-                    Vec4<T> q = kvl[k];
-                    q.x *= kscale;
-                    q.y *= kscale;
-                    q.z *= kscale;
-                    T po = q.x * oc[0], pn = q.x * ncd[0];
-                    if (D > 1) {
-                        po += q.y * oc[1];
-                        pn += q.y * ncd[1];
-                    }
-                    if (D > 2) {
-                        po += q.z * oc[2];
-                        pn += q.z * ncd[2];
-                    }
-                    T so, co, sn, cn;
-                    Prec<T>::sincos_rev(po, &so, &co);
-                    Prec<T>::sincos_rev(pn, &sn, &cn);
-                    dR[m] = cn - co;
-                    dI[m] = sn - so;
-                    // |rho + d|^2 - |rho|^2 = 2 Re(conj(rho) d) + |d|^2
-                    delta += all[k] * ((T)2 * (rhoR[m] * dR[m] + rhoI[m] * dI[m]) + dR[m] * dR[m] + dI[m] * dI[m]);
+                // |phase| <= sum |n_a| / 2 revolutions: no range reduction needed
+                T po = qx[m] * oc[0], pn = qx[m] * ncd[0];
+                if (D > 1) {
+                    po += qy[m] * oc[1];
+                    pn += qy[m] * ncd[1];
                 }
+                if (D > 2) {
+                    po += qz[m] * oc[2];
+                    pn += qz[m] * ncd[2];
+                }
+                T so, co, sn, cn;
+                Prec<T>::sincos_small(po, &so, &co);
+                Prec<T>::sincos_small(pn, &sn, &cn);
+                dR[m] = cn - co;
+                dI[m] = sn - so;
+                // |rho + d|^2 - |rho|^2 = 2 Re(conj(rho) d) + |d|^2
+                delta += qa[m] * ((T)2 * (rhoR[m] * dR[m] + rhoI[m] * dI[m]) + dR[m] * dR[m] + dI[m] * dI[m]);
             }
             if (gl == 0 && sy.one_body) {
                 T ro2 = (T)0, rn2 = (T)0;
