@@ -148,6 +148,30 @@ def cpu_baseline(budget_s=15.0):
                        f"single thread, {el:.1f} s")
 
 
+def run_batch_c4(steps, warmup):
+    """64 trapped 2-D drops of uneven N as one batch: FP32 sweeps and
+    measurement, FP64 moments and solves, one RK4 step per iteration."""
+    import tdvmc_amd as t
+    from tdvmc_amd.workloads import trap2d_batch64
+
+    cfgs = trap2d_batch64()
+    with t.Batch(cfgs) as b:
+        b.sweep(0, seed=3, precision=t.MIXED)         # thermalise (each system N moves)
+        for _ in range(warmup):
+            b.tdvp_step(1e-3, ode_type=2, precision=t.MIXED, use_graph=True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.tdvp_step(1e-3, ode_type=2, precision=t.MIXED, use_graph=True)
+        el = time.perf_counter() - t0
+        W = sum(c.info.n_walkers for c in b.systems)
+        Ns = [c.N for c in b.systems]
+    return {"systems": len(cfgs), "walkers": W, "N_range": [min(Ns), max(Ns)],
+            "walker_samples_per_s": 4 * steps * W / el, "tdvp_steps_per_s": steps / el,
+            "ms_per_step": el / steps * 1e3,
+            "step": "RK4: 4 x (one batched sweep of N moves per system + batched measurement + "
+                    "FP64 moments + FP64 solve per system)"}
+
+
 def kernel_roofline(name, avg_ms, flops, peak):
     achieved = flops / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
     return {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
@@ -164,6 +188,7 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched trap workload")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise kernel of this size (counter calibration)")
     args = ap.parse_args()
@@ -238,6 +263,8 @@ def main():
         line["mixed_over_fp64"] = value / v64
         line["fp64_tdvp_steps_per_s"] = fp64["steps"] / fp64["elapsed"]
         line["fp64_kernel_ms"] = {k: (v[0] / max(v[1], 1)) for k, v in fp64["prof"].items()}
+    if not args.no_batch:
+        line["batch_trap2d_64"] = run_batch_c4(max(1, args.steps // 2), 1)
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
     print(json.dumps(line))
 

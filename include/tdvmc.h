@@ -211,6 +211,31 @@ int tdvmc_synchronize(tdvmc_ctx* ctx);
  * reference (its sampler is one walker per MPI rank). */
 int tdvmc_autotune(tdvmc_ctx* ctx, int precision, int* lanes_out);
 
+/* ---- Batches of independent systems (no counterpart in the reference,
+ * which runs one system per process).  Each config describes one system with
+ * its own particle count, parameters and walkers; all systems of a batch share
+ * DIM and the box kind.  Per stage: one sweep launch and one measurement
+ * launch for all systems, one moments and one solve launch with a workgroup
+ * per system (FP64 moments), the RK arithmetic over all parameters at once.
+ * Member contexts (tdvmc_batch_system) stay usable for parameters, walkers,
+ * info and per-system evaluation; they are owned by the batch. */
+typedef struct tdvmc_batch tdvmc_batch;
+int tdvmc_batch_create(const char* const* configs, int n_systems, tdvmc_batch** out);
+int tdvmc_batch_destroy(tdvmc_batch* b);
+const char* tdvmc_batch_last_error(const tdvmc_batch* b);
+int tdvmc_batch_size(tdvmc_batch* b, int* n_systems);
+tdvmc_ctx* tdvmc_batch_system(tdvmc_batch* b, int s);
+/* lanes per walker of the batched sweep: 16, 32, 64, or 0 for the default */
+int tdvmc_batch_set_sweep_lanes(tdvmc_batch* b, int lanes);
+/* n_moves <= 0: each system's MC_NTHERMSTEPS; system s draws from seed
+ * mixed with s (walker w of system s follows the same random stream as walker
+ * w of a single context seeded with that mix). */
+int tdvmc_batch_sweep(tdvmc_batch* b, int64_t n_moves, uint64_t seed, int precision);
+/* measurement + moments of every system; out [n_systems] (may be NULL) */
+int tdvmc_batch_measure(tdvmc_batch* b, int precision, tdvmc_step_data* out);
+/* one Euler / RK4 TDVP step of every system with its own fresh samples */
+int tdvmc_batch_tdvp_step(tdvmc_batch* b, double dt, int ode_type, int precision, int use_graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,6 +92,15 @@ SYMBOLS = {
     "tdvmc_set_sweep_lanes": (C.c_int, [C.c_void_p, C.c_int]),
     "tdvmc_synchronize": (C.c_int, [C.c_void_p]),
     "tdvmc_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "tdvmc_batch_create": (C.c_int, [C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_void_p)]),
+    "tdvmc_batch_destroy": (C.c_int, [C.c_void_p]),
+    "tdvmc_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "tdvmc_batch_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "tdvmc_batch_system": (C.c_void_p, [C.c_void_p, C.c_int]),
+    "tdvmc_batch_set_sweep_lanes": (C.c_int, [C.c_void_p, C.c_int]),
+    "tdvmc_batch_sweep": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.c_int]),
+    "tdvmc_batch_measure": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "tdvmc_batch_tdvp_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int]),
 }
 
 
@@ -141,13 +150,19 @@ def parse_config(config) -> Info:
 class Context:
     """One system: its walkers, parameters and device buffers (a tdvmc_ctx)."""
 
-    def __init__(self, config):
+    def __init__(self, config, _handle=None, _owner=None):
         L = lib()
-        h = C.c_void_p()
-        rc = L.tdvmc_create(config_text(config).encode(), C.byref(h))
-        if rc != OK:
-            raise TdvmcError(rc, L.tdvmc_last_error(None).decode())
-        self._h = h
+        if _handle is not None:
+            # a member of a Batch: the batch owns (and destroys) it
+            self._h = C.c_void_p(_handle)
+            self._owner = _owner
+        else:
+            h = C.c_void_p()
+            rc = L.tdvmc_create(config_text(config).encode(), C.byref(h))
+            if rc != OK:
+                raise TdvmcError(rc, L.tdvmc_last_error(None).decode())
+            self._h = h
+            self._owner = None
         self.info = self.get_info()
 
     # ------------------------------------------------------------ plumbing
@@ -157,7 +172,8 @@ class Context:
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().tdvmc_destroy(self._h)
+            if getattr(self, "_owner", None) is None:
+                lib().tdvmc_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -322,3 +338,76 @@ class Context:
 
     def synchronize(self):
         self._check(lib().tdvmc_synchronize(self._h))
+
+
+def batch_system_seed(seed: int, s: int) -> int:
+    """Seed of system s in a batched sweep drawn with `seed`."""
+    return (seed ^ ((0x9E3779B97F4A7C15 * (s + 1)) & 0xFFFFFFFFFFFFFFFF)) & 0xFFFFFFFFFFFFFFFF
+
+
+class Batch:
+    """Independent systems sampled, measured and advanced together
+    (tdvmc_batch): one sweep and one measurement launch for all of them."""
+
+    def __init__(self, configs):
+        L = lib()
+        arr = (C.c_char_p * len(configs))(*[config_text(c).encode() for c in configs])
+        h = C.c_void_p()
+        rc = L.tdvmc_batch_create(arr, len(configs), C.byref(h))
+        if rc != OK:
+            raise TdvmcError(rc, L.tdvmc_batch_last_error(None).decode())
+        self._h = h
+        self.systems = [Context(None, _handle=L.tdvmc_batch_system(h, s), _owner=self) for s in range(len(configs))]
+
+    def _check(self, rc):
+        if rc != OK:
+            raise TdvmcError(rc, lib().tdvmc_batch_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for c in self.systems:
+                c._h = None
+            lib().tdvmc_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __len__(self):
+        return len(self.systems)
+
+    def set_sweep_lanes(self, lanes: int):
+        self._check(lib().tdvmc_batch_set_sweep_lanes(self._h, int(lanes)))
+
+    def sweep(self, n_moves: int = 0, seed: int = 1, precision: int = DEFAULT):
+        self._check(lib().tdvmc_batch_sweep(self._h, int(n_moves), int(seed), int(precision)))
+
+    def measure(self, precision: int = DEFAULT):
+        """Per system: dict(mO, mE, S, FR, FI) of the current walkers."""
+        outs, keep = (StepData * len(self.systems))(), []
+        for i, c in enumerate(self.systems):
+            P = c.P
+            arrs = dict(mO=np.zeros(P), S=np.zeros((P, P)), FR=np.zeros(P), FI=np.zeros(P))
+            keep.append(arrs)
+            outs[i].localOperators = _ptr(arrs["mO"])
+            outs[i].S = _ptr(arrs["S"])
+            outs[i].FR = _ptr(arrs["FR"])
+            outs[i].FI = _ptr(arrs["FI"])
+        self._check(lib().tdvmc_batch_measure(self._h, int(precision), C.cast(outs, C.c_void_p)))
+        res = []
+        for i, arrs in enumerate(keep):
+            arrs["mE"] = np.array([outs[i].localEnergyR, outs[i].localEnergyI])
+            res.append(arrs)
+        return res
+
+    def tdvp_step(self, dt: float, ode_type: int = -1, precision: int = DEFAULT, use_graph: bool = False):
+        self._check(lib().tdvmc_batch_tdvp_step(self._h, float(dt), int(ode_type), int(precision), int(use_graph)))

@@ -888,6 +888,285 @@ void release(tdvmc_ctx* ctx)
 
 }  // namespace
 
+
+// ---------------------------------------------------------------------------
+// Batches of independent systems (each a full context for its own buffers and
+// tables) sampled, measured and advanced together: one sweep launch and one
+// measurement launch for all systems, one moments and one solve launch with a
+// workgroup per system, the RK arithmetic on the concatenated parameters.
+struct tdvmc_batch {
+    std::vector<tdvmc_ctx*> sys;
+    std::string err;
+    hipStream_t stream = nullptr;
+    int D = 0, periodic = 0, kh_max = 0, P_max = 0, N_max = 0;
+    int lanes = 0;                    // lanes per walker of the batched sweep (0: default)
+    std::vector<int64_t> xoff;
+    int64_t xtotal = 0;
+    uint64_t seed = 1;
+    KSys* dSys = nullptr;
+    KTables<float>* dTabF = nullptr;
+    KTables<double>* dTabD = nullptr;
+    BatchParams<float>* dBpF = nullptr;
+    BatchParams<double>* dBpD = nullptr;
+    int64_t* dSweepStart = nullptr;
+    int64_t* dEvalStart = nullptr;
+    SweepArgs* dSweepArgs = nullptr;
+    EvalBatchItem<float>* dEvalF = nullptr;
+    EvalBatchItem<double>* dEvalD = nullptr;
+    MomentsItem* dMom = nullptr;
+    SolveArgs* dSolve[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* dX = nullptr;
+    double* dXtmp = nullptr;
+    double* dK[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t* dEpoch = nullptr;
+    int64_t sweep_blocks = 0, eval_blocks = 0;
+    size_t sweep_lds = 0, eval_lds_f = 0, eval_lds_d = 0;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_src = nullptr;
+    std::string graph_key;
+    std::string desc_key;             // configuration the descriptor arrays hold
+};
+
+namespace {
+
+#define BHIPC(expr)                                                                 \
+    do {                                                                            \
+        hipError_t e_ = (expr);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            b->err = std::string(#expr) + ": " + hipGetErrorString(e_);             \
+            return TDVMC_ERR_HIP;                                                   \
+        }                                                                           \
+    } while (0)
+
+// Descriptor arrays are allocated once per batch (their size is fixed by the
+// number of systems) and overwritten in place, so a captured step graph
+// never refers to freed memory.
+template <typename T>
+int upload_array(tdvmc_batch* b, T** dst, const std::vector<T>& src)
+{
+    if (!*dst) BHIPC(dalloc(dst, src.size()));
+    BHIPC(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, b->stream));
+    return TDVMC_OK;
+}
+
+int batch_lanes(const tdvmc_batch* b)
+{
+    return b->lanes ? b->lanes : (b->N_max <= 48 ? 16 : sweep_default_lanes(b->N_max, b->kh_max));
+}
+
+uint64_t batch_system_seed(uint64_t seed, int s) { return seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(s + 1)); }
+
+// Descriptor arrays of a batched sweep with n_moves per system (<= 0: each
+// system's MC_NTHERMSTEPS) and of the measurement, moments and solves.
+int batch_upload(tdvmc_batch* b, int64_t n_moves, uint64_t seed, bool want_rho)
+{
+    char key[96];
+    std::snprintf(key, sizeof(key), "%lld/%llu/%d/%d", (long long)n_moves, (unsigned long long)seed, want_rho ? 1 : 0,
+                  b->lanes);
+    if (b->desc_key == key) return TDVMC_OK;
+    const int n = (int)b->sys.size();
+    const int lanes = batch_lanes(b);
+    const int wpb = sweep_walkers_per_block(lanes);
+    std::vector<KSys> sy(n);
+    std::vector<KTables<float>> tf(n);
+    std::vector<KTables<double>> td(n);
+    std::vector<BatchParams<float>> bpf(n);
+    std::vector<BatchParams<double>> bpd(n);
+    std::vector<int64_t> sstart(n), estart(n);
+    std::vector<SweepArgs> sa(n);
+    std::vector<EvalBatchItem<float>> ef(n);
+    std::vector<EvalBatchItem<double>> ed(n);
+    std::vector<MomentsItem> mi(n);
+    std::vector<SolveArgs> sv[4];
+    int64_t sb = 0, eb = 0;
+    size_t slds = 0, elf = 0, eld = 0;
+    for (int s = 0; s < n; s++) {
+        tdvmc_ctx* c = b->sys[s];
+        sy[s] = c->sy;
+        tf[s] = tables_of(c, c->tf);
+        td[s] = tables_of(c, c->td);
+        bpf[s] = BatchParams<float>{c->sy, c->dUR, c->dUI, c->dKmap, c->tf.spl, c->tf.dspl, c->tf.alpha, c->dX,
+                                    b->xoff[s]};
+        bpd[s] = BatchParams<double>{c->sy, c->dUR, c->dUI, c->dKmap, c->td.spl, c->td.dspl, c->td.alpha, c->dX,
+                                     b->xoff[s]};
+        sstart[s] = sb;
+        sb += (c->W + wpb - 1) / wpb;
+        estart[s] = eb;
+        eb += c->W;
+        slds = std::max(slds, std::max(sweep_lds_bytes<float>(c->sy, lanes), sweep_lds_bytes<double>(c->sy, lanes)));
+        elf = std::max(elf, eval_lds_bytes<float>(c->sy, false));
+        eld = std::max(eld, eval_lds_bytes<double>(c->sy, false));
+        SweepArgs& a = sa[s];
+        a.R = c->dR;
+        a.W = c->W;
+        a.n_moves = n_moves > 0 ? n_moves : c->therm_moves;
+        a.seed = batch_system_seed(seed, s);
+        a.epoch = b->dEpoch;
+        a.acc = c->dAcc;
+        a.lanes_per_walker = lanes;
+        a.rho_out = (want_rho && c->sy.Kh > 0) ? static_cast<void*>(c->dRho) : nullptr;
+        const bool rho = want_rho && c->sy.Kh > 0;
+        ef[s] = EvalBatchItem<float>{c->dR, c->W, c->dExpo, c->dER, c->dEI, c->dO64, c->dOther, nullptr, c->phiR,
+                                     rho ? reinterpret_cast<const float*>(c->dRho) : nullptr};
+        ed[s] = EvalBatchItem<double>{c->dR, c->W, c->dExpo, c->dER, c->dEI, c->dO64, c->dOther, nullptr, c->phiR,
+                                      rho ? c->dRho : nullptr};
+        mi[s] = MomentsItem{c->sy.P, c->W, c->dO64, c->dER, c->dEI, c->dMean, c->dS, c->dF};
+        for (int k = 0; k < 4; k++) {
+            SolveArgs v;
+            v.P = c->sy.P;
+            v.imag_time = c->imag_time;
+            v.precondition = c->precondition;
+            v.eps = c->solver_type == 1 ? (c->precondition ? 0.002 : 0.0) : c->reg_eps;
+            v.mean = c->dMean;
+            v.S = c->dS;
+            v.F = c->dF;
+            v.work = c->dWork;
+            v.xdot = b->dK[k] + b->xoff[s];
+            v.status = nullptr;
+            v.mean_sub = c->solver_type == 1;
+            sv[k].push_back(v);
+        }
+    }
+    b->sweep_blocks = sb;
+    b->eval_blocks = eb;
+    b->sweep_lds = slds;
+    b->eval_lds_f = elf;
+    b->eval_lds_d = eld;
+    int rc;
+    if ((rc = upload_array(b, &b->dSys, sy)) || (rc = upload_array(b, &b->dTabF, tf)) ||
+        (rc = upload_array(b, &b->dTabD, td)) || (rc = upload_array(b, &b->dBpF, bpf)) ||
+        (rc = upload_array(b, &b->dBpD, bpd)) || (rc = upload_array(b, &b->dSweepStart, sstart)) ||
+        (rc = upload_array(b, &b->dEvalStart, estart)) || (rc = upload_array(b, &b->dSweepArgs, sa)) ||
+        (rc = upload_array(b, &b->dEvalF, ef)) || (rc = upload_array(b, &b->dEvalD, ed)) ||
+        (rc = upload_array(b, &b->dMom, mi)))
+        return rc;
+    for (int k = 0; k < 4; k++)
+        if ((rc = upload_array(b, &b->dSolve[k], sv[k]))) return rc;
+    BHIPC(hipStreamSynchronize(b->stream));
+    b->desc_key = key;
+    return TDVMC_OK;
+}
+
+int batch_sweep_launch(tdvmc_batch* b, int prec)
+{
+    const int n = (int)b->sys.size();
+    const int lanes = batch_lanes(b);
+    if (prec == TDVMC_MIXED) {
+        SweepBatch<float> bt{n, b->dSweepStart, b->dSys, b->dTabF, b->dSweepArgs};
+        BHIPC(launch_sweep_batch<float>(bt, b->D, b->periodic, lanes, b->kh_max, b->sweep_blocks, b->sweep_lds,
+                                        b->stream));
+    } else {
+        SweepBatch<double> bt{n, b->dSweepStart, b->dSys, b->dTabD, b->dSweepArgs};
+        BHIPC(launch_sweep_batch<double>(bt, b->D, b->periodic, lanes, b->kh_max, b->sweep_blocks, b->sweep_lds,
+                                         b->stream));
+    }
+    BHIPC(launch_add_u32(b->dEpoch, 1u, b->stream));
+    return TDVMC_OK;
+}
+
+int batch_measure_launch(tdvmc_batch* b, int prec)
+{
+    const int n = (int)b->sys.size();
+    if (prec == TDVMC_MIXED) {
+        EvalBatch<float> bt{n, b->dEvalStart, b->dSys, b->dTabF, b->dEvalF};
+        BHIPC(launch_eval_batch<float>(bt, b->D, b->periodic, b->eval_blocks, b->eval_lds_f, b->stream));
+    } else {
+        EvalBatch<double> bt{n, b->dEvalStart, b->dSys, b->dTabD, b->dEvalD};
+        BHIPC(launch_eval_batch<double>(bt, b->D, b->periodic, b->eval_blocks, b->eval_lds_d, b->stream));
+    }
+    BHIPC(launch_moments_batch(b->dMom, n, b->P_max, b->stream));
+    return TDVMC_OK;
+}
+
+int batch_prep(tdvmc_batch* b, int prec)
+{
+    const int n = (int)b->sys.size();
+    if (prec == TDVMC_MIXED)
+        BHIPC(launch_prep_tables_batch<float>(b->dBpF, n, b->stream));
+    else
+        BHIPC(launch_prep_tables_batch<double>(b->dBpD, n, b->stream));
+    for (auto* c : b->sys) c->tf.dirty = c->td.dirty = true;
+    return TDVMC_OK;
+}
+
+// x (concatenated) -> xdot for stage k: parameters, tables, sweeps, measurement, moments, solves
+int batch_rhs(tdvmc_batch* b, int prec, const double* x, int k)
+{
+    const int n = (int)b->sys.size();
+    BHIPC(launch_unpack_batch<double>(b->dBpD, n, x, b->stream));
+    int rc = batch_prep(b, prec);
+    if (rc) return rc;
+    rc = batch_sweep_launch(b, prec);
+    if (rc) return rc;
+    rc = batch_measure_launch(b, prec);
+    if (rc) return rc;
+    BHIPC(launch_solve_batch(b->dSolve[k], n, b->P_max, b->stream));
+    return TDVMC_OK;
+}
+
+int batch_step_enqueue(tdvmc_batch* b, int prec, int ode, double dt)
+{
+    const int n = (int)b->sys.size();
+    const int P2 = (int)((b->xtotal - 2) / 2);   // axpy / rk4 run over 2 P2 + 2 = xtotal entries
+    BHIPC(launch_gather_batch<double>(b->dBpD, n, b->dX, 1, b->stream));
+    int rc;
+    if (ode == 0) {
+        if ((rc = batch_rhs(b, prec, b->dX, 0))) return rc;
+        BHIPC(launch_axpy_params(P2, b->dX, b->dK[0], dt, b->dX, b->stream));
+    } else {
+        if ((rc = batch_rhs(b, prec, b->dX, 0))) return rc;
+        BHIPC(launch_axpy_params(P2, b->dX, b->dK[0], dt / 2.0, b->dXtmp, b->stream));
+        if ((rc = batch_rhs(b, prec, b->dXtmp, 1))) return rc;
+        BHIPC(launch_axpy_params(P2, b->dX, b->dK[1], dt / 2.0, b->dXtmp, b->stream));
+        if ((rc = batch_rhs(b, prec, b->dXtmp, 2))) return rc;
+        BHIPC(launch_axpy_params(P2, b->dX, b->dK[2], dt, b->dXtmp, b->stream));
+        if ((rc = batch_rhs(b, prec, b->dXtmp, 3))) return rc;
+        BHIPC(launch_rk4_combine(P2, b->dX, b->dK[0], b->dK[1], b->dK[2], b->dK[3], dt, b->stream));
+    }
+    BHIPC(launch_gather_batch<double>(b->dBpD, n, b->dX, 0, b->stream));
+    BHIPC(launch_unpack_batch<double>(b->dBpD, n, b->dX, b->stream));
+    return TDVMC_OK;
+}
+
+void batch_drop_graph(tdvmc_batch* b)
+{
+    if (b->graph) hipGraphExecDestroy(b->graph);
+    if (b->graph_src) hipGraphDestroy(b->graph_src);
+    b->graph = nullptr;
+    b->graph_src = nullptr;
+    b->graph_key.clear();
+}
+
+void batch_release(tdvmc_batch* b)
+{
+    if (b->stream) hipStreamSynchronize(b->stream);
+    batch_drop_graph(b);
+    dfree(b->dSys);
+    dfree(b->dTabF);
+    dfree(b->dTabD);
+    dfree(b->dBpF);
+    dfree(b->dBpD);
+    dfree(b->dSweepStart);
+    dfree(b->dEvalStart);
+    dfree(b->dSweepArgs);
+    dfree(b->dEvalF);
+    dfree(b->dEvalD);
+    dfree(b->dMom);
+    for (int k = 0; k < 4; k++) {
+        dfree(b->dSolve[k]);
+        dfree(b->dK[k]);
+    }
+    dfree(b->dX);
+    dfree(b->dXtmp);
+    dfree(b->dEpoch);
+    for (auto* c : b->sys) tdvmc_destroy(c);
+    b->sys.clear();
+    if (b->stream) hipStreamDestroy(b->stream);
+    b->stream = nullptr;
+}
+
+}  // namespace
+
 extern "C" {
 
 int tdvmc_create(const char* config, tdvmc_ctx** out)
@@ -1618,6 +1897,206 @@ int tdvmc_synchronize(tdvmc_ctx* ctx)
     if (!ctx) return TDVMC_ERR_BAD_ARG;
     HIPC(hipStreamSynchronize(ctx->stream));
     return prof_flush(ctx);
+}
+
+// ---------------------------------------------------------------- batches
+
+int tdvmc_batch_create(const char* const* configs, int n_systems, tdvmc_batch** out)
+{
+    if (!configs || !out || n_systems < 1) {
+        g_create_error = "tdvmc_batch_create: need at least one config";
+        return TDVMC_ERR_BAD_ARG;
+    }
+    *out = nullptr;
+    tdvmc_batch* b = new tdvmc_batch();
+    auto fail = [&](int rc, const std::string& msg) {
+        g_create_error = msg;
+        batch_release(b);
+        delete b;
+        return rc;
+    };
+    for (int s = 0; s < n_systems; s++) {
+        tdvmc_ctx* c = nullptr;
+        const int rc = tdvmc_create(configs[s], &c);
+        if (rc) return fail(rc, "system " + std::to_string(s) + ": " + g_create_error);
+        b->sys.push_back(c);
+        if (s == 0) {
+            b->D = c->sy.D;
+            b->periodic = c->sy.periodic;
+            b->seed = c->seed;
+        } else if (c->sy.D != b->D || c->sy.periodic != b->periodic) {
+            return fail(TDVMC_ERR_BAD_CONFIG, "all systems of a batch need the same DIM and box kind");
+        }
+        if (c->normalize_wf || c->adjust_params || c->accept_check)
+            return fail(TDVMC_ERR_BAD_CONFIG, "USE_NORMALIZE_WF, USE_ADJUST_PARAMETERS and the parameter acceptance "
+                                              "check are not available for batches");
+        if (c->spw != 1) return fail(TDVMC_ERR_BAD_CONFIG, "batches take one sample per walker and stage");
+        b->kh_max = std::max(b->kh_max, c->sy.Kh);
+        b->P_max = std::max(b->P_max, c->sy.P);
+        b->N_max = std::max(b->N_max, c->sy.N);
+        b->xoff.push_back(b->xtotal);
+        b->xtotal += 2 * (int64_t)c->sy.P + 2;
+    }
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(TDVMC_ERR_HIP, "hipStreamCreate failed");
+    if (dalloc(&b->dX, (size_t)b->xtotal) != hipSuccess || dalloc(&b->dXtmp, (size_t)b->xtotal) != hipSuccess ||
+        dalloc(&b->dEpoch, 1) != hipSuccess)
+        return fail(TDVMC_ERR_HIP, "out of device memory");
+    for (int k = 0; k < 4; k++)
+        if (dalloc(&b->dK[k], (size_t)b->xtotal) != hipSuccess) return fail(TDVMC_ERR_HIP, "out of device memory");
+    if (hipMemsetAsync(b->dX, 0, (size_t)b->xtotal * sizeof(double), b->stream) != hipSuccess ||
+        launch_fill_u32(b->dEpoch, 0u, 1, b->stream) != hipSuccess)
+        return fail(TDVMC_ERR_HIP, "initialisation failed");
+    for (auto* c : b->sys)
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(TDVMC_ERR_HIP, "initialisation failed");
+    const int rc = batch_upload(b, 0, b->seed, true);
+    if (rc) return fail(rc, b->err);
+    *out = b;
+    return TDVMC_OK;
+}
+
+int tdvmc_batch_destroy(tdvmc_batch* b)
+{
+    if (!b) return TDVMC_OK;
+    batch_release(b);
+    delete b;
+    return TDVMC_OK;
+}
+
+const char* tdvmc_batch_last_error(const tdvmc_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int tdvmc_batch_size(tdvmc_batch* b, int* n_systems)
+{
+    if (!b || !n_systems) return TDVMC_ERR_BAD_ARG;
+    *n_systems = (int)b->sys.size();
+    return TDVMC_OK;
+}
+
+tdvmc_ctx* tdvmc_batch_system(tdvmc_batch* b, int s)
+{
+    if (!b || s < 0 || s >= (int)b->sys.size()) return nullptr;
+    return b->sys[s];
+}
+
+// the member contexts' own streams must be idle before batch work touches their buffers
+static int batch_sync_members(tdvmc_batch* b)
+{
+    for (auto* c : b->sys) BHIPC(hipStreamSynchronize(c->stream));
+    return TDVMC_OK;
+}
+
+static int batch_precision(tdvmc_batch* b, int precision, int* prec)
+{
+    int p = precision;
+    if (p < 0) p = b->sys[0]->precision;
+    if (p != TDVMC_FP64 && p != TDVMC_MIXED) {
+        b->err = "precision must be TDVMC_FP64 or TDVMC_MIXED";
+        return TDVMC_ERR_BAD_ARG;
+    }
+    *prec = p;
+    return TDVMC_OK;
+}
+
+int tdvmc_batch_set_sweep_lanes(tdvmc_batch* b, int lanes)
+{
+    if (!b) return TDVMC_ERR_BAD_ARG;
+    if (!(lanes == 0 || lanes == 16 || lanes == 32 || lanes == 64) ||
+        (lanes && (b->kh_max + lanes - 1) / lanes > 8)) {
+        b->err = "lanes per walker must be 0, 16, 32 or 64 (at most 8 k-space terms per lane)";
+        return TDVMC_ERR_BAD_ARG;
+    }
+    b->lanes = lanes;
+    batch_drop_graph(b);
+    return TDVMC_OK;
+}
+
+int tdvmc_batch_sweep(tdvmc_batch* b, int64_t n_moves, uint64_t seed, int precision)
+{
+    if (!b) return TDVMC_ERR_BAD_ARG;
+    int prec, rc = batch_precision(b, precision, &prec);
+    if (rc) return rc;
+    if ((rc = batch_sync_members(b))) return rc;
+    if ((rc = batch_upload(b, n_moves, seed, true))) return rc;
+    if ((rc = batch_prep(b, prec))) return rc;
+    if ((rc = batch_sweep_launch(b, prec))) return rc;
+    BHIPC(hipStreamSynchronize(b->stream));
+    for (auto* c : b->sys) {
+        const int64_t m = n_moves > 0 ? n_moves : c->therm_moves;
+        c->trials_total += m * c->W;
+    }
+    return TDVMC_OK;
+}
+
+int tdvmc_batch_measure(tdvmc_batch* b, int precision, tdvmc_step_data* out)
+{
+    if (!b) return TDVMC_ERR_BAD_ARG;
+    int prec, rc = batch_precision(b, precision, &prec);
+    if (rc) return rc;
+    if ((rc = batch_sync_members(b))) return rc;
+    if ((rc = batch_upload(b, 0, b->seed, false))) return rc;
+    if ((rc = batch_prep(b, prec))) return rc;
+    if ((rc = batch_measure_launch(b, prec))) return rc;
+    BHIPC(hipStreamSynchronize(b->stream));
+    if (out)
+        for (size_t s = 0; s < b->sys.size(); s++) {
+            rc = fill_step_data(b->sys[s], b->sys[s]->W, &out[s]);
+            if (rc) {
+                b->err = b->sys[s]->err;
+                return rc;
+            }
+        }
+    return TDVMC_OK;
+}
+
+int tdvmc_batch_tdvp_step(tdvmc_batch* b, double dt, int ode_type, int precision, int use_graph)
+{
+    if (!b) return TDVMC_ERR_BAD_ARG;
+    int prec, rc = batch_precision(b, precision, &prec);
+    if (rc) return rc;
+    const int ode = ode_type < 0 ? b->sys[0]->ode_type : ode_type;
+    if (ode != 0 && ode != 2) {
+        b->err = "ODE type must be 0 (Euler) or 2 (RK4)";
+        return TDVMC_ERR_BAD_ARG;
+    }
+    // parameters set through the member contexts live in their packed vectors,
+    // which the step gathers first
+    if ((rc = batch_sync_members(b))) return rc;
+    if (use_graph) {
+        char key[160];
+        std::snprintf(key, sizeof(key), "%d/%d/%.17g/%d", prec, ode, dt, b->lanes);
+        if (!b->graph || b->graph_key != key) {
+            batch_drop_graph(b);
+            if ((rc = batch_upload(b, 0, b->seed, true))) return rc;
+            BHIPC(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            rc = batch_step_enqueue(b, prec, ode, dt);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (rc) {
+                if (g) hipGraphDestroy(g);
+                return rc;
+            }
+            BHIPC(e);
+            b->graph_src = g;
+            BHIPC(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+            b->graph_key = key;
+        }
+        // a direct sweep / measurement may have rewritten the descriptors since the capture
+        if ((rc = batch_upload(b, 0, b->seed, true))) return rc;
+        BHIPC(hipGraphLaunch(b->graph, b->stream));
+    } else {
+        if ((rc = batch_upload(b, 0, b->seed, true))) return rc;
+        if ((rc = batch_step_enqueue(b, prec, ode, dt))) return rc;
+    }
+    BHIPC(hipStreamSynchronize(b->stream));
+    for (auto* c : b->sys) {
+        c->tf.dirty = c->td.dirty = true;
+        c->host_params_stale = true;
+        c->time += dt;
+        c->step += 1;
+        c->trials_total += (ode == 2 ? 4 : 1) * c->therm_moves * c->W;
+        resolve_potential(c);
+    }
+    return TDVMC_OK;
 }
 
 }  // extern "C"

@@ -36,6 +36,49 @@ struct SweepArgs {
 };
 template <typename T>
 hipError_t launch_sweep(const KSys& sy, const KTables<T>& tb, const SweepArgs& a, hipStream_t st);
+
+// A batch of independent systems in one launch (device arrays, n systems;
+// block_start [n] is the first workgroup of each system, ascending).  All
+// systems share DIM, the box kind and the lanes per walker.
+template <typename T>
+struct SweepBatch {
+    int n;
+    const int64_t* block_start;
+    const KSys* sys;
+    const KTables<T>* tab;
+    const SweepArgs* args;
+};
+template <typename T>
+hipError_t launch_sweep_batch(const SweepBatch<T>& bt, int D, bool periodic, int lanes, int kh_max, int64_t blocks,
+                              size_t lds, hipStream_t st);
+int sweep_walkers_per_block(int lanes);
+
+// Measurement of a batch of systems in one launch (one workgroup per walker);
+// O is written in FP64, no per-particle outputs.
+template <typename T>
+struct EvalBatchItem {
+    const double* R;           // [W][N][D]
+    int64_t W;
+    double* expo;
+    double* ER;
+    double* EI;
+    double* O;                 // [W][P]
+    double* other;             // [W][9] or null
+    double* outer;             // [W] or null
+    double phiR;
+    const T* rho;              // rho_k handed over by the sweep, or null
+};
+template <typename T>
+struct EvalBatch {
+    int n;
+    const int64_t* block_start;
+    const KSys* sys;
+    const KTables<T>* tab;
+    const EvalBatchItem<T>* item;
+};
+template <typename T>
+hipError_t launch_eval_batch(const EvalBatch<T>& bt, int D, bool periodic, int64_t blocks, size_t lds, hipStream_t st);
+int sweep_default_lanes(int N, int Kh);
 template <typename T>
 size_t sweep_lds_bytes(const KSys& sy, int lanes_per_walker);
 
@@ -70,6 +113,39 @@ struct SolveArgs {
     int mean_sub;              // 1: subtract the mean of each solution vector (solver type 1)
 };
 hipError_t launch_solve(const SolveArgs& a, hipStream_t st);
+// one workgroup per system; args_dev [n] in device memory
+hipError_t launch_solve_batch(const SolveArgs* args_dev, int n, int P_max, hipStream_t st);
+
+// ---- batches of systems (kernels_misc.hip)
+template <typename T>
+struct BatchParams {
+    KSys sy;
+    double* uR;                // [P] this system's parameters (device)
+    double* uI;
+    const int* kmap;
+    T* spl;
+    T* dspl;
+    T* alpha;
+    double* xsys;              // [2P+2] this system's packed parameters
+    int64_t xoff;              // offset of this system in the concatenated vector
+};
+template <typename T>
+hipError_t launch_prep_tables_batch(const BatchParams<T>* bp_dev, int n, hipStream_t st);
+template <typename T>
+hipError_t launch_unpack_batch(const BatchParams<T>* bp_dev, int n, const double* x, hipStream_t st);
+template <typename T>
+hipError_t launch_gather_batch(const BatchParams<T>* bp_dev, int n, double* x, int to_batch, hipStream_t st);
+struct MomentsItem {
+    int P;
+    int64_t W;
+    const double* O;           // [W][P]
+    const double* ER;
+    const double* EI;
+    double* mean;              // [P+2]
+    double* S;                 // [P][P]
+    double* F;                 // [2][P]
+};
+hipError_t launch_moments_batch(const MomentsItem* items_dev, int n, int P_max, hipStream_t st);
 
 // after a step: alpha -= alpha[P-1] (adjust), phi R = -exponent (normalize)
 hipError_t launch_post_step(int P, double* x, const double* expo, int normalize, int adjust, hipStream_t st);

@@ -87,11 +87,11 @@ __device__ __forceinline__ double block_sum(double v, double* red)
     return s;
 }
 
+// One workgroup evaluates walker w of the system described by (sy, tb).
 template <typename T, int D, bool PP, bool PER, typename TO>
-__global__ __launch_bounds__(kThreads, 4) void eval_kernel(KSys sy, KTables<T> tb, const double* __restrict__ Rg,
-                                                         int64_t W, EvalOut<TO> out, const T* __restrict__ rho_in)
+__device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, const double* __restrict__ Rg,
+                                          int64_t W, const EvalOut<TO>& out, const T* __restrict__ rho_in, int64_t w)
 {
-    const int64_t w = blockIdx.x;
     if (w >= W) return;
     const int N = sy.N, Ps = sy.P_s, K = sy.Kh;   // K: +-k representatives
     const int tid = threadIdx.x, lane = tid & 63;
@@ -509,6 +509,31 @@ __global__ __launch_bounds__(kThreads, 4) void eval_kernel(KSys sy, KTables<T> t
     }
 }
 
+template <typename T, int D, bool PP, bool PER, typename TO>
+__global__ __launch_bounds__(kThreads, 4) void eval_kernel(KSys sy, KTables<T> tb, const double* __restrict__ Rg,
+                                                         int64_t W, EvalOut<TO> out, const T* __restrict__ rho_in)
+{
+    eval_body<T, D, PP, PER, TO>(sy, tb, Rg, W, out, rho_in, blockIdx.x);
+}
+
+// Batch of independent systems in one launch, one workgroup per walker.
+template <typename T, int D, bool PER>
+__global__ __launch_bounds__(kThreads, 4) void eval_batch_kernel(EvalBatch<T> bt)
+{
+    int lo = 0, hi = bt.n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (bt.block_start[mid] <= (int64_t)blockIdx.x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const EvalBatchItem<T>& it = bt.item[lo];
+    EvalOut<double> o{it.expo, it.ER, it.EI, it.O, it.other, nullptr, nullptr, it.outer, it.phiR};
+    eval_body<T, D, false, PER, double>(bt.sys[lo], bt.tab[lo], it.R, it.W, o, it.rho,
+                                        (int64_t)blockIdx.x - bt.block_start[lo]);
+}
+
 }  // namespace
 
 template <typename T>
@@ -562,6 +587,35 @@ hipError_t launch_eval(const KSys& sy, const KTables<T>& tb, const double* R, in
     default: return hipErrorInvalidValue;
     }
 }
+
+template <typename T, int D, bool PER>
+static hipError_t launch_eval_batch_k(const EvalBatch<T>& bt, int64_t blocks, size_t lds, hipStream_t st)
+{
+    auto kern = eval_batch_kernel<T, D, PER>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kThreads), lds, st, bt);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_eval_batch(const EvalBatch<T>& bt, int D, bool periodic, int64_t blocks, size_t lds, hipStream_t st)
+{
+    if (blocks <= 0) return hipSuccess;
+    switch (D) {
+    case 1: return periodic ? launch_eval_batch_k<T, 1, true>(bt, blocks, lds, st)
+                            : launch_eval_batch_k<T, 1, false>(bt, blocks, lds, st);
+    case 2: return periodic ? launch_eval_batch_k<T, 2, true>(bt, blocks, lds, st)
+                            : launch_eval_batch_k<T, 2, false>(bt, blocks, lds, st);
+    case 3: return periodic ? launch_eval_batch_k<T, 3, true>(bt, blocks, lds, st)
+                            : launch_eval_batch_k<T, 3, false>(bt, blocks, lds, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+template hipError_t launch_eval_batch<float>(const EvalBatch<float>&, int, bool, int64_t, size_t, hipStream_t);
+template hipError_t launch_eval_batch<double>(const EvalBatch<double>&, int, bool, int64_t, size_t, hipStream_t);
 
 template size_t eval_lds_bytes<float>(const KSys&, bool);
 template size_t eval_lds_bytes<double>(const KSys&, bool);

@@ -21,12 +21,12 @@ __device__ double spline_coeff(const double* u, int Ps, int s)
 }
 
 template <typename T>
-__global__ void prep_tables_kernel(KSys sy, const double* __restrict__ uR, const double* __restrict__ uI,
-                                   const int* __restrict__ kmap, T* __restrict__ spl, T* __restrict__ dspl,
-                                   T* __restrict__ alpha)
+__device__ void prep_tables_body(const KSys& sy, const double* __restrict__ uR, const double* __restrict__ uI,
+                                 const int* __restrict__ kmap, T* __restrict__ spl, T* __restrict__ dspl,
+                                 T* __restrict__ alpha, int j0, int jstep)
 {
     const int Ps = sy.P_s;
-    for (int j = threadIdx.x + blockIdx.x * blockDim.x; j < Ps - 1 + sy.Kh; j += blockDim.x * gridDim.x) {
+    for (int j = j0; j < Ps - 1 + sy.Kh; j += jstep) {
         if (j < Ps - 1) {
             for (int part = 0; part < 2; part++) {
                 const double* u = (part == 0 ? uR : uI) + sy.off_pair;
@@ -60,6 +60,119 @@ __global__ void prep_tables_kernel(KSys sy, const double* __restrict__ uR, const
             alpha[2 * k + 1] = (T)aI;
         }
     }
+}
+
+template <typename T>
+__global__ void prep_tables_kernel(KSys sy, const double* __restrict__ uR, const double* __restrict__ uI,
+                                   const int* __restrict__ kmap, T* __restrict__ spl, T* __restrict__ dspl,
+                                   T* __restrict__ alpha)
+{
+    prep_tables_body(sy, uR, uI, kmap, spl, dspl, alpha, threadIdx.x + blockIdx.x * blockDim.x,
+                     blockDim.x * gridDim.x);
+}
+
+// ---- batches of systems: one workgroup per system
+template <typename T>
+__global__ void prep_tables_batch_kernel(const BatchParams<T>* __restrict__ bp)
+{
+    const BatchParams<T> b = bp[blockIdx.x];
+    prep_tables_body(b.sy, b.uR, b.uI, b.kmap, b.spl, b.dspl, b.alpha, threadIdx.x, blockDim.x);
+}
+
+// x (concatenated packed parameters) -> per-system uR, uI
+template <typename T>
+__global__ void unpack_batch_kernel(const BatchParams<T>* __restrict__ bp, const double* __restrict__ x)
+{
+    const BatchParams<T> b = bp[blockIdx.x];
+    const double* xs = x + b.xoff;
+    for (int i = threadIdx.x; i < b.sy.P; i += blockDim.x) {
+        b.uR[i] = xs[i];
+        b.uI[i] = xs[b.sy.P + i];
+    }
+}
+
+// per-system packed parameters <-> the concatenated vector
+template <typename T>
+__global__ void gather_batch_kernel(const BatchParams<T>* __restrict__ bp, double* __restrict__ x, int to_batch)
+{
+    const BatchParams<T> b = bp[blockIdx.x];
+    const int n = 2 * b.sy.P + 2;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        if (to_batch)
+            x[b.xoff + i] = b.xsys[i];
+        else
+            b.xsys[i] = x[b.xoff + i];
+    }
+}
+
+// FP64 moments of one system per workgroup (two passes: means, then the
+// centred products), walkers staged through LDS in chunks of kMomChunk.
+constexpr int kMomChunk = 32;
+__global__ __launch_bounds__(256) void moments_batch_kernel(const MomentsItem* __restrict__ items)
+{
+    const MomentsItem it = items[blockIdx.x];
+    const int P = it.P, C = P + 2, tid = threadIdx.x;
+    const int64_t W = it.W;
+    extern __shared__ __align__(16) unsigned char smem[];
+    double* mean = reinterpret_cast<double*>(smem);      // [C]
+    double* X = mean + C;                                  // [kMomChunk][C] centred values
+    auto col = [&](int64_t w, int c) -> double {
+        return c < P ? it.O[w * P + c] : (c == P ? it.ER[w] : it.EI[w]);
+    };
+    for (int c = tid; c < C; c += blockDim.x) {
+        double sum = 0.0;
+        for (int64_t w = 0; w < W; w++) sum += col(w, c);
+        mean[c] = sum / (double)W;
+    }
+    __syncthreads();
+    // entries: S upper triangle (i <= j < P), then F (O_i with E_R, E_I)
+    const int nS = P * (P + 1) / 2, nE = nS + 2 * P;
+    for (int e0 = 0; e0 < nE; e0 += 4 * (int)blockDim.x) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        int ci[4], cj[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = e0 + q * blockDim.x + tid;
+            ci[q] = cj[q] = -1;
+            if (e < nS) {
+                int i = 0, r = e;
+                while (r >= P - i) {
+                    r -= P - i;
+                    i++;
+                }
+                ci[q] = i;
+                cj[q] = i + r;
+            } else if (e < nE) {
+                ci[q] = (e - nS) % P;
+                cj[q] = P + (e - nS) / P;
+            }
+        }
+        for (int64_t w0 = 0; w0 < W; w0 += kMomChunk) {
+            const int nw = (int)((W - w0) < kMomChunk ? (W - w0) : kMomChunk);
+            __syncthreads();
+            for (int e = tid; e < nw * C; e += blockDim.x) {
+                const int ww = e / C, c = e % C;
+                X[ww * C + c] = col(w0 + ww, c) - mean[c];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (ci[q] >= 0)
+                    for (int ww = 0; ww < nw; ww++) acc[q] += X[ww * C + ci[q]] * X[ww * C + cj[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (ci[q] < 0) continue;
+            const double v = acc[q] / (double)W;
+            if (cj[q] < P) {
+                it.S[(size_t)ci[q] * P + cj[q]] = v;
+                it.S[(size_t)cj[q] * P + ci[q]] = v;
+            } else {
+                it.F[(size_t)(cj[q] - P) * P + ci[q]] = v;
+            }
+        }
+    }
+    for (int c = tid; c < C; c += blockDim.x) it.mean[c] = mean[c];
 }
 
 __global__ void axpy_params_kernel(int n, const double* x0, const double* xdot, double c, double* y)
@@ -190,6 +303,45 @@ hipError_t launch_rk4_combine(int P, double* x, const double* k1, const double* 
 hipError_t launch_unpack_params(int P, const double* packed, double* uR, double* uI, hipStream_t st)
 {
     hipLaunchKernelGGL(unpack_params_kernel, dim3((P + 255) / 256), dim3(256), 0, st, P, packed, uR, uI);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_prep_tables_batch(const BatchParams<T>* bp_dev, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(prep_tables_batch_kernel<T>, dim3(n), dim3(256), 0, st, bp_dev);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_unpack_batch(const BatchParams<T>* bp_dev, int n, const double* x, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(unpack_batch_kernel<T>, dim3(n), dim3(256), 0, st, bp_dev, x);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_gather_batch(const BatchParams<T>* bp_dev, int n, double* x, int to_batch, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_batch_kernel<T>, dim3(n), dim3(256), 0, st, bp_dev, x, to_batch);
+    return hipGetLastError();
+}
+template hipError_t launch_prep_tables_batch<float>(const BatchParams<float>*, int, hipStream_t);
+template hipError_t launch_prep_tables_batch<double>(const BatchParams<double>*, int, hipStream_t);
+template hipError_t launch_unpack_batch<double>(const BatchParams<double>*, int, const double*, hipStream_t);
+template hipError_t launch_gather_batch<double>(const BatchParams<double>*, int, double*, int, hipStream_t);
+
+hipError_t launch_moments_batch(const MomentsItem* items_dev, int n, int P_max, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const size_t lds = (size_t)(P_max + 2) * (1 + kMomChunk) * sizeof(double);
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)moments_batch_kernel,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(moments_batch_kernel, dim3(n), dim3(256), lds, st, items_dev);
     return hipGetLastError();
 }
 

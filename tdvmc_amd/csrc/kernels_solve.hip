@@ -18,6 +18,8 @@
 // of the right-hand side is updated by the whole workgroup.  The factor lives
 // column-major in the work buffer (L2-resident for P <= 600).
 
+#include <algorithm>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -40,7 +42,7 @@ __device__ double block_reduce(double v, double* red)
 }
 
 template <int NB>
-__global__ __launch_bounds__(kThreads) void solve_kernel(SolveArgs a)
+__device__ __forceinline__ void solve_body(const SolveArgs& a)
 {
     const int P = a.P, tid = threadIdx.x;
     double* Lm = a.work;                 // column-major P x P: Lm[j * P + i] = L[i][j]
@@ -290,7 +292,7 @@ __device__ void argmax_reduce(double& v, int& e, double* rv, int* re)
         }
 }
 
-__global__ __launch_bounds__(kThreads) void solve_qr_kernel(SolveArgs a)
+__device__ __forceinline__ void solve_qr_body(const SolveArgs& a)
 {
     const int P = a.P, tid = threadIdx.x;
     double* Q = a.work;                  // column-major: Q[j * P + i] = A[i][j]
@@ -503,6 +505,26 @@ __global__ __launch_bounds__(kThreads) void solve_qr_kernel(SolveArgs a)
 }
 
 template <int NB>
+__global__ __launch_bounds__(kThreads) void solve_kernel(SolveArgs a)
+{
+    solve_body<NB>(a);
+}
+__global__ __launch_bounds__(kThreads) void solve_qr_kernel(SolveArgs a)
+{
+    solve_qr_body(a);
+}
+// one workgroup per system of a batch
+template <int NB>
+__global__ __launch_bounds__(kThreads) void solve_batch_kernel(const SolveArgs* __restrict__ args)
+{
+    const SolveArgs a = args[blockIdx.x];
+    if (a.mean_sub)
+        solve_qr_body(a);
+    else
+        solve_body<NB>(a);
+}
+
+template <int NB>
 hipError_t launch_nb(const SolveArgs& a, hipStream_t st)
 {
     const size_t lds = (size_t)NB * a.P * sizeof(double);
@@ -516,6 +538,21 @@ hipError_t launch_nb(const SolveArgs& a, hipStream_t st)
 }
 
 }  // namespace
+
+hipError_t launch_solve_batch(const SolveArgs* args_dev, int n, int P_max, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    // LDS: Cholesky panels (NB x P) or the QR transpositions (2 P ints)
+    const size_t lds = std::max((size_t)32 * P_max * sizeof(double), 2 * (size_t)P_max * sizeof(int));
+    if ((size_t)32 * P_max * sizeof(double) > 150 * 1024) return hipErrorInvalidValue;
+    auto kern = solve_batch_kernel<32>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(n), dim3(kThreads), lds, st, args_dev);
+    return hipGetLastError();
+}
 
 hipError_t launch_solve(const SolveArgs& a, hipStream_t st)
 {

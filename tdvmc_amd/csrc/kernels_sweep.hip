@@ -129,8 +129,10 @@ __host__ __device__ inline int sweep_table_len(const KSys& sy)
     return j > sy.P_s - 1 ? j : sy.P_s - 1;
 }
 
+// One workgroup's share of the sweep: walkers blk * WPB .. (blk + 1) * WPB - 1
+// of the system described by (sy, tb, a).
 template <typename T, int D, int LPW, int KPL, bool PER>
-__global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb, SweepArgs a)
+__device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb, const SweepArgs& a, int64_t blk)
 {
     using C = Coord<T, PER>;
     using S = typename C::S;
@@ -139,7 +141,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
     const int NT = sweep_table_len(sy);
     const int tid = threadIdx.x;
     const int grp = tid / LPW, gl = tid % LPW;
-    const int64_t w = (int64_t)blockIdx.x * WPB + grp;
+    const int64_t w = blk * WPB + grp;
 
     extern __shared__ __align__(16) unsigned char smem[];
     Vec4<T>* spl = reinterpret_cast<Vec4<T>*>(smem);                      // NT x (A0..A3) of Re u
@@ -360,6 +362,28 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb,
     if (gl == 0 && a.acc) a.acc[w] += nacc;
 }
 
+template <typename T, int D, int LPW, int KPL, bool PER>
+__global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb, SweepArgs a)
+{
+    sweep_body<T, D, LPW, KPL, PER>(sy, tb, a, blockIdx.x);
+}
+
+// Batch of independent systems in one launch: workgroup b belongs to system
+// s with block_start[s] <= b < block_start[s + 1].
+template <typename T, int D, int LPW, int KPL, bool PER>
+__global__ __launch_bounds__(kThreads) void sweep_batch_kernel(SweepBatch<T> bt)
+{
+    int lo = 0, hi = bt.n;   // last s with block_start[s] <= blockIdx.x
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (bt.block_start[mid] <= (int64_t)blockIdx.x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    sweep_body<T, D, LPW, KPL, PER>(bt.sys[lo], bt.tab[lo], bt.args[lo], (int64_t)blockIdx.x - bt.block_start[lo]);
+}
+
 // Default lanes per walker without tuning: 32 from N = 24 on (two walkers
 // per wavefront share the per-trial overhead and waste fewer partner slots
 // than one walker per wavefront), 16 below; more when the k-space terms would
@@ -427,6 +451,61 @@ static hipError_t launch_sweep_d(const KSys& sy, const KTables<T>& tb, const Swe
     }
 }
 
+template <typename T, int D, int LPW, int KPL, bool PER>
+static hipError_t launch_sweep_batch_k(const SweepBatch<T>& bt, int64_t blocks, size_t lds, hipStream_t st)
+{
+    auto kern = sweep_batch_kernel<T, D, LPW, KPL, PER>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kThreads), lds, st, bt);
+    return hipGetLastError();
+}
+
+template <typename T, int D, int LPW, bool PER>
+static hipError_t launch_sweep_batch_l(const SweepBatch<T>& bt, int kh_max, int64_t blocks, size_t lds,
+                                       hipStream_t st)
+{
+    const int kpl = (kh_max + LPW - 1) / LPW;
+    if (kpl == 0) return launch_sweep_batch_k<T, D, LPW, 0, PER>(bt, blocks, lds, st);
+    if constexpr (!PER) {
+        return hipErrorInvalidValue;
+    } else {
+        if (kpl <= 2) return launch_sweep_batch_k<T, D, LPW, 2, PER>(bt, blocks, lds, st);
+        if (kpl <= 4) return launch_sweep_batch_k<T, D, LPW, 4, PER>(bt, blocks, lds, st);
+        if (kpl <= 8) return launch_sweep_batch_k<T, D, LPW, 8, PER>(bt, blocks, lds, st);
+        return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t launch_sweep_batch(const SweepBatch<T>& bt, int D, bool periodic, int lanes, int kh_max, int64_t blocks,
+                              size_t lds, hipStream_t st)
+{
+    if (blocks <= 0) return hipSuccess;
+#define TDVMC_SWEEP_BATCH_D(DD)                                                                              \
+    switch (lanes) {                                                                                         \
+    case 16: return periodic ? launch_sweep_batch_l<T, DD, 16, true>(bt, kh_max, blocks, lds, st)             \
+                             : launch_sweep_batch_l<T, DD, 16, false>(bt, kh_max, blocks, lds, st);           \
+    case 32: return periodic ? launch_sweep_batch_l<T, DD, 32, true>(bt, kh_max, blocks, lds, st)             \
+                             : launch_sweep_batch_l<T, DD, 32, false>(bt, kh_max, blocks, lds, st);           \
+    case 64: return periodic ? launch_sweep_batch_l<T, DD, 64, true>(bt, kh_max, blocks, lds, st)             \
+                             : launch_sweep_batch_l<T, DD, 64, false>(bt, kh_max, blocks, lds, st);           \
+    default: return hipErrorInvalidValue;                                                                    \
+    }
+    switch (D) {
+    case 1: TDVMC_SWEEP_BATCH_D(1)
+    case 2: TDVMC_SWEEP_BATCH_D(2)
+    case 3: TDVMC_SWEEP_BATCH_D(3)
+    default: return hipErrorInvalidValue;
+    }
+#undef TDVMC_SWEEP_BATCH_D
+}
+
+int sweep_walkers_per_block(int lanes) { return kThreads / lanes; }
+int sweep_default_lanes(int N, int Kh) { return pick_lpw(N, Kh); }
+
 template <typename T>
 hipError_t launch_sweep(const KSys& sy, const KTables<T>& tb, const SweepArgs& a, hipStream_t st)
 {
@@ -439,6 +518,10 @@ hipError_t launch_sweep(const KSys& sy, const KTables<T>& tb, const SweepArgs& a
     }
 }
 
+template hipError_t launch_sweep_batch<float>(const SweepBatch<float>&, int, bool, int, int, int64_t, size_t,
+                                              hipStream_t);
+template hipError_t launch_sweep_batch<double>(const SweepBatch<double>&, int, bool, int, int, int64_t, size_t,
+                                               hipStream_t);
 template hipError_t launch_sweep<float>(const KSys&, const KTables<float>&, const SweepArgs&, hipStream_t);
 template hipError_t launch_sweep<double>(const KSys&, const KTables<double>&, const SweepArgs&, hipStream_t);
 template size_t sweep_lds_bytes<float>(const KSys&, int);

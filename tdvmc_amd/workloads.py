@@ -18,6 +18,8 @@
     trap2d_drop    2-D harmonic trap (omega = 1), Gaussian pair repulsion
                    V0 = 2, width 0.5, one-body width w ~ U(0.8, 1.2) and 24 spline
                    parameters; N from {3, 6, 10, 20, 64, 100}
+    trap2d_batch64 64 such drops with N and w drawn from those ranges, 4096
+                   walkers each, as one batch
 """
 from __future__ import annotations
 
@@ -103,6 +105,22 @@ def trap2d_drop(N, width, W=4096, seed=5):
     return _base(SYSTEM_TYPE="BosonsTrap", N=N, DIM=2, LBOX=0.0, N_PARAM=P_s, ONE_BODY=1, TRAP_OMEGA=1.0,
                  SPLINE_CUTOFF=6.0, HBAR2_2M=0.5, PAIR_POTENTIAL="GAUSSIAN", SYSTEM_PARAMS=[0.5, 2.0],
                  PARAMS_REAL=params.tolist(), MC_STEP=0.5, MC_NTHERMSTEPS=N, N_WALKERS=W, PRECISION="MIXED")
+
+
+def trap2d_batch64(W=4096, n_systems=64, seed_n=5, seed_w=6):
+    """64 independent 2-D trapped drops (one batch): N drawn uniformly from
+    {3, 6, 10, 20, 64, 100} (seed 5), one-body width w ~ U(0.8, 1.2) (seed 6),
+    24 pair-spline parameters, W walkers each."""
+    rn = np.random.default_rng(seed_n)
+    rw = np.random.default_rng(seed_w)
+    Ns = rn.choice([3, 6, 10, 20, 64, 100], size=n_systems)
+    widths = rw.uniform(0.8, 1.2, size=n_systems)
+    cfgs = []
+    for i, (N, w) in enumerate(zip(Ns, widths)):
+        cfg = trap2d_drop(int(N), float(w), W=W, seed=100 + i)
+        cfg["SEED"] = 1000 + i
+        cfgs.append(cfg)
+    return cfgs
 
 
 WORKLOADS = {

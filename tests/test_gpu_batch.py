@@ -1,0 +1,125 @@
+"""Batches of independent systems of uneven particle count (2-D harmonic
+traps with a one-body factor): one sweep launch and one measurement launch
+for all systems must reproduce the single-system path exactly (same random
+stream per walker), and a batched TDVP step must match the oracle."""
+import numpy as np
+import pytest
+
+from oracle import oracle as o
+from tests.helpers import make_cfg, oracle_system, rel
+
+pytestmark = pytest.mark.gpu
+tdvmc = pytest.importorskip("tdvmc_amd")
+
+NS = (3, 10, 20, 64)
+
+
+def _cfgs(W=300, therm=0, ns=NS, **extra):
+    out = []
+    for i, N in enumerate(ns):
+        out.append(make_cfg(N=N, D=2, L=0.0, P_s=8, pot="GAUSSIAN", sysparams=(0.5, 2.0), one_body=1,
+                            trap_omega=1.0, rc=6.0, hbar2_2m=0.5, W=W, step=0.4, therm=therm, SEED=11 + i,
+                            **extra))
+    return out
+
+
+def _params(cfg, i):
+    s = oracle_system(cfg)
+    rng = np.random.default_rng(100 + i)
+    uR = rng.normal(0, 0.05, s.P)
+    uR[0] = 0.5 / rng.uniform(0.8, 1.2) ** 2      # one-body width parameter 1/(2 w^2)
+    uI = rng.normal(0, 0.01, s.P)
+    return s, uR, uI
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_sweep_matches_single_contexts(prec):
+    cfgs = _cfgs()
+    seed = 77
+    print("batch sweep", prec, flush=True)
+    with tdvmc.Batch(cfgs) as b:
+        starts = []
+        for i, c in enumerate(b.systems):
+            s, uR, uI = _params(cfgs[i], i)
+            c.set_params(uR, uI)
+            starts.append((c.get_walkers(), uR, uI))
+        b.set_sweep_lanes(32)
+        b.sweep(40, seed=seed, precision=prec)
+        got = [c.get_walkers() for c in b.systems]
+        acc = [c.acceptance() for c in b.systems]
+    for i, cfg in enumerate(cfgs):
+        R0, uR, uI = starts[i]
+        with tdvmc.Context(cfg) as ctx:
+            ctx.set_params(uR, uI)
+            ctx.set_walkers(R0)
+            ctx.set_sweep_lanes(32)
+            ctx.sweep(40, seed=tdvmc.batch_system_seed(seed, i), precision=prec)
+            assert np.array_equal(ctx.get_walkers(), got[i]), f"system {i} (N={NS[i]})"
+            assert ctx.acceptance() == acc[i]
+
+
+def test_batch_measure_matches_single_and_oracle():
+    cfgs = _cfgs()
+    print("batch measure", flush=True)
+    with tdvmc.Batch(cfgs) as b:
+        walkers = []
+        for i, c in enumerate(b.systems):
+            s, uR, uI = _params(cfgs[i], i)
+            c.set_params(uR, uI)
+            walkers.append((s, c.get_walkers(), uR, uI))
+        res = b.measure(precision=tdvmc.FP64)
+    for i, (s, R, uR, uI) in enumerate(walkers):
+        out = o.eval_configs(s, R, uR, uI)
+        m = o.moments(out["O"], out["ER"], out["EI"])
+        r = res[i]
+        assert rel(r["mO"], m["mO"]) < 1e-10
+        assert rel(r["mE"], m["mE"]) < 1e-10
+        assert rel(r["S"], m["S"]) < 1e-10
+        assert rel(r["FR"], m["FR"]) < 1e-9
+        assert rel(r["FI"], m["FI"]) < 1e-9
+
+
+@pytest.mark.parametrize("solver", [0, 1])
+def test_batch_euler_step_vs_oracle(solver):
+    cfgs = _cfgs(therm=0, IMAGINARY_TIME=1, LINEAR_EQUATION_SOLVER_TYPE=solver, REG_EPS=1e-3)
+    dt = 0.01
+    print("batch euler", solver, flush=True)
+    with tdvmc.Batch(cfgs) as b:
+        ref = []
+        for i, c in enumerate(b.systems):
+            s, uR, uI = _params(cfgs[i], i)
+            c.set_params(uR, uI)
+            R = c.get_walkers()
+            out = o.eval_configs(s, R, uR, uI)
+            m = o.moments(out["O"], out["ER"], out["EI"])
+            x = o.solve_param_dot(m, imag_time=1, precondition=0, eps=1e-3, solver_type=solver)
+            ref.append((uR + dt * x["xR"], uI + dt * x["xI"], dt * np.max(np.abs(x["xR"])),
+                        dt * np.max(np.abs(x["xI"]))))
+        b.tdvp_step(dt, ode_type=0, precision=tdvmc.FP64)
+        for i, c in enumerate(b.systems):
+            gR, gI, _, _ = c.get_params()
+            xR, xI, sR, sI = ref[i]
+            assert np.max(np.abs(gR - xR)) < 1e-9 * sR + 1e-13, f"system {i}"
+            assert np.max(np.abs(gI - xI)) < 1e-9 * sI + 1e-13, f"system {i}"
+
+
+def test_batch_graph_replays_match_eager():
+    # (these small, weakly constrained systems stay stable in imaginary time
+    # at dt = 1e-3 with 300 walkers; larger N or dt diverge in single
+    # contexts just the same)
+    cfgs = _cfgs(therm=20, ns=(3, 10, 20), IMAGINARY_TIME=1)
+    res = []
+    for graph in (False, True):
+        print("batch graph", graph, flush=True)
+        with tdvmc.Batch(cfgs) as b:
+            for i, c in enumerate(b.systems):
+                s, uR, uI = _params(cfgs[i], i)
+                c.set_params(uR, uI)
+            for _ in range(3):
+                b.tdvp_step(1e-3, ode_type=2, precision=tdvmc.FP64, use_graph=graph)
+                b.measure(precision=tdvmc.FP64)             # eager work between replays
+            res.append([np.concatenate(c.get_params()[:2]) for c in b.systems]
+                       + [c.get_walkers() for c in b.systems])
+    for a, g in zip(res[0], res[1]):
+        assert np.all(np.isfinite(a))
+        assert np.allclose(a, g, rtol=1e-11, atol=1e-13)

@@ -14,7 +14,7 @@ tdvmc = pytest.importorskip("tdvmc_amd")
 
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "tdvmc.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(tdvmc_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*|tdvmc_ctx\*)\s+(tdvmc_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
