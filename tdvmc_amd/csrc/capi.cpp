@@ -817,8 +817,8 @@ int alloc_ctx(tdvmc_ctx* ctx)
     HIPC(hipMemcpyAsync(ctx->dKmap, ctx->kmap.data(), ctx->kmap.size() * sizeof(int), hipMemcpyHostToDevice,
                         ctx->stream));
     HIPC(dalloc(&ctx->tf.spl, (size_t)(Ps - 1) * 8));
-    HIPC(dalloc(&ctx->tf.dspl, (size_t)(Ps - 1) * 8));
-    HIPC(dalloc(&ctx->td.dspl, (size_t)(Ps - 1) * 8));
+    HIPC(dalloc(&ctx->tf.dspl, (size_t)Ps * kDsplStride));
+    HIPC(dalloc(&ctx->td.dspl, (size_t)Ps * kDsplStride));
     HIPC(dalloc(&ctx->tf.kv, 4 * (size_t)K));
     HIPC(dalloc(&ctx->tf.alpha, 2 * (size_t)K));
     HIPC(dalloc(&ctx->td.spl, (size_t)(Ps - 1) * 8));

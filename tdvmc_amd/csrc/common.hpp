@@ -35,8 +35,10 @@ struct KSys {
 template <typename T>
 struct KTables {
     const T* spl;     // [(P_s-1)][8]: cubic-in-t coefficients A0..A3 of Re u, then of Im u
-    const T* dspl;    // [(P_s-1)][8]: u' = d1 + t (d2 + t d3), u'' = hinv d2 + t e3 per unit length,
-                      //   (d1, d2, d3, e3) of Re u, then of Im u
+    const T* dspl;    // [P_s][kDsplStride]: derivatives per unit length on knot interval j,
+                      //   u' = d1 + t (d2 + t d3), u'' = hinv (d2 + 2 t d3), stored as
+                      //   (d1, d2, d3 of Re u, d1 | d2, d3 of Im u, 0, 0); row P_s-1 holds
+                      //   u'(rc) and u''(rc) / hinv as d1, d2 (used at t = 0 only)
     const T* kv;      // [Kh][4]: n/L of the representative (phase in revolutions = kv . r), |k|^2
     const T* alpha;   // [Kh][2]: (Re, Im) of alpha_k (+ alpha_{-k} when paired)
     const int* kmap;  // [Kh][2]: parameter slots (relative to off_k) of k and of -k (-1: unpaired)
@@ -45,6 +47,7 @@ struct KTables {
 };
 
 constexpr int kMaxD = 3;
+constexpr int kDsplStride = 8;    // entries per knot interval of KTables::dspl
 
 template <int D> struct PadDim { static constexpr int value = (D == 3) ? 4 : D; };
 
@@ -122,6 +125,7 @@ template <> struct Prec<float> {
     }
     __device__ static __forceinline__ float fract_(float x) { return __builtin_amdgcn_fractf(x); }
     __device__ static __forceinline__ float exp_(float x) { return __expf(x); }
+    __device__ static __forceinline__ float exp2_(float x) { return __builtin_amdgcn_exp2f(x); }
     __device__ static __forceinline__ float log_(float x) { return __logf(x); }
 };
 
@@ -143,7 +147,55 @@ template <> struct Prec<double> {
     __device__ static __forceinline__ void sincos_small(double t, double* s, double* c) { sincos_rev(t, s, c); }
     __device__ static __forceinline__ double fract_(double x) { return x - __builtin_floor(x); }
     __device__ static __forceinline__ double exp_(double x) { return exp(x); }
+    __device__ static __forceinline__ double exp2_(double x) { return exp2(x); }
     __device__ static __forceinline__ double log_(double x) { return log(x); }
+};
+
+// Storage of one coordinate in LDS.  FP32 in a periodic box: the position as
+// an unsigned 32-bit fraction of the box, so a wrapping integer difference is
+// the minimum image (one subtract and one convert per component, exact);
+// distances come out in units of 2^-32 box lengths.  Otherwise the compute
+// type in box units (periodic, minimum image by rint) or in real units.
+template <typename T, bool PER>
+struct Coord {
+    using S = T;
+    static constexpr double kUnit = 1.0;                 // length unit of diff() in box (or real) units
+    __device__ static S load(double x) { return (T)(PER ? x - rint(x) : x); }
+    __device__ static double store(S p) { return (double)p; }
+    __device__ static T diff(S a, S b)
+    {
+        T d = a - b;
+        if (PER) d -= Prec<T>::rint_(d);
+        return d;
+    }
+    __device__ static S move(S p, T dz)
+    {
+        T v = p + dz;
+        if (PER) v -= Prec<T>::rint_(v);
+        return v;
+    }
+    __device__ static T unit_coord(S p) { return p; }   // in box (or real) units
+    __device__ static T raw(S p) { return p; }           // value in units of kUnit
+};
+template <>
+struct Coord<float, true> {
+    using S = uint32_t;
+    static constexpr double kUnit = 2.3283064365386963e-10;   // 2^-32
+    __device__ static S load(double x)
+    {
+        const double f = x - floor(x);   // [0, 1)
+        return (uint32_t)(uint64_t)(f * 4294967296.0);
+    }
+    __device__ static double store(S p) { return (double)(int32_t)p * kUnit; }   // [-1/2, 1/2)
+    __device__ static float diff(S a, S b) { return (float)(int32_t)(a - b); }
+    __device__ static S move(S p, float dz)
+    {
+        // dz in box units; only its image in [-1/2, 1/2) matters
+        const float w = dz - __builtin_rintf(dz);
+        return p + (uint32_t)__float2int_rn(w * 4294967296.0f);
+    }
+    __device__ static float unit_coord(S p) { return (float)(int32_t)p * (float)kUnit; }
+    __device__ static float raw(S p) { return (float)(int32_t)p; }
 };
 
 // minimum-image component (round to nearest multiple of L)

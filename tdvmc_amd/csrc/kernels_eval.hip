@@ -8,17 +8,19 @@
 // k-space Jastrow sum_k alpha_k |rho_k|^2 (k and -k share |rho_k|^2 and are
 // handled once with the summed amplitude) and a one-body Gaussian factor.
 //
-// Work split (N particles, K k-vectors, 256 threads):
+// Work split (N particles, K k-vectors, 256 threads), in this order:
+//   rho_k    : thread = k, loop over particles (LDS broadcast reads), or the
+//              values the sweep handed over;
 //   pairs    : every unordered pair once, on a half ring: lane = row n of a
 //              64-row block, partner i = (n + s) mod N, the s range split over
 //              the 4 waves; own-row sums stay in registers, partner sums ride
 //              a DPP register ring and reach LDS once per particle and step;
-//   rho_k    : thread = k, loop over particles (LDS broadcast reads);
 //   k terms  : lane = row n, the k range split over the 4 waves;
 //   E_L      : thread = particle, FP64 block reduction.
 // The pair's position t in its knot interval goes to an LDS histogram of the
 // moments 1, t, t^2, t^3 per interval ([interval][4][8], 8 lanes per column);
-// the B-spline basis sums are rebuilt from these moments at the end.
+// the B-spline basis sums are rebuilt from these moments at the end, and the
+// count of pairs beyond the cutoff follows from the histogram's total.
 // Coordinates and per-particle sums are staged in LDS as structure-of-arrays
 // (bank-conflict-free access to consecutive particles).  Every LDS
 // accumulator is FP64 in both precisions: FP64 LDS atomics run several
@@ -64,6 +66,10 @@ __device__ __forceinline__ float wave_rotate(float v)
 {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), kDppWaveRol1, 0xF, 0xF, false));
 }
+__device__ __forceinline__ uint32_t wave_rotate(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, kDppWaveRol1, 0xF, 0xF, false);
+}
 __device__ __forceinline__ double wave_rotate(double v)
 {
     const int2 x = *reinterpret_cast<int2*>(&v);
@@ -87,11 +93,317 @@ __device__ __forceinline__ double block_sum(double v, double* red)
     return s;
 }
 
+// Pair potential of one pair in coordinate-difference units (constants from
+// PairConsts); POT: 1 square well, 2 Gaussian, 3 Lennard-Jones.
+template <typename T>
+struct PairConsts {
+    T a;    // square well: range
+    T b;    // square well: depth; Gaussian: amplitude
+    T g;    // Gaussian: exponent factor of r^2 (base 2 in FP32, base e in FP64)
+    T a2;   // Lennard-Jones: sigma^2
+    T c4;   // Lennard-Jones: 4 epsilon
+};
+
+template <typename T, int POT>
+__device__ __forceinline__ T pair_pot(const PairConsts<T>& pc, T r, T r2, T rinv)
+{
+    if constexpr (POT == 1) {
+        return r < pc.a ? pc.b : (T)0;
+    } else if constexpr (POT == 2) {
+        if constexpr (sizeof(T) == 4)
+            return pc.b * Prec<T>::exp2_(r2 * pc.g);
+        else
+            return pc.b * Prec<T>::exp_(r2 * pc.g);
+    } else if constexpr (POT == 3) {
+        const T q = pc.a2 * (rinv * rinv);   // (sigma / r)^2
+        const T q6 = q * q * q;
+        return pc.c4 * (q6 * q6 - q6);
+    } else {
+        return (T)0;
+    }
+}
+
+// All pairs of one walker (see the work split above).  Coordinates are
+// Coord<T, PER> values; lengths below are in units of one coordinate
+// difference (2^-32 box lengths for FP32 in a periodic box, box lengths for
+// FP64 in a periodic box, real units in open space).  The potential of a
+// periodic box acts within the cutoff only (the same test as the spline); in
+// open space it acts on every pair.  Pairs beyond the cutoff are not counted
+// here: their number is N (N - 1) / 2 minus the histogram count.  CPLX false:
+// every imaginary parameter is zero, so the imaginary parts (exact zeros) are
+// not computed.
+template <typename T, int D, bool PP, bool PER, int POT, bool CPLX>
+__device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<T, PER>::S* __restrict__ pos,
+                                           const T* __restrict__ dsp, double* __restrict__ hist,
+                                           double* __restrict__ gsum, double* __restrict__ lsum, int wave,
+                                           int lane, double& vpot, double& lap_tot_R, double& lap_tot_I)
+{
+    using C = Coord<T, PER>;
+    using S = typename C::S;
+    const int N = sy.N;
+    const double unit = C::kUnit / (PER ? sy.Linv : 1.0);   // length of one coordinate-difference unit
+    const T hsc = (T)(sy.hinv * unit), rc = (T)(sy.rc / unit);
+    const T hinvT = (T)sy.hinv;
+    const T lapsc = (T)((sy.D - 1) / unit);   // (D-1)/r = lapsc * rinv
+    PairConsts<T> pc{};
+    if constexpr (POT == 1) {
+        pc.a = (T)(sy.pot_a / unit);
+        pc.b = (T)sy.pot_b;
+    } else if constexpr (POT == 2) {
+        const double au = sy.pot_a / unit;
+        pc.b = (T)sy.pot_b;
+        pc.g = (T)((sizeof(T) == 4 ? -0.5 * 1.4426950408889634 : -0.5) / (au * au));
+    } else if constexpr (POT == 3) {
+        const double au = sy.pot_a / unit;
+        pc.a2 = (T)(au * au);
+        pc.c4 = (T)(4.0 * sy.pot_b);
+    }
+    double* const hbase = hist + (lane & (kHistCols - 1));
+
+    // Pairs in 64 x 64 tiles (A, B), A <= B, of particle blocks.  Lane l owns
+    // row particle 64 A + l; column particles ride a register ring that
+    // rotates one lane per step (wave_rol), so at step s lane l meets ring
+    // slot (l + s) mod 64 and after 64 steps every row has met every column.
+    // Diagonal tiles take s = 1..32 (s = 32 by lanes < 32 only), so every
+    // unordered pair occurs once.  A last block of at most 32 particles is
+    // held twice in the ring (slot q: particle q mod 32) and met in 32 steps
+    // s = 0..31: 32 consecutive slots hold every residue mod 32 once.  Row
+    // sums stay in registers, column sums rotate with the column coordinates;
+    // both reach LDS once per tile segment.  The steps of all tiles are split
+    // evenly over the 4 waves.  Off-diagonal tiles of full blocks run without
+    // per-step validity tests.
+    const int nfull = N / 64, rem = N - 64 * nfull;
+    const int nblk = nfull + (rem > 0 ? 1 : 0);
+    const bool fold = rem > 0 && rem <= 32 && nfull > 0;
+    const int total = nfull * (nfull - 1) / 2 * 64 + nblk * 32 + (rem > 0 ? nfull * (fold ? 32 : 64) : 0);
+    const int g0 = total * wave / kWaves, g1 = total * (wave + 1) / kWaves;
+    int off = 0;
+    for (int A = 0; A < nblk; A++) {
+        for (int Bk = A; Bk < nblk; Bk++) {
+            const bool diag = A == Bk;
+            const bool folded = !diag && fold && Bk == nblk - 1;
+            const int cnt = (diag || folded) ? 32 : 64;
+            const int lo = max(g0, off), hi = min(g1, off + cnt);
+            off += cnt;
+            if (lo >= hi) continue;
+            const int s_first = lo - (off - cnt) + (diag ? 1 : 0);
+            const int nsteps = hi - lo;
+            const int cmask = folded ? 31 : 63;   // ring slot -> column offset in block Bk
+            const int ra = A * 64 + lane;
+            const bool rval = ra < N;
+            S rn[D], cp[D];
+            int cidx = Bk * 64 + ((lane + s_first) & cmask);
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                rn[a] = rval ? pos[a * N + ra] : (S)0;
+                cp[a] = cidx < N ? pos[a * N + cidx] : (S)0;
+            }
+            T gR[D], gI[D], pR[D], pI[D];
+#pragma unroll
+            for (int a = 0; a < D; a++) gR[a] = gI[a] = pR[a] = pI[a] = (T)0;
+            T lR = (T)0, lI = (T)0, plR = (T)0, plI = (T)0, vs = (T)0;
+
+            auto pair = [&]() {
+                T d[D];
+                T r2 = (T)0;
+#pragma unroll
+                for (int a = 0; a < D; a++) {
+                    d[a] = C::diff(rn[a], cp[a]);
+                    r2 += d[a] * d[a];
+                }
+                // FP32: one v_rsq gives 1/r, and r = r^2 / r; FP64: sqrt and a divide
+                T r, rinv;
+                Prec<T>::r_and_rinv(r2, &r, &rinv);
+                if constexpr (!PER && POT != 0) vs += pair_pot<T, POT>(pc, r, r2, rinv);
+                if (r <= rc) {
+                    if constexpr (PER && POT != 0) vs += pair_pot<T, POT>(pc, r, r2, rinv);
+                    // knot interval j in [0, P_s-1] (P_s-1 only at r = rc, t = 0)
+                    const T x = r * hsc;
+                    const int j = (int)x;
+                    const T t = Prec<T>::fract_(x);
+                    // moments 1, t, t^2, t^3 of the pair's position in knot
+                    // interval j (FP64); the basis sums B_s are rebuilt from
+                    // them at the end
+                    const double td = (double)t, t2 = td * td, t3 = t2 * td;
+                    double* hb = hbase + j * (4 * kHistCols);
+                    lds_add(hb, 1.0);
+                    lds_add(hb + kHistCols, td);
+                    lds_add(hb + 2 * kHistCols, t2);
+                    lds_add(hb + 3 * kHistCols, t3);
+                    // u'(r), u''(r) per unit length on interval j
+                    const T* c = dsp + j * kDsplStride;
+                    const T upR = c[0] + t * (c[1] + t * c[2]);
+                    const T uppR = hinvT * (c[1] + (t + t) * c[2]);
+                    const T fR = upR * rinv;
+#pragma unroll
+                    for (int a = 0; a < D; a++) {
+                        gR[a] += fR * d[a];
+                        pR[a] -= fR * d[a];
+                    }
+                    const T lfR = uppR + lapsc * fR;
+                    lR += lfR;
+                    if (PP) plR += lfR;
+                    if constexpr (CPLX) {
+                        const T upI = c[3] + t * (c[4] + t * c[5]);
+                        const T uppI = hinvT * (c[4] + (t + t) * c[5]);
+                        const T fI = upI * rinv;
+#pragma unroll
+                        for (int a = 0; a < D; a++) {
+                            gI[a] += fI * d[a];
+                            pI[a] -= fI * d[a];
+                        }
+                        const T lfI = uppI + lapsc * fI;
+                        lI += lfI;
+                        if (PP) plI += lfI;
+                    }
+                }
+            };
+            auto rotate = [&]() {
+#pragma unroll
+                for (int a = 0; a < D; a++) {
+                    cp[a] = wave_rotate(cp[a]);
+                    pR[a] = wave_rotate(pR[a]);
+                    if (CPLX) pI[a] = wave_rotate(pI[a]);
+                }
+                if (PP) {
+                    plR = wave_rotate(plR);
+                    if (CPLX) plI = wave_rotate(plI);
+                }
+            };
+            if (!diag && !folded && (Bk + 1) * 64 <= N) {
+                // both blocks full, no pair repeats: every lane is live at every step
+                for (int u = 0; u < nsteps; u++) {
+                    pair();
+                    rotate();
+                }
+            } else {
+                for (int u = 0; u < nsteps; u++) {
+                    const int s = s_first + u;
+                    const int ci = Bk * 64 + ((lane + s) & cmask);
+                    if (rval && ci < N && (!diag || s < 32 || lane < 32)) pair();
+                    rotate();
+                }
+            }
+            vpot += (double)vs;
+            // column sums: lane l now holds ring slot (l + s_first + nsteps) mod 64
+            cidx = Bk * 64 + ((lane + s_first + nsteps) & cmask);
+            if (cidx < N) {
+#pragma unroll
+                for (int a = 0; a < D; a++) {
+                    lds_add(gsum + a * N + cidx, (double)pR[a]);
+                    if (CPLX) lds_add(gsum + (D + a) * N + cidx, (double)pI[a]);
+                }
+                if (PP) {
+                    lds_add(lsum + cidx, (double)plR);
+                    if (CPLX) lds_add(lsum + N + cidx, (double)plI);
+                }
+            }
+            if (rval) {
+#pragma unroll
+                for (int a = 0; a < D; a++) {
+                    lds_add(gsum + a * N + ra, (double)gR[a]);
+                    if (CPLX) lds_add(gsum + (D + a) * N + ra, (double)gI[a]);
+                }
+                if (PP) {
+                    lds_add(lsum + ra, (double)lR);
+                    if (CPLX) lds_add(lsum + N + ra, (double)lI);
+                } else {
+                    // every pair term enters the Laplacians of both particles
+                    lap_tot_R += 2.0 * (double)lR;
+                    if (CPLX) lap_tot_I += 2.0 * (double)lI;
+                }
+            }
+        }
+    }
+}
+
+// k-space gradient and Laplacian of sum_k alpha_k |rho_k|^2 for every particle
+// (lane = particle, the k range split over the 4 waves).
+template <typename T, int D, bool PP, bool PER, bool CPLX>
+__device__ __forceinline__ void eval_kgrad(const KSys& sy, const KTables<T>& tb, const typename Coord<T, PER>::S* pos,
+                                           const T* rho, const T* kq, double* gsum, double* lsum, int wave, int lane,
+                                           double& lap_tot_R, double& lap_tot_I)
+{
+    using C = Coord<T, PER>;
+    const int N = sy.N, K = sy.Kh;
+    const int kchunk = (K + kWaves - 1) / kWaves;
+    const int k_lo = wave * kchunk, k_hi = min(K, (wave + 1) * kchunk);
+    for (int rb = 0; rb < N; rb += 64) {
+        const int n = rb + lane;
+        if (n >= N) continue;
+        T rn[D];
+#pragma unroll
+        for (int a = 0; a < D; a++) rn[a] = C::raw(pos[a * N + n]);
+        T gR[D], gI[D];
+#pragma unroll
+        for (int a = 0; a < D; a++) gR[a] = gI[a] = (T)0;
+        T lR = (T)0, lI = (T)0;
+        // grad |rho_k|^2 = -2 k Im(e^{ikr} conj rho), k = 2 pi kv;
+        // lap |rho_k|^2 = 2 |k|^2 (1 - Re(e^{ikr} conj rho)); the
+        // constant factors are applied after the loop
+        for (int k = k_lo; k < k_hi; k++) {
+            const T* q = tb.kv + k * 4;
+            const T* qs = kq + k * 4;
+            T ph = (T)0;
+#pragma unroll
+            for (int a = 0; a < D; a++) ph += qs[a] * rn[a];
+            T sn, cs;
+            Prec<T>::sincos_small(ph, &sn, &cs);
+            const T rr = rho[2 * k], ri = rho[2 * k + 1];
+            const T pre = cs * rr + sn * ri;    // Re(e^{ikr} conj(rho))
+            const T pim = sn * rr - cs * ri;    // Im(e^{ikr} conj(rho))
+            const T aR = tb.alpha[2 * k];
+            const T sR = aR * pim;
+#pragma unroll
+            for (int a = 0; a < D; a++) gR[a] += sR * q[a];
+            const T lk = q[3] * ((T)1 - pre);
+            lR += aR * lk;
+            if constexpr (CPLX) {
+                const T aI = tb.alpha[2 * k + 1];
+                const T sI = aI * pim;
+#pragma unroll
+                for (int a = 0; a < D; a++) gI[a] += sI * q[a];
+                lI += aI * lk;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            lds_add(gsum + a * N + n, (double)(gR[a] * (T)(-2.0 * 6.283185307179586)));
+            if (CPLX) lds_add(gsum + (D + a) * N + n, (double)(gI[a] * (T)(-2.0 * 6.283185307179586)));
+        }
+        if (PP) {
+            lds_add(lsum + n, (double)(lR * (T)2));
+            if (CPLX) lds_add(lsum + N + n, (double)(lI * (T)2));
+        } else {
+            lap_tot_R += (double)(lR * (T)2);
+            if (CPLX) lap_tot_I += (double)(lI * (T)2);
+        }
+    }
+}
+
+// Pairs and the k-space gradient for one value of CPLX, the potential type
+// resolved to a template argument.
+template <typename T, int D, bool PP, bool PER, bool CPLX>
+__device__ __forceinline__ void eval_forces(const KSys& sy, const KTables<T>& tb,
+                                            const typename Coord<T, PER>::S* pos, const T* dsp, double* hist,
+                                            double* gsum, double* lsum, int wave, int lane, double& vpot,
+                                            double& lap_tot_R, double& lap_tot_I)
+{
+    switch (sy.pot_type) {
+    case 1: eval_pairs<T, D, PP, PER, 1, CPLX>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
+    case 2: eval_pairs<T, D, PP, PER, 2, CPLX>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
+    case 3: eval_pairs<T, D, PP, PER, 3, CPLX>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
+    default: eval_pairs<T, D, PP, PER, 0, CPLX>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
+    }
+}
+
 // One workgroup evaluates walker w of the system described by (sy, tb).
 template <typename T, int D, bool PP, bool PER, typename TO>
 __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, const double* __restrict__ Rg,
                                           int64_t W, const EvalOut<TO>& out, const T* __restrict__ rho_in, int64_t w)
 {
+    using C = Coord<T, PER>;
+    using S = typename C::S;
     if (w >= W) return;
     const int N = sy.N, Ps = sy.P_s, K = sy.Kh;   // K: +-k representatives
     const int tid = threadIdx.x, lane = tid & 63;
@@ -101,28 +413,25 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     // LDS carve-up (sizes mirrored by eval_lds_bytes); per-particle arrays are
     // structure-of-arrays so consecutive particles sit in consecutive banks
     double* red = reinterpret_cast<double*>(smem);             // 8 doubles
-    double* hist = red + 8;                                    // (Ps-1) * 4 * kHistCols
-    double* gsum = hist + (size_t)(Ps - 1) * 4 * kHistCols;    // [2D][N]: Re g, Im g
+    double* hist = red + 8;                                    // P_s * 4 * kHistCols
+    double* gsum = hist + (size_t)Ps * 4 * kHistCols;          // [2D][N]: Re g, Im g
     double* lsum = gsum + (size_t)N * 2 * D;                   // [2][N] (PP only)
-    T* pos = reinterpret_cast<T*>(lsum + (PP ? (size_t)N * 2 : 0));   // [D][N]
-    T* rho = pos + (size_t)N * D;                              // 2K
-    // derivative table (Ps-1) * 8, 16-byte aligned for 128-bit reads
-    T* dsp = reinterpret_cast<T*>(smem + ((reinterpret_cast<unsigned char*>(rho + 2 * (size_t)K) - smem + 15) &
-                                          ~(size_t)15));
+    S* pos = reinterpret_cast<S*>(lsum + (PP ? (size_t)N * 2 : 0));   // [D][N]
+    T* rho = reinterpret_cast<T*>(pos + (size_t)N * D);        // 2K
+    // k-vectors in revolutions per coordinate unit ([K][4]), then the
+    // derivative table (P_s * kDsplStride), both 16-byte aligned
+    T* kq = reinterpret_cast<T*>(smem + ((reinterpret_cast<unsigned char*>(rho + 2 * (size_t)K) - smem + 15) &
+                                         ~(size_t)15));
+    T* dsp = kq + 4 * (size_t)K;
     const T* kv = tb.kv;                                       // K * 4 (global, read with uniform k)
 
-    // periodic box: coordinates in box units (x / L, wrapped into the primary
-    // cell -- every quantity is invariant under it), so the minimum image is
-    // d - rint(d); lengths below are scaled to match
+    // periodic box: coordinates in box units wrapped into the primary cell
+    // (every quantity is invariant under it), stored as Coord values
     const double scale = PER ? sy.Linv : 1.0;
     const double* Rw = Rg + w * (size_t)N * D;
     for (int i = tid; i < N; i += kThreads) {
 #pragma unroll
-        for (int a = 0; a < D; a++) {
-            double x = Rw[(size_t)i * D + a] * scale;
-            if (PER) x = x - rint(x);
-            pos[a * N + i] = (T)x;
-        }
+        for (int a = 0; a < D; a++) pos[a * N + i] = C::load(Rw[(size_t)i * D + a] * scale);
 #pragma unroll
         for (int c = 0; c < 2 * D; c++) gsum[c * N + i] = 0.0;
         if (PP) {
@@ -130,166 +439,26 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
             lsum[N + i] = 0.0;
         }
     }
-    for (int i = tid; i < (Ps - 1) * 4 * kHistCols; i += kThreads) hist[i] = 0.0;
-    for (int i = tid; i < (Ps - 1) * 8; i += kThreads) dsp[i] = tb.dspl[i];
-    __syncthreads();
-
-    // ------------------------------------------------------------ pairs
-    const T hsc = (T)(sy.hinv / scale), rc = (T)(sy.rc * scale);
-    const T pcut = (T)(sy.pot_cutoff * scale);
-    const T pa = (T)(sy.pot_a * scale), pb = (T)sy.pot_b;
-    const T lapsc = (T)((sy.D - 1) * scale);   // (D-1)/r = (D-1) scale / r_box
-    double vpot = 0.0, outer = 0.0, lap_tot_R = 0.0, lap_tot_I = 0.0;
-    const T hinvT = (T)sy.hinv;
-
-    // Pairs in 64 x 64 tiles (A, B), A <= B, of particle blocks.  Lane l owns
-    // row particle 64 A + l; column particles ride a register ring that
-    // rotates one lane per step (wave_rol), so at step s lane l meets column
-    // 64 B + ((l + s) mod 64) and after 64 steps every row has met every
-    // column.  Diagonal tiles take s = 1..32 (s = 32 by lanes < 32 only), so
-    // every unordered pair occurs once.  Row sums stay in registers, column
-    // sums rotate with the column coordinates; both reach LDS once per tile
-    // segment.  The steps of all tiles are split evenly over the 4 waves.
-    const int nblk = (N + 63) / 64;
-    const int total = nblk * (nblk - 1) / 2 * 64 + nblk * 32;
-    const int g0 = total * wave / kWaves, g1 = total * (wave + 1) / kWaves;
-    int off = 0;
-    for (int A = 0; A < nblk; A++) {
-        for (int Bk = A; Bk < nblk; Bk++) {
-            const bool diag = A == Bk;
-            const int cnt = diag ? 32 : 64;
-            const int lo = max(g0, off), hi = min(g1, off + cnt);
-            off += cnt;
-            if (lo >= hi) continue;
-            const int s_first = lo - (off - cnt) + (diag ? 1 : 0);
-            const int nsteps = hi - lo;
-            const int ra = A * 64 + lane;
-            const bool rval = ra < N;
-            T rn[D], cp[D];
-            int cidx = Bk * 64 + ((lane + s_first) & 63);
+    for (int i = tid; i < Ps * 4 * kHistCols; i += kThreads) hist[i] = 0.0;
+    for (int i = tid; i < Ps * kDsplStride; i += kThreads) dsp[i] = tb.dspl[i];
+    for (int k = tid; k < K; k += kThreads) {
+        // periodic: kv L is the integer lattice vector n (rounded to remove
+        // the FP32 storage error of n / L)
 #pragma unroll
-            for (int a = 0; a < D; a++) {
-                rn[a] = rval ? pos[a * N + ra] : (T)0;
-                cp[a] = cidx < N ? pos[a * N + cidx] : (T)0;
-            }
-            T gR[D], gI[D], pR[D], pI[D];
-#pragma unroll
-            for (int a = 0; a < D; a++) gR[a] = gI[a] = pR[a] = pI[a] = (T)0;
-            T lR = (T)0, lI = (T)0, plR = (T)0, plI = (T)0;
-            for (int u = 0; u < nsteps; u++) {
-                const int s = s_first + u;
-                cidx = Bk * 64 + ((lane + s) & 63);
-                const bool live = rval && cidx < N && (!diag || s < 32 || lane < 32);
-                if (live) {
-                    T d[D];
-                    T r2 = (T)0;
-#pragma unroll
-                    for (int a = 0; a < D; a++) {
-                        d[a] = rn[a] - cp[a];
-                        if (PER) d[a] -= Prec<T>::rint_(d[a]);
-                        r2 += d[a] * d[a];
-                    }
-                    // FP32: one v_rsq gives 1/r, and r = r^2 / r; FP64: sqrt and a divide
-                    T r, rinv;
-                    Prec<T>::r_and_rinv(r2, &r, &rinv);
-                    if (sy.pot_type != 0 && r <= pcut) {
-                        T v;
-                        if (sy.pot_type == 1) {
-                            v = (r < pa) ? pb : (T)0;
-                        } else if (sy.pot_type == 2) {
-                            T x = r / pa;
-                            v = pb * Prec<T>::exp_((T)-0.5 * x * x);
-                        } else {
-                            T q = pa * pa * Prec<T>::rcp_(r2);
-                            T q6 = q * q * q;
-                            v = (T)4 * pb * q6 * (q6 - (T)1);
-                        }
-                        vpot += (double)v;
-                    }
-                    if (r <= rc) {
-                        T t;
-                        const int j = knot_interval(r, hsc, Ps, &t);
-                        // moments 1, t, t^2, t^3 of the pair's position in knot
-                        // interval j (FP64); the basis sums B_s are rebuilt from
-                        // them at the end
-                        const double td = (double)t, t2 = td * td, t3 = t2 * td;
-                        double* hb = hist + j * 4 * kHistCols + (lane & (kHistCols - 1));
-                        lds_add(hb, 1.0);
-                        lds_add(hb + kHistCols, td);
-                        lds_add(hb + 2 * kHistCols, t2);
-                        lds_add(hb + 3 * kHistCols, t3);
-                        // u'(r), u''(r) per unit length on interval j; the
-                        // 24-bit multiply keeps the LDS offset a full-rate op
-                        const T* c = reinterpret_cast<const T*>(reinterpret_cast<const unsigned char*>(dsp) +
-                                                                __umul24((unsigned)j, 8 * sizeof(T)));
-                        const T upR = c[0] + t * (c[1] + t * c[2]);
-                        const T uppR = hinvT * c[1] + t * c[3];
-                        const T upI = c[4] + t * (c[5] + t * c[6]);
-                        const T uppI = hinvT * c[5] + t * c[7];
-                        const T fR = upR * rinv, fI = upI * rinv;
-#pragma unroll
-                        for (int a = 0; a < D; a++) {
-                            gR[a] += fR * d[a];
-                            gI[a] += fI * d[a];
-                            pR[a] -= fR * d[a];
-                            pI[a] -= fI * d[a];
-                        }
-                        const T lfR = uppR + lapsc * fR;
-                        const T lfI = uppI + lapsc * fI;
-                        lR += lfR;
-                        lI += lfI;
-                        if (PP) {
-                            plR += lfR;
-                            plI += lfI;
-                        }
-                    } else {
-                        outer += 1.0;
-                    }
-                }
-#pragma unroll
-                for (int a = 0; a < D; a++) {
-                    cp[a] = wave_rotate(cp[a]);
-                    pR[a] = wave_rotate(pR[a]);
-                    pI[a] = wave_rotate(pI[a]);
-                }
-                if (PP) {
-                    plR = wave_rotate(plR);
-                    plI = wave_rotate(plI);
-                }
-            }
-            // column sums: lane l now holds column 64 B + ((l + s_first + nsteps) mod 64)
-            cidx = Bk * 64 + ((lane + s_first + nsteps) & 63);
-            if (cidx < N) {
-#pragma unroll
-                for (int a = 0; a < D; a++) {
-                    lds_add(gsum + a * N + cidx, (double)pR[a]);
-                    lds_add(gsum + (D + a) * N + cidx, (double)pI[a]);
-                }
-                if (PP) {
-                    lds_add(lsum + cidx, (double)plR);
-                    lds_add(lsum + N + cidx, (double)plI);
-                }
-            }
-            if (rval) {
-#pragma unroll
-                for (int a = 0; a < D; a++) {
-                    lds_add(gsum + a * N + ra, (double)gR[a]);
-                    lds_add(gsum + (D + a) * N + ra, (double)gI[a]);
-                }
-                if (PP) {
-                    lds_add(lsum + ra, (double)lR);
-                    lds_add(lsum + N + ra, (double)lI);
-                } else {
-                    // every pair term enters the Laplacians of both particles
-                    lap_tot_R += 2.0 * (double)lR;
-                    lap_tot_I += 2.0 * (double)lI;
-                }
-            }
+        for (int a = 0; a < D; a++) {
+            const double n = PER ? rint((double)kv[k * 4 + a] / scale) : (double)kv[k * 4 + a];
+            kq[k * 4 + a] = (T)(n * C::kUnit);
         }
+        for (int a = D; a < 4; a++) kq[k * 4 + a] = (T)0;
     }
+    // any nonzero imaginary parameter?  If none, the imaginary parts of the
+    // pair and k-space sums are exact zeros and are skipped
+    int nz = 0;
+    for (int i = tid; i < sy.P; i += kThreads) nz |= tb.uI[i] != 0.0;
+    const bool cplx = __syncthreads_or(nz) != 0;
 
     // ------------------------------------------------------------ rho_k
-    const T ksc = (T)(1.0 / scale);   // phase in revolutions = (kv / scale) . s
+    // phase in revolutions = kq . raw coordinate, |phase| <= sum_a |n_a| / 2
     double ex_k = 0.0;
     for (int k = tid; k < K; k += kThreads) {
         T re = (T)0, im = (T)0;
@@ -299,13 +468,13 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
         } else {
             T q[D];
 #pragma unroll
-            for (int a = 0; a < D; a++) q[a] = kv[k * 4 + a] * ksc;
+            for (int a = 0; a < D; a++) q[a] = kq[k * 4 + a];
             for (int n = 0; n < N; n++) {
                 T ph = (T)0;
 #pragma unroll
-                for (int a = 0; a < D; a++) ph += q[a] * pos[a * N + n];
+                for (int a = 0; a < D; a++) ph += q[a] * C::raw(pos[a * N + n]);
                 T sn, cs;
-                Prec<T>::sincos_rev(ph, &sn, &cs);
+                Prec<T>::sincos_small(ph, &sn, &cs);
                 re += cs;
                 im += sn;
             }
@@ -325,75 +494,31 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     }
     __syncthreads();
 
-    // ------------------------------------------------------------ k-space gradient / Laplacian
-    if (K > 0) {
-        const int kchunk = (K + kWaves - 1) / kWaves;
-        const int k_lo = wave * kchunk, k_hi = min(K, (wave + 1) * kchunk);
-        for (int rb = 0; rb < N; rb += 64) {
-            const int n = rb + lane;
-            if (n < N) {
-                T rn[D];
-#pragma unroll
-                for (int a = 0; a < D; a++) rn[a] = pos[a * N + n];
-                T gR[D], gI[D];
-#pragma unroll
-                for (int a = 0; a < D; a++) gR[a] = gI[a] = (T)0;
-                T lR = (T)0, lI = (T)0;
-                // grad |rho_k|^2 = -2 k Im(e^{ikr} conj rho), k = 2 pi kv;
-                // lap |rho_k|^2 = 2 |k|^2 (1 - Re(e^{ikr} conj rho)); the
-                // constant factors are applied after the loop
-                for (int k = k_lo; k < k_hi; k++) {
-                    const T* q = tb.kv + k * 4;
-                    T ph = (T)0;
-#pragma unroll
-                    for (int a = 0; a < D; a++) ph += (q[a] * ksc) * rn[a];
-                    T sn, cs;
-                    Prec<T>::sincos_rev(ph, &sn, &cs);
-                    const T rr = rho[2 * k], ri = rho[2 * k + 1];
-                    const T pre = cs * rr + sn * ri;    // Re(e^{ikr} conj(rho))
-                    const T pim = sn * rr - cs * ri;    // Im(e^{ikr} conj(rho))
-                    const T aR = tb.alpha[2 * k], aI = tb.alpha[2 * k + 1];
-                    const T sR = aR * pim, sI = aI * pim;
-#pragma unroll
-                    for (int a = 0; a < D; a++) {
-                        gR[a] += sR * q[a];
-                        gI[a] += sI * q[a];
-                    }
-                    const T lk = q[3] * ((T)1 - pre);
-                    lR += aR * lk;
-                    lI += aI * lk;
-                }
-#pragma unroll
-                for (int a = 0; a < D; a++) {
-                    gR[a] *= (T)(-2.0 * 6.283185307179586);
-                    gI[a] *= (T)(-2.0 * 6.283185307179586);
-                }
-                lR *= (T)2;
-                lI *= (T)2;
-#pragma unroll
-                for (int a = 0; a < D; a++) {
-                    lds_add(gsum + a * N + n, (double)gR[a]);
-                    lds_add(gsum + (D + a) * N + n, (double)gI[a]);
-                }
-                if (PP) {
-                    lds_add(lsum + n, (double)lR);
-                    lds_add(lsum + N + n, (double)lI);
-                } else {
-                    lap_tot_R += (double)lR;
-                    lap_tot_I += (double)lI;
-                }
-            }
-        }
+    // ------------------------------------------------------------ pairs, k-space gradient / Laplacian
+    double vpot = 0.0, lap_tot_R = 0.0, lap_tot_I = 0.0;
+    if (PP || cplx) {
+        eval_forces<T, D, PP, PER, true>(sy, tb, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I);
+        if (K > 0) eval_kgrad<T, D, PP, PER, true>(sy, tb, pos, rho, kq, gsum, lsum, wave, lane, lap_tot_R, lap_tot_I);
+    } else if constexpr (!PP) {
+        eval_forces<T, D, PP, PER, false>(sy, tb, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I);
+        if (K > 0) eval_kgrad<T, D, PP, PER, false>(sy, tb, pos, rho, kq, gsum, lsum, wave, lane, lap_tot_R, lap_tot_I);
     }
     __syncthreads();
 
     // histogram columns -> column 0 (interval moments M[j][p])
-    for (int e = tid; e < (Ps - 1) * 4; e += kThreads) {
+    for (int e = tid; e < Ps * 4; e += kThreads) {
         double m = 0.0;
         for (int c = 0; c < kHistCols; c++) m += hist[e * kHistCols + c];
         hist[e * kHistCols] = m;
     }
     __syncthreads();
+    // pairs beyond the cutoff: all pairs minus those in the histogram (exact counts)
+    double outer = 0.0;
+    if (tid == 0) {
+        double inner = 0.0;
+        for (int j = 0; j < Ps; j++) inner += hist[j * 4 * kHistCols];
+        outer = 0.5 * (double)N * (double)(N - 1) - inner;
+    }
 
     // ------------------------------------------------------------ assembly
     const double obR = sy.one_body ? tb.uR[0] : 0.0, obI = sy.one_body ? tb.uI[0] : 0.0;
@@ -456,7 +581,7 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
             double v = 0.0;
             for (int m = 0; m < 4; m++) {
                 const int j = s - m;
-                if (j < 0 || j > Ps - 2) continue;
+                if (j < 0 || j > Ps - 1) continue;
 #pragma unroll
                 for (int p = 0; p < 4; p++) v += beta[m][p] * hist[(j * 4 + p) * kHistCols];
             }
@@ -480,7 +605,6 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     sr2 = block_sum(sr2, red);
     vext = block_sum(vext, red);
     vpot = block_sum(vpot, red);
-    outer = block_sum(outer, red);
     double ex = block_sum(ex_k + ex_p, red);
     if (tid == 0) {
         if (sy.one_body) {
@@ -539,12 +663,13 @@ __global__ __launch_bounds__(kThreads, 4) void eval_batch_kernel(EvalBatch<T> bt
 template <typename T>
 size_t eval_lds_bytes(const KSys& sy, bool per_particle)
 {
-    const size_t nd = (size_t)(sy.P_s - 1) * 4 * kHistCols + (size_t)sy.N * 2 * sy.D +
+    const size_t nd = (size_t)sy.P_s * 4 * kHistCols + (size_t)sy.N * 2 * sy.D +
                       (per_particle ? 2 * (size_t)sy.N : 0);
+    // coordinates (Coord storage has the size of T) and rho
     const size_t nt = (size_t)sy.N * sy.D + 2 * (size_t)sy.Kh;
-    // the derivative table (8 entries per interval) starts 16-byte aligned
+    // the k-vector table and the derivative table start 16-byte aligned
     const size_t head = (64 + nd * sizeof(double) + nt * sizeof(T) + 15) & ~(size_t)15;
-    return head + (size_t)(sy.P_s - 1) * 8 * sizeof(T);
+    return head + (4 * (size_t)sy.Kh + (size_t)sy.P_s * kDsplStride) * sizeof(T);
 }
 
 template <typename T, int D, bool PP, bool PER, typename TO>

@@ -39,51 +39,6 @@ constexpr int kThreads = 256;
 template <typename T>
 struct Vec4 { T x, y, z, w; };
 
-// Storage of one coordinate in LDS.  FP32 in a periodic box: the position as
-// an unsigned 32-bit fraction of the box, so a wrapping integer difference is
-// the minimum image (one subtract and one convert per component, exact);
-// distances come out in units of 2^-32 box lengths.  Otherwise the compute
-// type in box units (periodic, minimum image by rint) or in real units.
-template <typename T, bool PER>
-struct Coord {
-    using S = T;
-    static constexpr double kUnit = 1.0;                 // length unit of diff() in box (or real) units
-    __device__ static S load(double x) { return (T)(PER ? x - rint(x) : x); }
-    __device__ static double store(S p) { return (double)p; }
-    __device__ static T diff(S a, S b)
-    {
-        T d = a - b;
-        if (PER) d -= Prec<T>::rint_(d);
-        return d;
-    }
-    __device__ static S move(S p, T dz)
-    {
-        T v = p + dz;
-        if (PER) v -= Prec<T>::rint_(v);
-        return v;
-    }
-    __device__ static T unit_coord(S p) { return p; }   // in box (or real) units
-};
-template <>
-struct Coord<float, true> {
-    using S = uint32_t;
-    static constexpr double kUnit = 2.3283064365386963e-10;   // 2^-32
-    __device__ static S load(double x)
-    {
-        const double f = x - floor(x);   // [0, 1)
-        return (uint32_t)(uint64_t)(f * 4294967296.0);
-    }
-    __device__ static double store(S p) { return (double)(int32_t)p * kUnit; }   // [-1/2, 1/2)
-    __device__ static float diff(S a, S b) { return (float)(int32_t)(a - b); }
-    __device__ static S move(S p, float dz)
-    {
-        // dz in box units; only its image in [-1/2, 1/2) matters
-        const float w = dz - __builtin_rintf(dz);
-        return p + (uint32_t)__float2int_rn(w * 4294967296.0f);
-    }
-    __device__ static float unit_coord(S p) { return (float)(int32_t)p * (float)kUnit; }
-};
-
 template <int LPW, typename T>
 __device__ __forceinline__ T bcast(T v, int src)
 {

@@ -73,6 +73,9 @@ CASES = {
                             hbar2_2m=0.5, trap_omega=1.0, rc=3.0), 8),
     "lj3d_n64_even": (dict(N=64, D=3, L=5.6, P_s=20, pot="LJ", sysparams=(1.0, 1.0), K=8), 4),
     "bulk3d_n343_k256": (dict(N=343, D=3, L=9.79, P_s=40, pot="LJ", sysparams=(1.0, 1.0), K=256), 2),
+    # 90 = 64 + 26: the last block is met twice per ring (folded); 100 = 64 + 36: not folded
+    "sqwell2d_n90": (dict(N=90, D=2, L=10.0, P_s=14, pot="SQUARE_WELL", sysparams=(0.6, 1.5), K=12), 3),
+    "lj3d_n100": (dict(N=100, D=3, L=7.0, P_s=18, pot="LJ", sysparams=(1.0, 1.0)), 3),
 }
 
 
@@ -141,3 +144,28 @@ def test_mixed_vs_oracle(name):
     assert e_x < 2e-6
     assert e_E < 5e-5
     assert e_g < 5e-5
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_real_parameters_vs_oracle(name):
+    """All imaginary parameters zero (imaginary-time runs from a real start):
+    the kernels skip the imaginary parts, which are exact zeros."""
+    cfg, s, R, uR, uI = _case_data(name, seed=1)
+    uI = np.zeros_like(uI)
+    print("case real", name, flush=True)
+    ref = o.eval_configs(s, R, uR, uI, phiR=0.1, time=1.0)
+    sc = energy_scale(ref)
+    with _ctx(cfg) as ctx:
+        ctx.set_params(uR, uI, 0.1, 0.0)
+        ctx.set_time(1.0)
+        f64 = ctx.eval_configs(R, precision=tdvmc.FP64)
+        mix = ctx.eval_configs(R, precision=tdvmc.MIXED)
+    assert np.all(f64["EI"] == 0.0) and np.all(mix["EI"] == 0.0)
+    assert rel(f64["O"], ref["O"]) < 1e-10
+    assert rel(f64["expo"], ref["expo"]) < 1e-10
+    assert np.max(np.abs(f64["ER"] - ref["ER"]) / sc) < 1e-10
+    for k in (0, 1, 3, 4, 5, 6, 7, 8):
+        assert rel(f64["other"][:, k], ref["other"][:, k]) < 1e-10 * max(1.0, np.max(np.abs(ref["other"][:, 0])) /
+                                                                        max(np.max(np.abs(ref["other"][:, k])), 1e-300)), k
+    assert rel(mix["O"], ref["O"]) < 2e-6
+    assert np.max(np.abs(mix["ER"] - ref["ER"]) / sc) < 5e-5
