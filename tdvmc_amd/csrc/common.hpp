@@ -36,9 +36,9 @@ template <typename T>
 struct KTables {
     const T* spl;     // [(P_s-1)][8]: cubic-in-t coefficients A0..A3 of Re u, then of Im u
     const T* dspl;    // [P_s][kDsplStride]: derivatives per unit length on knot interval j,
-                      //   u' = d1 + t (d2 + t d3), u'' = hinv (d2 + 2 t d3), stored as
-                      //   (d1, d2, d3 of Re u, d1 | d2, d3 of Im u, 0, 0); row P_s-1 holds
-                      //   u'(rc) and u''(rc) / hinv as d1, d2 (used at t = 0 only)
+                      //   u' = d1 + t (d2 + t d3), u'' = hinv (d2 + t e3), e3 = 2 d3, stored as
+                      //   (d1, d2, d3, e3) of Re u, then of Im u; row P_s-1 holds u'(rc)
+                      //   and u''(rc) / hinv as d1, d2 (used at t = 0 only)
     const T* kv;      // [Kh][4]: n/L of the representative (phase in revolutions = kv . r), |k|^2
     const T* alpha;   // [Kh][2]: (Re, Im) of alpha_k (+ alpha_{-k} when paired)
     const int* kmap;  // [Kh][2]: parameter slots (relative to off_k) of k and of -k (-1: unpaired)

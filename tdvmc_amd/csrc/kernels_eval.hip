@@ -144,7 +144,8 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
     const double unit = C::kUnit / (PER ? sy.Linv : 1.0);   // length of one coordinate-difference unit
     const T hsc = (T)(sy.hinv * unit), rc = (T)(sy.rc / unit);
     const T hinvT = (T)sy.hinv;
-    const T lapsc = (T)((sy.D - 1) / unit);   // (D-1)/r = lapsc * rinv
+    const double lapscD = (sy.D - 1) / unit;   // (D-1)/r = lapsc * rinv
+    const T lapsc = (T)lapscD;
     PairConsts<T> pc{};
     if constexpr (POT == 1) {
         pc.a = (T)(sy.pot_a / unit);
@@ -201,7 +202,10 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
             T gR[D], gI[D], pR[D], pI[D];
 #pragma unroll
             for (int a = 0; a < D; a++) gR[a] = gI[a] = pR[a] = pI[a] = (T)0;
-            T lR = (T)0, lI = (T)0, plR = (T)0, plI = (T)0, vs = (T)0;
+            // Laplacian pieces: lR / lI = sum u''/hinv, mR / mI = sum u'/r (without
+            // per-particle output; combined once per segment), or the full
+            // per-pair term (with it)
+            T lR = (T)0, lI = (T)0, mR = (T)0, mI = (T)0, plR = (T)0, plI = (T)0, vs = (T)0;
 
             auto pair = [&]() {
                 T d[D];
@@ -233,28 +237,36 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
                     // u'(r), u''(r) per unit length on interval j
                     const T* c = dsp + j * kDsplStride;
                     const T upR = c[0] + t * (c[1] + t * c[2]);
-                    const T uppR = hinvT * (c[1] + (t + t) * c[2]);
                     const T fR = upR * rinv;
 #pragma unroll
                     for (int a = 0; a < D; a++) {
                         gR[a] += fR * d[a];
                         pR[a] -= fR * d[a];
                     }
-                    const T lfR = uppR + lapsc * fR;
-                    lR += lfR;
-                    if (PP) plR += lfR;
+                    if constexpr (PP) {
+                        const T lfR = hinvT * (c[1] + t * c[3]) + lapsc * fR;
+                        lR += lfR;
+                        plR += lfR;
+                    } else {
+                        lR += c[1] + t * c[3];
+                        mR += fR;
+                    }
                     if constexpr (CPLX) {
-                        const T upI = c[3] + t * (c[4] + t * c[5]);
-                        const T uppI = hinvT * (c[4] + (t + t) * c[5]);
+                        const T upI = c[4] + t * (c[5] + t * c[6]);
                         const T fI = upI * rinv;
 #pragma unroll
                         for (int a = 0; a < D; a++) {
                             gI[a] += fI * d[a];
                             pI[a] -= fI * d[a];
                         }
-                        const T lfI = uppI + lapsc * fI;
-                        lI += lfI;
-                        if (PP) plI += lfI;
+                        if constexpr (PP) {
+                            const T lfI = hinvT * (c[5] + t * c[7]) + lapsc * fI;
+                            lI += lfI;
+                            plI += lfI;
+                        } else {
+                            lI += c[5] + t * c[7];
+                            mI += fI;
+                        }
                     }
                 }
             };
@@ -309,8 +321,8 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
                     if (CPLX) lds_add(lsum + N + ra, (double)lI);
                 } else {
                     // every pair term enters the Laplacians of both particles
-                    lap_tot_R += 2.0 * (double)lR;
-                    if (CPLX) lap_tot_I += 2.0 * (double)lI;
+                    lap_tot_R += 2.0 * (sy.hinv * (double)lR + lapscD * (double)mR);
+                    if (CPLX) lap_tot_I += 2.0 * (sy.hinv * (double)lI + lapscD * (double)mI);
                 }
             }
         }

@@ -40,26 +40,22 @@ __device__ void prep_tables_body(const KSys& sy, const double* __restrict__ uR, 
                 spl[j * 8 + part * 4 + 2] = (T)A2;
                 spl[j * 8 + part * 4 + 3] = (T)A3;
                 // derivatives per unit length, t = (r - knot_j) / h:
-                // u' = d1 + t (d2 + t d3), u'' = hinv (d2 + 2 t d3)
+                // u' = d1 + t (d2 + t d3), u'' = hinv (d2 + t e3)
                 const double hi = sy.hinv;
-                T* dr = dspl + j * kDsplStride + part * 3;
+                T* dr = dspl + j * kDsplStride + part * 4;
                 dr[0] = (T)(A1 * hi);
                 dr[1] = (T)(2.0 * A2 * hi);
                 dr[2] = (T)(3.0 * A3 * hi);
+                dr[3] = (T)(6.0 * A3 * hi);
                 if (j == Ps - 2) {
                     // extra row for r = rc exactly (interval P_s-1, t = 0): the
                     // values of interval P_s-2 at t = 1
-                    T* de = dspl + (Ps - 1) * kDsplStride + part * 3;
+                    T* de = dspl + (Ps - 1) * kDsplStride + part * 4;
                     de[0] = (T)((A1 + 2.0 * A2 + 3.0 * A3) * hi);
                     de[1] = (T)((2.0 * A2 + 6.0 * A3) * hi);
                     de[2] = (T)0;
+                    de[3] = (T)0;
                 }
-            }
-            dspl[j * kDsplStride + 6] = (T)0;
-            dspl[j * kDsplStride + 7] = (T)0;
-            if (j == Ps - 2) {
-                dspl[(Ps - 1) * kDsplStride + 6] = (T)0;
-                dspl[(Ps - 1) * kDsplStride + 7] = (T)0;
             }
         } else {
             // amplitude of |rho_k|^2: alpha_k, plus alpha_{-k} when -k is in the set
