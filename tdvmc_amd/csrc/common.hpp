@@ -168,12 +168,16 @@ struct Coord {
         if (PER) d -= Prec<T>::rint_(d);
         return d;
     }
-    __device__ static S move(S p, T dz)
+    // a displacement dz (box or real units) as an increment, and its application
+    using Inc = T;
+    __device__ static Inc inc(T dz) { return dz; }
+    __device__ static S add(S p, Inc d)
     {
-        T v = p + dz;
+        T v = p + d;
         if (PER) v -= Prec<T>::rint_(v);
         return v;
     }
+    __device__ static S move(S p, T dz) { return add(p, inc(dz)); }
     __device__ static T unit_coord(S p) { return p; }   // in box (or real) units
     __device__ static T raw(S p) { return p; }           // value in units of kUnit
 };
@@ -188,12 +192,15 @@ struct Coord<float, true> {
     }
     __device__ static double store(S p) { return (double)(int32_t)p * kUnit; }   // [-1/2, 1/2)
     __device__ static float diff(S a, S b) { return (float)(int32_t)(a - b); }
-    __device__ static S move(S p, float dz)
+    using Inc = uint32_t;
+    __device__ static Inc inc(float dz)
     {
         // dz in box units; only its image in [-1/2, 1/2) matters
         const float w = dz - __builtin_rintf(dz);
-        return p + (uint32_t)__float2int_rn(w * 4294967296.0f);
+        return (uint32_t)__float2int_rn(w * 4294967296.0f);
     }
+    __device__ static S add(S p, Inc d) { return p + d; }   // wraps: the periodic image
+    __device__ static S move(S p, float dz) { return add(p, inc(dz)); }
     __device__ static float unit_coord(S p) { return (float)(int32_t)p * (float)kUnit; }
     __device__ static float raw(S p) { return (float)(int32_t)p; }
 };

@@ -51,6 +51,11 @@ __device__ __forceinline__ int bcast_i(int v, int src)
     if (LPW == 64) return readlane_i(v, src);
     return __shfl(v, src, LPW);
 }
+template <int LPW>
+__device__ __forceinline__ uint32_t bcast(uint32_t v, int src)
+{
+    return (uint32_t)bcast_i<LPW>((int)v, src);
+}
 
 // u(r) from the spline table.  Periodic box: the table extends over every
 // possible minimum-image distance, entries past the cutoff hold the
@@ -91,6 +96,7 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
 {
     using C = Coord<T, PER>;
     using S = typename C::S;
+    using Inc = typename C::Inc;
     constexpr int WPB = kThreads / LPW;   // walkers per block
     const int N = sy.N, Ps = sy.P_s, K = sy.Kh;   // k-space: one representative per +-k pair
     const int NT = sweep_table_len(sy);
@@ -102,7 +108,7 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
     Vec4<T>* spl = reinterpret_cast<Vec4<T>*>(smem);                      // NT x (A0..A3) of Re u
     Vec4<T>* kvl = spl + NT;                                               // K x (n/L, |k|^2)
     T* all = reinterpret_cast<T*>(kvl + K);                                // K x Re alpha
-    S* posb = reinterpret_cast<S*>(all + ((K + 3) & ~3));                  // WPB x D x N (structure of arrays)
+    S* posb = reinterpret_cast<S*>(all + ((K + 3) & ~3));                  // WPB x N x D (particle-major)
     S* pos = posb + (size_t)grp * N * D;
 
     const T u_out = (T)tb.uR[sy.outer_idx];
@@ -125,25 +131,23 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
         all[i] = tb.alpha[2 * i];
     }
     // lengths in box units when periodic (scale = 1/L), unscaled otherwise;
-    // diff() returns multiples of C::kUnit of that unit
+    // diff() and raw() return multiples of C::kUnit of that unit
     const double scale = PER ? sy.Linv : 1.0;
     const bool active = w < a.W;
     double* Rw = a.R + (active ? w : 0) * (size_t)N * D;
     if (active) {
-        for (int i = gl; i < N; i += LPW) {
-#pragma unroll
-            for (int c = 0; c < D; c++) pos[c * N + i] = C::load(Rw[(size_t)i * D + c] * scale);
-        }
+        for (int e = gl; e < N * D; e += LPW) pos[e] = C::load(Rw[e] * scale);
     }
     __syncthreads();
     if (!active) return;
 
     const T hinv = (T)(sy.hinv / scale * C::kUnit), rc = (T)(sy.rc * scale / C::kUnit);
     const T step = (T)(sy.step * scale);
-    const T ob = sy.one_body ? (T)(tb.uR[0] / (scale * scale)) : (T)0;
-    const T kscale = (T)(1.0 / scale);   // phase in revolutions = (kv / scale) . s
+    // one-body factor on raw coordinates: -ob sum raw^2
+    const T ob = sy.one_body ? (T)(tb.uR[0] / (scale * scale) * C::kUnit * C::kUnit) : (T)0;
 
-    // this lane's k-vectors (n, in box units) and amplitudes, and their rho_k
+    // this lane's k-vectors (revolutions per raw coordinate unit) and
+    // amplitudes, and their rho_k
     T rhoR[KPL > 0 ? KPL : 1], rhoI[KPL > 0 ? KPL : 1];
     T qx[KPL > 0 ? KPL : 1], qy[KPL > 0 ? KPL : 1], qz[KPL > 0 ? KPL : 1], qa[KPL > 0 ? KPL : 1];
 #pragma unroll
@@ -152,25 +156,27 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
         const int k = gl + m * LPW;
         qx[m] = qy[m] = qz[m] = qa[m] = (T)0;   // lanes past K: zero amplitude, no effect
         if (k < K) {
-            Vec4<T> q = kvl[k];
-            q.x *= kscale;
-            q.y *= kscale;
-            q.z *= kscale;
-            qx[m] = q.x;
-            qy[m] = q.y;
-            qz[m] = q.z;
+            // periodic: kv L is the integer lattice vector n (rounded to
+            // remove the storage error of n / L)
+            const Vec4<T> q = kvl[k];
+            auto qs = [&](T v) { return (T)((PER ? rint((double)v / scale) : (double)v) * C::kUnit); };
+            qx[m] = qs(q.x);
+            qy[m] = qs(q.y);
+            qz[m] = qs(q.z);
             qa[m] = all[k];
             for (int n = 0; n < N; n++) {
-                T ph = q.x * C::unit_coord(pos[n]);
-                if (D > 1) ph += q.y * C::unit_coord(pos[N + n]);
-                if (D > 2) ph += q.z * C::unit_coord(pos[2 * N + n]);
+                T ph = qx[m] * C::raw(pos[n * D]);
+                if (D > 1) ph += qy[m] * C::raw(pos[n * D + 1]);
+                if (D > 2) ph += qz[m] * C::raw(pos[n * D + 2]);
                 T s, c;
-                Prec<T>::sincos_rev(ph, &s, &c);
+                Prec<T>::sincos_small(ph, &s, &c);
                 rhoR[m] += c;
                 rhoI[m] += s;
             }
         }
     }
+    // u at distance 0: the moved particle meets itself in the partner loop
+    const T u_zero = pair_u<T, PER>((T)0, rc, hinv, Ps, spl, u_out);
 
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     const uint32_t ep = a.epoch ? *a.epoch : 0u;
@@ -179,7 +185,8 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
     int seq = 0;   // sequential order: particle of trial base + gl
 
     for (int64_t base = 0; base < ntr; base += LPW) {
-        // random numbers of trial base + gl
+        // lane gl prepares trial base + gl: particle, displacement increments,
+        // log u, and the self term u(|move|) - u(0) that the partner loop adds
         const uint32_t tr = (uint32_t)(base + gl);
         Philox4 x = philox4x32_10(tr, ep, (uint32_t)w, 0u, k0, k1);
         int pm;
@@ -191,10 +198,11 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
             seq += LPW;
             while (seq >= N) seq -= N;
         }
-        T z[3];
-        T logu;
+        Inc dl[3];
+        T logu, uself;
         {
             // Box-Muller on (v1, v2) -> two normals; D = 3 draws a second block
+            T z[3];
             const float u1 = u01f(x.v[1]), u2 = u01f(x.v[2]);
             const float rr = sqrtf(-2.0f * __logf(u1));
             float sn, cs;
@@ -212,41 +220,46 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
                 z[2] = (T)0;
             }
             logu = (T)__logf(u01f(x.v[3]));
+            T s2 = (T)0;
+#pragma unroll
+            for (int c = 0; c < D; c++) {
+                dl[c] = C::inc(step * z[c]);
+                const T dm = C::diff((S)0, C::add((S)0, dl[c]));
+                s2 += dm * dm;
+            }
+            uself = pair_u<T, PER>(Prec<T>::sqrt_(s2), rc, hinv, Ps, spl, u_out) - u_zero;
         }
         const int nb = (int)((ntr - base) < LPW ? (ntr - base) : LPW);
         for (int tt = 0; tt < nb; tt++) {
             const int p = bcast_i<LPW>(pm, tt);
-            T dz[3];
-#pragma unroll
-            for (int c = 0; c < D; c++) dz[c] = bcast<LPW>(z[c], tt);
-            const T lu = bcast<LPW>(logu, tt);
-
             S old[D], nw[D];
 #pragma unroll
             for (int c = 0; c < D; c++) {
-                old[c] = pos[c * N + p];
-                nw[c] = C::move(old[c], step * dz[c]);
+                old[c] = pos[p * D + c];
+                nw[c] = C::add(old[c], bcast<LPW>(dl[c], tt));
             }
+            const T lu = bcast<LPW>(logu, tt);
+            const T us = bcast<LPW>(uself, tt);
             T delta = (T)0;
 #pragma unroll 2
             for (int i = gl; i < N; i += LPW) {
                 T ro2 = (T)0, rn2 = (T)0;
 #pragma unroll
                 for (int c = 0; c < D; c++) {
-                    const S pi = pos[c * N + i];
+                    const S pi = pos[i * D + c];
                     const T dOld = C::diff(pi, old[c]), dNew = C::diff(pi, nw[c]);
                     ro2 += dOld * dOld;
                     rn2 += dNew * dNew;
                 }
                 const T uo = pair_u<T, PER>(Prec<T>::sqrt_(ro2), rc, hinv, Ps, spl, u_out);
                 const T un = pair_u<T, PER>(Prec<T>::sqrt_(rn2), rc, hinv, Ps, spl, u_out);
-                delta += (i != p) ? un - uo : (T)0;
+                delta += un - uo;
             }
             T oc[D], ncd[D];
 #pragma unroll
             for (int c = 0; c < D; c++) {
-                oc[c] = C::unit_coord(old[c]);
-                ncd[c] = C::unit_coord(nw[c]);
+                oc[c] = C::raw(old[c]);
+                ncd[c] = C::raw(nw[c]);
             }
             T dR[KPL > 0 ? KPL : 1], dI[KPL > 0 ? KPL : 1];
 #pragma unroll
@@ -269,21 +282,23 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
                 // |rho + d|^2 - |rho|^2 = 2 Re(conj(rho) d) + |d|^2
                 delta += qa[m] * ((T)2 * (rhoR[m] * dR[m] + rhoI[m] * dI[m]) + dR[m] * dR[m] + dI[m] * dI[m]);
             }
-            if (gl == 0 && sy.one_body) {
-                T ro2 = (T)0, rn2 = (T)0;
+            if (sy.one_body) {
+                if (gl == 0) {
+                    T ro2 = (T)0, rn2 = (T)0;
 #pragma unroll
-                for (int c = 0; c < D; c++) {
-                    ro2 += oc[c] * oc[c];
-                    rn2 += ncd[c] * ncd[c];
+                    for (int c = 0; c < D; c++) {
+                        ro2 += oc[c] * oc[c];
+                        rn2 += ncd[c] * ncd[c];
+                    }
+                    delta += -ob * (rn2 - ro2);
                 }
-                delta += -ob * (rn2 - ro2);
             }
-            delta = group_sum<LPW>(delta);
+            delta = group_sum<LPW>(delta) - us;
             const bool accept = ((T)2 * delta >= lu);
             if (accept) {
                 if (gl == 0) {
 #pragma unroll
-                    for (int c = 0; c < D; c++) pos[c * N + p] = nw[c];
+                    for (int c = 0; c < D; c++) pos[p * D + c] = nw[c];
                 }
 #pragma unroll
                 for (int m = 0; m < KPL; m++) {
@@ -299,10 +314,7 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
         }
     }
 
-    for (int i = gl; i < N; i += LPW) {
-#pragma unroll
-        for (int c = 0; c < D; c++) Rw[(size_t)i * D + c] = C::store(pos[c * N + i]) * (PER ? sy.L : 1.0);
-    }
+    for (int e = gl; e < N * D; e += LPW) Rw[e] = C::store(pos[e]) * (PER ? sy.L : 1.0);
     if (a.rho_out) {
         T* ro = static_cast<T*>(a.rho_out) + (size_t)w * K * 2;
 #pragma unroll
