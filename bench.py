@@ -1,6 +1,7 @@
 """Throughput of the TDVMC hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+    python bench.py --drift STEPS [--walkers W]     (real-time quench, E(t) in both forms)
 
 Workload (default): 3-D bulk, N = 343 He-4-like bosons (Lennard-Jones,
 density 0.365 sigma^-3), Jastrow = 40-parameter pair spline + 256 k-space terms
@@ -90,6 +91,62 @@ def run_gpu(cfg, steps, warmup, precision, use_graph=True):
     ctx.close()
     return dict(elapsed=elapsed, samples=samples, steps=steps, W=W, prof=prof,
                 acceptance=acc / max(trials, 1), finite=ok, info=info, lanes=lanes)
+
+
+def run_drift(cfg, steps, therm_sweeps=100):
+    """Real-time TDVP after the quench: the same start (walkers thermalised
+    with `therm_sweeps` sweeps of N moves from the lattice start, the
+    configured parameters) integrated `steps` RK4 steps in the FP64 form and
+    in the mixed form.  <E_L>(t) comes from a device measurement of the
+    current walkers after every step (not timed)."""
+    import tdvmc_amd as t
+
+    with t.Context(cfg) as c0:
+        info = c0.get_info()
+        for i in range(therm_sweeps):
+            c0.sweep(info.N, seed=5 + i, precision=t.MIXED)
+        R0 = c0.get_walkers()
+        er = c0.eval_configs(R0, precision=t.FP64)["ER"]
+        print(f"drift: start after {therm_sweeps} sweeps, <E_L> = {er.mean():.6f}", file=sys.stderr, flush=True)
+    W = R0.shape[0]
+    sigma = float(er.std() / np.sqrt(W))
+    dt = info.timestep
+    out = {}
+    for name, prec in (("fp64", t.FP64), ("mixed", t.MIXED)):
+        with t.Context(cfg) as ctx:
+            ctx.set_walkers(R0)
+            E = [ctx.measure(prec)["localEnergyR"]]
+            spent = 0.0
+            for s in range(steps):
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                ctx.tdvp_step(dt, ode_type=2, precision=prec, use_graph=True)
+                ctx.synchronize()
+                spent += time.perf_counter() - t0
+                E.append(ctx.measure(prec)["localEnergyR"])
+                if (s + 1) % 20 == 0:
+                    print(f"drift {name}: step {s + 1}/{steps}, E = {E[-1]:.6f}, {spent:.1f} s", file=sys.stderr,
+                          flush=True)
+            uR = ctx.get_params()[0]
+            E = np.array(E)
+            out[name] = dict(E=E, steps_per_s=steps / spent, walker_samples_per_s=4 * steps * W / spent,
+                             finite=bool(np.all(np.isfinite(uR)) and np.all(np.isfinite(E))))
+    d = out["mixed"]["E"] - out["fp64"]["E"]
+    res = {
+        "mode": "drift", "workload": None, "walkers": W, "N": info.N, "DIM": info.DIM, "P": info.P,
+        "K": info.K, "steps": steps, "dt": dt, "ode": "RK4, real time", "therm_sweeps": therm_sweeps,
+        "sigma_E": sigma,
+        "max_abs_mixed_minus_fp64": float(np.max(np.abs(d))),
+        "max_abs_mixed_minus_fp64_over_sigma": float(np.max(np.abs(d)) / sigma),
+    }
+    for name in ("fp64", "mixed"):
+        E = out[name]["E"]
+        res[name] = {"E0": float(E[0]), "E_end": float(E[-1]), "drift": float(E[-1] - E[0]),
+                     "max_abs_dev_from_E0": float(np.max(np.abs(E - E[0]))),
+                     "tdvp_steps_per_s": out[name]["steps_per_s"],
+                     "walker_samples_per_s": out[name]["walker_samples_per_s"], "finite": out[name]["finite"],
+                     "E_t": [float(x) for x in E]}
+    return res
 
 
 def copy_probe(mib):
@@ -191,6 +248,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched trap workload")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise kernel of this size (counter calibration)")
+    ap.add_argument("--drift", type=int, default=0, metavar="STEPS",
+                    help="only run the real-time quench workload for STEPS RK4 steps in both forms "
+                         "and report <E_L>(t) and its drift")
+    ap.add_argument("--drift-therm", type=int, default=100, metavar="SWEEPS",
+                    help="thermalisation sweeps before the drift run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU path is design-only "
@@ -201,6 +263,14 @@ def main():
         return
 
     from tdvmc_amd.workloads import WORKLOADS
+
+    if args.drift:
+        name = args.workload if args.workload != "bulk3d_lj343_k256" else "bulk2d_gauss400_k512"
+        kw = {"W": args.walkers} if args.walkers else {}
+        res = run_drift(WORKLOADS[name](**kw), args.drift, args.drift_therm)
+        res["workload"] = name
+        print(json.dumps(res))
+        return
 
     make = WORKLOADS[args.workload]
     kw = {"W": args.walkers} if args.walkers else {}
