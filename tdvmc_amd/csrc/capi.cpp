@@ -892,8 +892,10 @@ void release(tdvmc_ctx* ctx)
 // ---------------------------------------------------------------------------
 // Batches of independent systems (each a full context for its own buffers and
 // tables) sampled, measured and advanced together: one sweep launch and one
-// measurement launch for all systems, one moments and one solve launch with a
-// workgroup per system, the RK arithmetic on the concatenated parameters.
+// measurement launch for all systems, the moments in two launches (chunks of
+// walkers of every system, then a reduction per system), one solve launch
+// with a workgroup per system, the RK arithmetic on the concatenated
+// parameters.
 struct tdvmc_batch {
     std::vector<tdvmc_ctx*> sys;
     std::string err;
@@ -914,6 +916,9 @@ struct tdvmc_batch {
     EvalBatchItem<float>* dEvalF = nullptr;
     EvalBatchItem<double>* dEvalD = nullptr;
     MomentsItem* dMom = nullptr;
+    double* dMomPart = nullptr;       // partial sums of the batched moments
+    int mom_nchunk = 0;
+    size_t mom_lds = 0;
     SolveArgs* dSolve[4] = {nullptr, nullptr, nullptr, nullptr};
     double* dX = nullptr;
     double* dXtmp = nullptr;
@@ -978,7 +983,7 @@ int batch_upload(tdvmc_batch* b, int64_t n_moves, uint64_t seed, bool want_rho)
     std::vector<EvalBatchItem<double>> ed(n);
     std::vector<MomentsItem> mi(n);
     std::vector<SolveArgs> sv[4];
-    int64_t sb = 0, eb = 0;
+    int64_t sb = 0, eb = 0, part_len = 0;
     size_t slds = 0, elf = 0, eld = 0;
     for (int s = 0; s < n; s++) {
         tdvmc_ctx* c = b->sys[s];
@@ -1010,7 +1015,13 @@ int batch_upload(tdvmc_batch* b, int64_t n_moves, uint64_t seed, bool want_rho)
                                      rho ? reinterpret_cast<const float*>(c->dRho) : nullptr};
         ed[s] = EvalBatchItem<double>{c->dR, c->W, c->dExpo, c->dER, c->dEI, c->dO64, c->dOther, nullptr, c->phiR,
                                       rho ? c->dRho : nullptr};
-        mi[s] = MomentsItem{c->sy.P, c->W, c->dO64, c->dER, c->dEI, c->dMean, c->dS, c->dF};
+        {
+            const int cw = moments_chunk_walkers(c->sy.P);
+            const int nch = (int)((c->W + cw - 1) / cw);
+            mi[s] = MomentsItem{c->sy.P, c->W, c->dO64, c->dER, c->dEI, c->dMean, c->dS, c->dF,
+                                reinterpret_cast<double*>((intptr_t)part_len), cw, nch};
+            part_len += (int64_t)nch * moments_part_len(c->sy.P);
+        }
         for (int k = 0; k < 4; k++) {
             SolveArgs v;
             v.P = c->sy.P;
@@ -1026,6 +1037,15 @@ int batch_upload(tdvmc_batch* b, int64_t n_moves, uint64_t seed, bool want_rho)
             v.mean_sub = c->solver_type == 1;
             sv[k].push_back(v);
         }
+    }
+    // partial-sum scratch of the moments: sized by the systems, allocated once
+    if (!b->dMomPart) BHIPC(dalloc(&b->dMomPart, (size_t)part_len));
+    b->mom_nchunk = 0;
+    b->mom_lds = 0;
+    for (auto& m : mi) {
+        m.part = b->dMomPart + (intptr_t)m.part;
+        b->mom_nchunk = std::max(b->mom_nchunk, m.nchunk);
+        b->mom_lds = std::max(b->mom_lds, (size_t)(m.P + 2) * (1 + (size_t)m.cw) * sizeof(double));
     }
     b->sweep_blocks = sb;
     b->eval_blocks = eb;
@@ -1074,7 +1094,7 @@ int batch_measure_launch(tdvmc_batch* b, int prec)
         EvalBatch<double> bt{n, b->dEvalStart, b->dSys, b->dTabD, b->dEvalD};
         BHIPC(launch_eval_batch<double>(bt, b->D, b->periodic, b->eval_blocks, b->eval_lds_d, b->stream));
     }
-    BHIPC(launch_moments_batch(b->dMom, n, b->P_max, b->stream));
+    BHIPC(launch_moments_batch(b->dMom, n, b->mom_nchunk, b->mom_lds, b->P_max, b->stream));
     return TDVMC_OK;
 }
 
@@ -1152,6 +1172,7 @@ void batch_release(tdvmc_batch* b)
     dfree(b->dEvalF);
     dfree(b->dEvalD);
     dfree(b->dMom);
+    dfree(b->dMomPart);
     for (int k = 0; k < 4; k++) {
         dfree(b->dSolve[k]);
         dfree(b->dK[k]);

@@ -144,8 +144,17 @@ struct MomentsItem {
     double* mean;              // [P+2]
     double* S;                 // [P][P]
     double* F;                 // [2][P]
+    double* part;              // [nchunk][moments_part_len(P)] partial sums (scratch)
+    int cw;                    // walkers per chunk
+    int nchunk;
 };
-hipError_t launch_moments_batch(const MomentsItem* items_dev, int n, int P_max, hipStream_t st);
+// walkers per chunk of the batched moments (the chunk's [O | E_R | E_I] rows fit 48 KB of LDS)
+int moments_chunk_walkers(int P);
+// doubles of one chunk's partial sums: P + 2 column sums, P (P + 1) / 2 + 2 P products
+__host__ __device__ inline int64_t moments_part_len(int P) { return (int64_t)(P + 2) + (int64_t)P * (P + 1) / 2 + 2 * (int64_t)P; }
+// lds: largest (P + 2) (1 + cw) * 8 over the systems
+hipError_t launch_moments_batch(const MomentsItem* items_dev, int n, int nchunk_max, size_t lds, int P_max,
+                                hipStream_t st);
 
 // after a step: alpha -= alpha[P-1] (adjust), phi R = -exponent (normalize)
 hipError_t launch_post_step(int P, double* x, const double* expo, int normalize, int adjust, hipStream_t st);

@@ -115,74 +115,98 @@ __global__ void gather_batch_kernel(const BatchParams<T>* __restrict__ bp, doubl
     }
 }
 
-// FP64 moments of one system per workgroup (two passes: means, then the
-// centred products), walkers staged through LDS in chunks of kMomChunk.
-constexpr int kMomChunk = 32;
-__global__ __launch_bounds__(256) void moments_batch_kernel(const MomentsItem* __restrict__ items)
+// FP64 moments of a batch of systems in two launches.  Walkers are shifted
+// by the values of the system's walker 0 (x = v - v_0: the sums of x and x x^T
+// carry no large common offset), split into chunks of cw walkers, and every
+// (chunk, system) workgroup writes its partial sums; a second launch adds
+// the chunks of each system in a fixed order and centres:
+// cov(a, b) = sum(x_a x_b) / W - mean(x_a) mean(x_b).
+__device__ __forceinline__ double moments_col(const MomentsItem& it, int64_t w, int c)
+{
+    const int P = it.P;
+    return c < P ? it.O[w * P + c] : (c == P ? it.ER[w] : it.EI[w]);
+}
+
+// entry e of the product list: S upper triangle (i <= j < P), then F (O_i
+// with E_R, O_i with E_I) as columns P and P + 1
+__device__ __forceinline__ void moments_entry(int P, int e, int* i, int* j)
+{
+    const int nS = P * (P + 1) / 2;
+    if (e < nS) {
+        int a = 0, r = e;
+        while (r >= P - a) {
+            r -= P - a;
+            a++;
+        }
+        *i = a;
+        *j = a + r;
+    } else {
+        *i = (e - nS) % P;
+        *j = P + (e - nS) / P;
+    }
+}
+
+__global__ __launch_bounds__(256) void moments_partial_kernel(const MomentsItem* __restrict__ items)
+{
+    const MomentsItem it = items[blockIdx.y];
+    const int chunk = blockIdx.x;
+    if ((int64_t)chunk * it.cw >= it.W) return;
+    const int P = it.P, C = P + 2, tid = threadIdx.x;
+    const int64_t w0 = (int64_t)chunk * it.cw;
+    const int nw = (int)((it.W - w0) < it.cw ? (it.W - w0) : it.cw);
+    extern __shared__ __align__(16) unsigned char smem[];
+    double* sh = reinterpret_cast<double*>(smem);   // [C] shift
+    double* X = sh + C;                              // [cw][C] shifted values
+    for (int c = tid; c < C; c += blockDim.x) sh[c] = moments_col(it, 0, c);
+    __syncthreads();
+    for (int e = tid; e < nw * C; e += blockDim.x) {
+        const int ww = e / C, c = e - ww * C;
+        X[e] = moments_col(it, w0 + ww, c) - sh[c];
+    }
+    __syncthreads();
+    const int64_t len = moments_part_len(P);
+    double* out = it.part + (int64_t)chunk * len;
+    for (int64_t e = tid; e < len; e += blockDim.x) {
+        double acc = 0.0;
+        if (e < C) {
+            for (int ww = 0; ww < nw; ww++) acc += X[ww * C + e];
+        } else {
+            int i, j;
+            moments_entry(P, (int)(e - C), &i, &j);
+            for (int ww = 0; ww < nw; ww++) acc += X[ww * C + i] * X[ww * C + j];
+        }
+        out[e] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void moments_reduce_kernel(const MomentsItem* __restrict__ items)
 {
     const MomentsItem it = items[blockIdx.x];
     const int P = it.P, C = P + 2, tid = threadIdx.x;
-    const int64_t W = it.W;
+    const int64_t len = moments_part_len(P);
+    const double invW = 1.0 / (double)it.W;
     extern __shared__ __align__(16) unsigned char smem[];
-    double* mean = reinterpret_cast<double*>(smem);      // [C]
-    double* X = mean + C;                                  // [kMomChunk][C] centred values
-    auto col = [&](int64_t w, int c) -> double {
-        return c < P ? it.O[w * P + c] : (c == P ? it.ER[w] : it.EI[w]);
-    };
+    double* m = reinterpret_cast<double*>(smem);   // [C] means of the shifted values
     for (int c = tid; c < C; c += blockDim.x) {
-        double sum = 0.0;
-        for (int64_t w = 0; w < W; w++) sum += col(w, c);
-        mean[c] = sum / (double)W;
+        double s = 0.0;
+        for (int k = 0; k < it.nchunk; k++) s += it.part[(int64_t)k * len + c];
+        m[c] = s * invW;
+        it.mean[c] = moments_col(it, 0, c) + m[c];
     }
     __syncthreads();
-    // entries: S upper triangle (i <= j < P), then F (O_i with E_R, E_I)
-    const int nS = P * (P + 1) / 2, nE = nS + 2 * P;
-    for (int e0 = 0; e0 < nE; e0 += 4 * (int)blockDim.x) {
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        int ci[4], cj[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int e = e0 + q * blockDim.x + tid;
-            ci[q] = cj[q] = -1;
-            if (e < nS) {
-                int i = 0, r = e;
-                while (r >= P - i) {
-                    r -= P - i;
-                    i++;
-                }
-                ci[q] = i;
-                cj[q] = i + r;
-            } else if (e < nE) {
-                ci[q] = (e - nS) % P;
-                cj[q] = P + (e - nS) / P;
-            }
-        }
-        for (int64_t w0 = 0; w0 < W; w0 += kMomChunk) {
-            const int nw = (int)((W - w0) < kMomChunk ? (W - w0) : kMomChunk);
-            __syncthreads();
-            for (int e = tid; e < nw * C; e += blockDim.x) {
-                const int ww = e / C, c = e % C;
-                X[ww * C + c] = col(w0 + ww, c) - mean[c];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (ci[q] >= 0)
-                    for (int ww = 0; ww < nw; ww++) acc[q] += X[ww * C + ci[q]] * X[ww * C + cj[q]];
-        }
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            if (ci[q] < 0) continue;
-            const double v = acc[q] / (double)W;
-            if (cj[q] < P) {
-                it.S[(size_t)ci[q] * P + cj[q]] = v;
-                it.S[(size_t)cj[q] * P + ci[q]] = v;
-            } else {
-                it.F[(size_t)(cj[q] - P) * P + ci[q]] = v;
-            }
+    for (int64_t e = C + tid; e < len; e += blockDim.x) {
+        double s = 0.0;
+        for (int k = 0; k < it.nchunk; k++) s += it.part[(int64_t)k * len + e];
+        int i, j;
+        moments_entry(P, (int)(e - C), &i, &j);
+        const double v = s * invW - m[i] * m[j];
+        if (j < P) {
+            it.S[(size_t)i * P + j] = v;
+            it.S[(size_t)j * P + i] = v;
+        } else {
+            it.F[(size_t)(j - P) * P + i] = v;
         }
     }
-    for (int c = tid; c < C; c += blockDim.x) it.mean[c] = mean[c];
 }
 
 __global__ void axpy_params_kernel(int n, const double* x0, const double* xdot, double c, double* y)
@@ -342,16 +366,27 @@ template hipError_t launch_prep_tables_batch<double>(const BatchParams<double>*,
 template hipError_t launch_unpack_batch<double>(const BatchParams<double>*, int, const double*, hipStream_t);
 template hipError_t launch_gather_batch<double>(const BatchParams<double>*, int, double*, int, hipStream_t);
 
-hipError_t launch_moments_batch(const MomentsItem* items_dev, int n, int P_max, hipStream_t st)
+int moments_chunk_walkers(int P)
 {
-    if (n <= 0) return hipSuccess;
-    const size_t lds = (size_t)(P_max + 2) * (1 + kMomChunk) * sizeof(double);
+    int cw = 49152 / (8 * (P + 2)) - 1;
+    cw = cw > 256 ? 256 : cw;
+    return cw < 1 ? 1 : cw;
+}
+
+hipError_t launch_moments_batch(const MomentsItem* items_dev, int n, int nchunk_max, size_t lds, int P_max,
+                                hipStream_t st)
+{
+    if (n <= 0 || nchunk_max <= 0) return hipSuccess;
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void*)moments_batch_kernel,
+        hipError_t e = hipFuncSetAttribute((const void*)moments_partial_kernel,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(moments_batch_kernel, dim3(n), dim3(256), lds, st, items_dev);
+    hipLaunchKernelGGL(moments_partial_kernel, dim3((unsigned)nchunk_max, (unsigned)n), dim3(256), lds, st, items_dev);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(moments_reduce_kernel, dim3((unsigned)n), dim3(256), (size_t)(P_max + 2) * sizeof(double), st,
+                       items_dev);
     return hipGetLastError();
 }
 
