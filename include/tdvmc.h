@@ -227,6 +227,11 @@ int tdvmc_batch_size(tdvmc_batch* b, int* n_systems);
 tdvmc_ctx* tdvmc_batch_system(tdvmc_batch* b, int s);
 /* lanes per walker of the batched sweep: 16, 32, 64, or 0 for the default */
 int tdvmc_batch_set_sweep_lanes(tdvmc_batch* b, int lanes);
+/* Times each system's measurement with 1, 2 and 4 wavefronts per walker (4,
+ * 2 or 1 walkers per workgroup) and keeps the fastest per system;
+ * waves_out [n_systems] (may be NULL) receives the choice.  No counterpart in
+ * the reference (the reference evaluates one walker at a time). */
+int tdvmc_batch_autotune(tdvmc_batch* b, int precision, int* waves_out);
 /* n_moves <= 0: each system's MC_NTHERMSTEPS; system s draws from seed
  * mixed with s (walker w of system s follows the same random stream as walker
  * w of a single context seeded with that mix). */

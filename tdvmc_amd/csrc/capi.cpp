@@ -902,6 +902,8 @@ struct tdvmc_batch {
     hipStream_t stream = nullptr;
     int D = 0, periodic = 0, kh_max = 0, P_max = 0, N_max = 0;
     int lanes = 0;                    // lanes per walker of the batched sweep (0: default)
+    std::vector<int> eval_wpw;        // per system: wavefronts per walker of the measurement (0: default)
+    int tune_gen = 0;                 // changes whenever eval_wpw does
     std::vector<int64_t> xoff;
     int64_t xtotal = 0;
     uint64_t seed = 1;
@@ -961,13 +963,24 @@ int batch_lanes(const tdvmc_batch* b)
 
 uint64_t batch_system_seed(uint64_t seed, int s) { return seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(s + 1)); }
 
+// wavefronts per walker of system s's measurement: tuned or default, raised
+// until the walkers of one workgroup fit the LDS
+int batch_eval_wpw(const tdvmc_batch* b, int s)
+{
+    const tdvmc_ctx* c = b->sys[s];
+    int wpw = (s < (int)b->eval_wpw.size() && b->eval_wpw[s]) ? b->eval_wpw[s] : eval_default_wpw(c->sy.N);
+    const size_t slot = std::max(eval_lds_bytes<float>(c->sy, false), eval_lds_bytes<double>(c->sy, false));
+    while (wpw < 4 && (size_t)(4 / wpw) * slot > 160 * 1024) wpw *= 2;
+    return wpw;
+}
+
 // Descriptor arrays of a batched sweep with n_moves per system (<= 0: each
 // system's MC_NTHERMSTEPS) and of the measurement, moments and solves.
 int batch_upload(tdvmc_batch* b, int64_t n_moves, uint64_t seed, bool want_rho)
 {
-    char key[96];
-    std::snprintf(key, sizeof(key), "%lld/%llu/%d/%d", (long long)n_moves, (unsigned long long)seed, want_rho ? 1 : 0,
-                  b->lanes);
+    char key[128];
+    std::snprintf(key, sizeof(key), "%lld/%llu/%d/%d/%d", (long long)n_moves, (unsigned long long)seed,
+                  want_rho ? 1 : 0, b->lanes, b->tune_gen);
     if (b->desc_key == key) return TDVMC_OK;
     const int n = (int)b->sys.size();
     const int lanes = batch_lanes(b);
@@ -996,11 +1009,13 @@ int batch_upload(tdvmc_batch* b, int64_t n_moves, uint64_t seed, bool want_rho)
                                      b->xoff[s]};
         sstart[s] = sb;
         sb += (c->W + wpb - 1) / wpb;
+        const int wpw = batch_eval_wpw(b, s);
+        const size_t slot_f = eval_lds_bytes<float>(c->sy, false), slot_d = eval_lds_bytes<double>(c->sy, false);
         estart[s] = eb;
-        eb += c->W;
+        eb += (c->W + 4 / wpw - 1) / (4 / wpw);
         slds = std::max(slds, std::max(sweep_lds_bytes<float>(c->sy, lanes), sweep_lds_bytes<double>(c->sy, lanes)));
-        elf = std::max(elf, eval_lds_bytes<float>(c->sy, false));
-        eld = std::max(eld, eval_lds_bytes<double>(c->sy, false));
+        elf = std::max(elf, (size_t)(4 / wpw) * slot_f);
+        eld = std::max(eld, (size_t)(4 / wpw) * slot_d);
         SweepArgs& a = sa[s];
         a.R = c->dR;
         a.W = c->W;
@@ -1012,9 +1027,9 @@ int batch_upload(tdvmc_batch* b, int64_t n_moves, uint64_t seed, bool want_rho)
         a.rho_out = (want_rho && c->sy.Kh > 0) ? static_cast<void*>(c->dRho) : nullptr;
         const bool rho = want_rho && c->sy.Kh > 0;
         ef[s] = EvalBatchItem<float>{c->dR, c->W, c->dExpo, c->dER, c->dEI, c->dO64, c->dOther, nullptr, c->phiR,
-                                     rho ? reinterpret_cast<const float*>(c->dRho) : nullptr};
+                                     rho ? reinterpret_cast<const float*>(c->dRho) : nullptr, wpw, (int64_t)slot_f};
         ed[s] = EvalBatchItem<double>{c->dR, c->W, c->dExpo, c->dER, c->dEI, c->dO64, c->dOther, nullptr, c->phiR,
-                                      rho ? c->dRho : nullptr};
+                                      rho ? c->dRho : nullptr, wpw, (int64_t)slot_d};
         {
             const int cw = moments_chunk_walkers(c->sy.P);
             const int nch = (int)((c->W + cw - 1) / cw);
@@ -1184,6 +1199,67 @@ void batch_release(tdvmc_batch* b)
     b->sys.clear();
     if (b->stream) hipStreamDestroy(b->stream);
     b->stream = nullptr;
+}
+
+}  // namespace
+
+namespace {
+
+// Average time of system s's measurement alone with `wpw` wavefronts per
+// walker: a one-system launch whose descriptors point into the batch's
+// arrays, except for the item (a copy with the candidate shape).
+template <typename T>
+int batch_time_eval(tdvmc_batch* b, int s, int wpw, const EvalBatchItem<T>* items, const KTables<T>* tabs,
+                    float* ms)
+{
+    tdvmc_ctx* c = b->sys[s];
+    const size_t slot = eval_lds_bytes<T>(c->sy, false);
+    EvalBatchItem<T> it;
+    BHIPC(hipMemcpy(&it, items + s, sizeof(it), hipMemcpyDeviceToHost));
+    it.wpw = wpw;
+    it.slot_bytes = (int64_t)slot;
+    EvalBatchItem<T>* dIt = nullptr;
+    int64_t* dStart = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = TDVMC_OK;
+    auto fail_hip = [&](hipError_t e) {
+        b->err = std::string("HIP error: ") + hipGetErrorString(e);
+        rc = TDVMC_ERR_HIP;
+    };
+    const int64_t zero = 0;
+    hipError_t e;
+    if ((e = dalloc(&dIt, 1)) != hipSuccess || (e = dalloc(&dStart, 1)) != hipSuccess ||
+        (e = hipMemcpy(dIt, &it, sizeof(it), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(dStart, &zero, sizeof(zero), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
+        fail_hip(e);
+    } else {
+        const EvalBatch<T> bt{1, dStart, b->dSys + s, tabs + s, dIt};
+        const int64_t blocks = (c->W + 4 / wpw - 1) / (4 / wpw);
+        const size_t lds = (size_t)(4 / wpw) * slot;
+        constexpr int kReps = 3;
+        if ((e = launch_eval_batch<T>(bt, b->D, b->periodic != 0, blocks, lds, b->stream)) != hipSuccess ||
+            (e = hipEventRecord(e0, b->stream)) != hipSuccess) {
+            fail_hip(e);
+        } else {
+            for (int r = 0; r < kReps && rc == TDVMC_OK; r++)
+                if ((e = launch_eval_batch<T>(bt, b->D, b->periodic != 0, blocks, lds, b->stream)) != hipSuccess)
+                    fail_hip(e);
+            float t = 0.0f;
+            if (rc == TDVMC_OK) {
+                if ((e = hipEventRecord(e1, b->stream)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess ||
+                    (e = hipEventElapsedTime(&t, e0, e1)) != hipSuccess)
+                    fail_hip(e);
+                *ms = t / kReps;
+            }
+        }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipStreamSynchronize(b->stream);
+    dfree(dIt);
+    dfree(dStart);
+    return rc;
 }
 
 }  // namespace
@@ -2028,6 +2104,43 @@ int tdvmc_batch_set_sweep_lanes(tdvmc_batch* b, int lanes)
     }
     b->lanes = lanes;
     batch_drop_graph(b);
+    return TDVMC_OK;
+}
+
+int tdvmc_batch_autotune(tdvmc_batch* b, int precision, int* waves_out)
+{
+    if (!b) return TDVMC_ERR_BAD_ARG;
+    int prec, rc = batch_precision(b, precision, &prec);
+    if (rc) return rc;
+    if ((rc = batch_sync_members(b))) return rc;
+    if ((rc = batch_upload(b, 0, b->seed, true))) return rc;
+    if ((rc = batch_prep(b, prec))) return rc;
+    const int n = (int)b->sys.size();
+    std::vector<int> best(n, 4);
+    for (int s = 0; s < n; s++) {
+        const tdvmc_ctx* c = b->sys[s];
+        const size_t slot = std::max(eval_lds_bytes<float>(c->sy, false), eval_lds_bytes<double>(c->sy, false));
+        float tb = 0.0f;
+        bool have = false;
+        for (int wpw : {1, 2, 4}) {
+            if ((size_t)(4 / wpw) * slot > 160 * 1024) continue;
+            float t = 0.0f;
+            rc = prec == TDVMC_MIXED ? batch_time_eval<float>(b, s, wpw, b->dEvalF, b->dTabF, &t)
+                                     : batch_time_eval<double>(b, s, wpw, b->dEvalD, b->dTabD, &t);
+            if (rc) return rc;
+            if (!have || t < tb) {
+                tb = t;
+                best[s] = wpw;
+                have = true;
+            }
+        }
+    }
+    b->eval_wpw = best;
+    b->tune_gen++;
+    batch_drop_graph(b);
+    if ((rc = batch_upload(b, 0, b->seed, true))) return rc;
+    if (waves_out)
+        for (int s = 0; s < n; s++) waves_out[s] = batch_eval_wpw(b, s);
     return TDVMC_OK;
 }
 

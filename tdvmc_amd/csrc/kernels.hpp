@@ -53,8 +53,8 @@ hipError_t launch_sweep_batch(const SweepBatch<T>& bt, int D, bool periodic, int
                               size_t lds, hipStream_t st);
 int sweep_walkers_per_block(int lanes);
 
-// Measurement of a batch of systems in one launch (one workgroup per walker);
-// O is written in FP64, no per-particle outputs.
+// Measurement of a batch of systems in one launch (1, 2 or 4 wavefronts per
+// walker, chosen per system); O is written in FP64, no per-particle outputs.
 template <typename T>
 struct EvalBatchItem {
     const double* R;           // [W][N][D]
@@ -67,7 +67,11 @@ struct EvalBatchItem {
     double* outer;             // [W] or null
     double phiR;
     const T* rho;              // rho_k handed over by the sweep, or null
+    int wpw;                   // wavefronts per walker: 1, 2 or 4 (4 / wpw walkers per workgroup)
+    int64_t slot_bytes;        // LDS per walker (eval_lds_bytes)
 };
+// wavefronts per walker of the measurement without tuning
+int eval_default_wpw(int N);
 template <typename T>
 struct EvalBatch {
     int n;
@@ -151,7 +155,10 @@ struct MomentsItem {
 // walkers per chunk of the batched moments (the chunk's [O | E_R | E_I] rows fit 48 KB of LDS)
 int moments_chunk_walkers(int P);
 // doubles of one chunk's partial sums: P + 2 column sums, P (P + 1) / 2 + 2 P products
-__host__ __device__ inline int64_t moments_part_len(int P) { return (int64_t)(P + 2) + (int64_t)P * (P + 1) / 2 + 2 * (int64_t)P; }
+__host__ __device__ inline int64_t moments_part_len(int P)
+{
+    return (int64_t)(P + 2) + (int64_t)P * (P + 1) / 2 + 2 * (int64_t)P;
+}
 // lds: largest (P + 2) (1 + cw) * 8 over the systems
 hipError_t launch_moments_batch(const MomentsItem* items_dev, int n, int nchunk_max, size_t lds, int P_max,
                                 hipStream_t st);

@@ -13,9 +13,9 @@
 //              values the sweep handed over;
 //   pairs    : every unordered pair once, on a half ring: lane = row n of a
 //              64-row block, partner i = (n + s) mod N, the s range split over
-//              the 4 waves; own-row sums stay in registers, partner sums ride
+//              the walker's waves; own-row sums stay in registers, partner sums ride
 //              a DPP register ring and reach LDS once per particle and step;
-//   k terms  : lane = row n, the k range split over the 4 waves;
+//   k terms  : lane = row n, the k range split over the walker's waves;
 //   E_L      : thread = particle, FP64 block reduction.
 // The pair's position t in its knot interval goes to an LDS histogram of the
 // moments 1, t, t^2, t^3 per interval ([interval][4][8], 8 lanes per column);
@@ -35,7 +35,6 @@ namespace tdvmc {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
 constexpr int kHistCols = 8;    // histogram columns (LDS footprint vs bank conflicts: occupancy wins)
 
 template <typename TO>
@@ -79,19 +78,32 @@ __device__ __forceinline__ double wave_rotate(double v)
     return *reinterpret_cast<double*>(&y);
 }
 
-__device__ __forceinline__ double block_sum(double v, double* red)
+// Sums of NV values over the WPW wavefronts of one walker: wave reductions,
+// then (WPW > 1) across the walker's waves through LDS.  With WPW = 1 no
+// barrier is needed.
+template <int WPW, int NV>
+__device__ __forceinline__ void walker_sum(double (&v)[NV], double* red, int wave, int lane)
 {
-    // wave reduction, then across the 4 waves through LDS
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) red[wv] = v;
-    __syncthreads();
-    double s = 0.0;
 #pragma unroll
-    for (int w = 0; w < kWaves; w++) s += red[w];
-    return s;
+    for (int i = 0; i < NV; i++)
+        for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_xor(v[i], o, 64);
+    if constexpr (WPW > 1) {
+        __syncthreads();
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < NV; i++) red[wave * NV + i] = v[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NV; i++) {
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < WPW; w++) s += red[w * NV + i];
+            v[i] = s;
+        }
+    }
 }
+constexpr int kRedDoubles = 40;   // >= 4 waves x 9 values
 
 // Pair potential of one pair in coordinate-difference units (constants from
 // PairConsts); POT: 1 square well, 2 Gaussian, 3 Lennard-Jones.
@@ -132,7 +144,7 @@ __device__ __forceinline__ T pair_pot(const PairConsts<T>& pc, T r, T r2, T rinv
 // here: their number is N (N - 1) / 2 minus the histogram count.  CPLX false:
 // every imaginary parameter is zero, so the imaginary parts (exact zeros) are
 // not computed.
-template <typename T, int D, bool PP, bool PER, int POT, bool CPLX>
+template <typename T, int D, bool PP, bool PER, int POT, bool CPLX, int NW>
 __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<T, PER>::S* __restrict__ pos,
                                            const T* __restrict__ dsp, double* __restrict__ hist,
                                            double* __restrict__ gsum, double* __restrict__ lsum, int wave,
@@ -171,13 +183,13 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
     // s = 0..31: 32 consecutive slots hold every residue mod 32 once.  Row
     // sums stay in registers, column sums rotate with the column coordinates;
     // both reach LDS once per tile segment.  The steps of all tiles are split
-    // evenly over the 4 waves.  Off-diagonal tiles of full blocks run without
+    // evenly over the walker's NW waves.  Off-diagonal tiles of full blocks run without
     // per-step validity tests.
     const int nfull = N / 64, rem = N - 64 * nfull;
     const int nblk = nfull + (rem > 0 ? 1 : 0);
     const bool fold = rem > 0 && rem <= 32 && nfull > 0;
     const int total = nfull * (nfull - 1) / 2 * 64 + nblk * 32 + (rem > 0 ? nfull * (fold ? 32 : 64) : 0);
-    const int g0 = total * wave / kWaves, g1 = total * (wave + 1) / kWaves;
+    const int g0 = total * wave / NW, g1 = total * (wave + 1) / NW;
     int off = 0;
     for (int A = 0; A < nblk; A++) {
         for (int Bk = A; Bk < nblk; Bk++) {
@@ -330,15 +342,15 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
 }
 
 // k-space gradient and Laplacian of sum_k alpha_k |rho_k|^2 for every particle
-// (lane = particle, the k range split over the 4 waves).
-template <typename T, int D, bool PP, bool PER, bool CPLX>
+// (lane = particle, the k range split over the walker's NW waves).
+template <typename T, int D, bool PP, bool PER, bool CPLX, int NW>
 __device__ __forceinline__ void eval_kgrad(const KSys& sy, const KTables<T>& tb, const typename Coord<T, PER>::S* pos,
                                            const T* rho, const T* kq, double* gsum, double* lsum, int wave, int lane,
                                            double& lap_tot_R, double& lap_tot_I)
 {
     using C = Coord<T, PER>;
     const int N = sy.N, K = sy.Kh;
-    const int kchunk = (K + kWaves - 1) / kWaves;
+    const int kchunk = (K + NW - 1) / NW;
     const int k_lo = wave * kchunk, k_hi = min(K, (wave + 1) * kchunk);
     for (int rb = 0; rb < N; rb += 64) {
         const int n = rb + lane;
@@ -395,37 +407,45 @@ __device__ __forceinline__ void eval_kgrad(const KSys& sy, const KTables<T>& tb,
 
 // Pairs and the k-space gradient for one value of CPLX, the potential type
 // resolved to a template argument.
-template <typename T, int D, bool PP, bool PER, bool CPLX>
+template <typename T, int D, bool PP, bool PER, bool CPLX, int NW>
 __device__ __forceinline__ void eval_forces(const KSys& sy, const KTables<T>& tb,
                                             const typename Coord<T, PER>::S* pos, const T* dsp, double* hist,
                                             double* gsum, double* lsum, int wave, int lane, double& vpot,
                                             double& lap_tot_R, double& lap_tot_I)
 {
     switch (sy.pot_type) {
-    case 1: eval_pairs<T, D, PP, PER, 1, CPLX>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
-    case 2: eval_pairs<T, D, PP, PER, 2, CPLX>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
-    case 3: eval_pairs<T, D, PP, PER, 3, CPLX>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
-    default: eval_pairs<T, D, PP, PER, 0, CPLX>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
+    case 1: eval_pairs<T, D, PP, PER, 1, CPLX, NW>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
+    case 2: eval_pairs<T, D, PP, PER, 2, CPLX, NW>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
+    case 3: eval_pairs<T, D, PP, PER, 3, CPLX, NW>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
+    default: eval_pairs<T, D, PP, PER, 0, CPLX, NW>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
     }
 }
 
-// One workgroup evaluates walker w of the system described by (sy, tb).
-template <typename T, int D, bool PP, bool PER, typename TO>
+// WPW wavefronts evaluate walker w of the system described by (sy, tb): the
+// workgroup's walker slot `slot` (threads slot * 64 WPW ..), with its own
+// LDS region of slot_bytes.  A slot past the last walker repeats the last
+// walker's work and stores nothing, so every barrier is met by the whole
+// workgroup.
+template <typename T, int D, bool PP, bool PER, typename TO, int WPW>
 __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, const double* __restrict__ Rg,
-                                          int64_t W, const EvalOut<TO>& out, const T* __restrict__ rho_in, int64_t w)
+                                          int64_t W, const EvalOut<TO>& out, const T* __restrict__ rho_in,
+                                          int64_t w_slot, int slot, size_t slot_bytes)
 {
     using C = Coord<T, PER>;
     using S = typename C::S;
-    if (w >= W) return;
+    constexpr int NT = 64 * WPW;   // threads per walker
+    const bool active = w_slot < W;
+    const int64_t w = active ? w_slot : W - 1;
     const int N = sy.N, Ps = sy.P_s, K = sy.Kh;   // K: +-k representatives
-    const int tid = threadIdx.x, lane = tid & 63;
+    const int tid = threadIdx.x - slot * NT, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (scalar registers)
 
-    extern __shared__ __align__(16) unsigned char smem[];
+    extern __shared__ __align__(16) unsigned char smem_all[];
+    unsigned char* smem = smem_all + (size_t)slot * slot_bytes;
     // LDS carve-up (sizes mirrored by eval_lds_bytes); per-particle arrays are
     // structure-of-arrays so consecutive particles sit in consecutive banks
-    double* red = reinterpret_cast<double*>(smem);             // 8 doubles
-    double* hist = red + 8;                                    // P_s * 4 * kHistCols
+    double* red = reinterpret_cast<double*>(smem);             // kRedDoubles
+    double* hist = red + kRedDoubles;                          // P_s * 4 * kHistCols
     double* gsum = hist + (size_t)Ps * 4 * kHistCols;          // [2D][N]: Re g, Im g
     double* lsum = gsum + (size_t)N * 2 * D;                   // [2][N] (PP only)
     S* pos = reinterpret_cast<S*>(lsum + (PP ? (size_t)N * 2 : 0));   // [D][N]
@@ -441,7 +461,7 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     // (every quantity is invariant under it), stored as Coord values
     const double scale = PER ? sy.Linv : 1.0;
     const double* Rw = Rg + w * (size_t)N * D;
-    for (int i = tid; i < N; i += kThreads) {
+    for (int i = tid; i < N; i += NT) {
 #pragma unroll
         for (int a = 0; a < D; a++) pos[a * N + i] = C::load(Rw[(size_t)i * D + a] * scale);
 #pragma unroll
@@ -451,9 +471,9 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
             lsum[N + i] = 0.0;
         }
     }
-    for (int i = tid; i < Ps * 4 * kHistCols; i += kThreads) hist[i] = 0.0;
-    for (int i = tid; i < Ps * kDsplStride; i += kThreads) dsp[i] = tb.dspl[i];
-    for (int k = tid; k < K; k += kThreads) {
+    for (int i = tid; i < Ps * 4 * kHistCols; i += NT) hist[i] = 0.0;
+    for (int i = tid; i < Ps * kDsplStride; i += NT) dsp[i] = tb.dspl[i];
+    for (int k = tid; k < K; k += NT) {
         // periodic: kv L is the integer lattice vector n (rounded to remove
         // the FP32 storage error of n / L)
 #pragma unroll
@@ -466,13 +486,13 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     // any nonzero imaginary parameter?  If none, the imaginary parts of the
     // pair and k-space sums are exact zeros and are skipped
     int nz = 0;
-    for (int i = tid; i < sy.P; i += kThreads) nz |= tb.uI[i] != 0.0;
+    for (int i = tid; i < sy.P; i += NT) nz |= tb.uI[i] != 0.0;
     const bool cplx = __syncthreads_or(nz) != 0;
 
     // ------------------------------------------------------------ rho_k
     // phase in revolutions = kq . raw coordinate, |phase| <= sum_a |n_a| / 2
     double ex_k = 0.0;
-    for (int k = tid; k < K; k += kThreads) {
+    for (int k = tid; k < K; k += NT) {
         T re = (T)0, im = (T)0;
         if (rho_in) {
             re = rho_in[((size_t)w * K + k) * 2];
@@ -497,10 +517,10 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
         const double ok = (double)re * (double)re + (double)im * (double)im;
         const int m0 = tb.kmap[2 * k], m1 = tb.kmap[2 * k + 1];
         double a = tb.uR[sy.off_k + m0];
-        out.O_store(w, sy.P, sy.off_k + m0, ok);
+        if (active) out.O_store(w, sy.P, sy.off_k + m0, ok);
         if (m1 >= 0) {
             a += tb.uR[sy.off_k + m1];
-            out.O_store(w, sy.P, sy.off_k + m1, ok);
+            if (active) out.O_store(w, sy.P, sy.off_k + m1, ok);
         }
         ex_k += a * ok;
     }
@@ -509,16 +529,20 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     // ------------------------------------------------------------ pairs, k-space gradient / Laplacian
     double vpot = 0.0, lap_tot_R = 0.0, lap_tot_I = 0.0;
     if (PP || cplx) {
-        eval_forces<T, D, PP, PER, true>(sy, tb, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I);
-        if (K > 0) eval_kgrad<T, D, PP, PER, true>(sy, tb, pos, rho, kq, gsum, lsum, wave, lane, lap_tot_R, lap_tot_I);
+        eval_forces<T, D, PP, PER, true, WPW>(sy, tb, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R,
+                                              lap_tot_I);
+        if (K > 0)
+            eval_kgrad<T, D, PP, PER, true, WPW>(sy, tb, pos, rho, kq, gsum, lsum, wave, lane, lap_tot_R, lap_tot_I);
     } else if constexpr (!PP) {
-        eval_forces<T, D, PP, PER, false>(sy, tb, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I);
-        if (K > 0) eval_kgrad<T, D, PP, PER, false>(sy, tb, pos, rho, kq, gsum, lsum, wave, lane, lap_tot_R, lap_tot_I);
+        eval_forces<T, D, PP, PER, false, WPW>(sy, tb, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R,
+                                               lap_tot_I);
+        if (K > 0)
+            eval_kgrad<T, D, PP, PER, false, WPW>(sy, tb, pos, rho, kq, gsum, lsum, wave, lane, lap_tot_R, lap_tot_I);
     }
     __syncthreads();
 
     // histogram columns -> column 0 (interval moments M[j][p])
-    for (int e = tid; e < Ps * 4; e += kThreads) {
+    for (int e = tid; e < Ps * 4; e += NT) {
         double m = 0.0;
         for (int c = 0; c < kHistCols; c++) m += hist[e * kHistCols + c];
         hist[e * kHistCols] = m;
@@ -535,7 +559,7 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     // ------------------------------------------------------------ assembly
     const double obR = sy.one_body ? tb.uR[0] : 0.0, obI = sy.one_body ? tb.uI[0] : 0.0;
     double sR1 = 0, sI1 = 0, sR1I1 = 0, sR2 = lap_tot_R, sI2 = lap_tot_I, sr2 = 0, vext = 0;
-    for (int n = tid; n < N; n += kThreads) {
+    for (int n = tid; n < N; n += NT) {
         double gr[D], gi[D];
         double r2n = 0.0;
 #pragma unroll
@@ -567,7 +591,7 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
         sR1I1 += 2.0 * dRI;
         sR1 += nR;
         sI1 += nI;
-        if (PP) {
+        if (PP && active) {
 #pragma unroll
             for (int a = 0; a < D; a++) {
                 out.grad[(((size_t)w * N + n) * D + a) * 2] = gr[a];
@@ -584,7 +608,7 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     // sum_p beta[m][p] t^p, so B_s = sum_m sum_p beta[m][p] M[s-m][p] with the
     // interval moments M[j][p] = sum over pairs in j of t^p.
     double ex_p = 0.0;
-    for (int k = tid; k < Ps; k += kThreads) {
+    for (int k = tid; k < Ps; k += NT) {
         constexpr double beta[4][4] = {{1.0 / 6, -3.0 / 6, 3.0 / 6, -1.0 / 6},
                                        {4.0 / 6, 0.0, -6.0 / 6, 3.0 / 6},
                                        {1.0 / 6, 3.0 / 6, 3.0 / 6, -3.0 / 6},
@@ -606,19 +630,21 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
         else if (k == Ps - 2) o = B(Ps - 1) + B(Ps + 1);
         else o = B(Ps);
         ex_p += tb.uR[sy.off_pair + k] * o;
-        out.O_store(w, sy.P, sy.off_pair + k, o);
+        if (active) out.O_store(w, sy.P, sy.off_pair + k, o);
     }
 
-    sR1 = block_sum(sR1, red);
-    sI1 = block_sum(sI1, red);
-    sR1I1 = block_sum(sR1I1, red);
-    sR2 = block_sum(sR2, red);
-    sI2 = block_sum(sI2, red);
-    sr2 = block_sum(sr2, red);
-    vext = block_sum(vext, red);
-    vpot = block_sum(vpot, red);
-    double ex = block_sum(ex_k + ex_p, red);
-    if (tid == 0) {
+    double sums[9] = {sR1, sI1, sR1I1, sR2, sI2, sr2, vext, vpot, ex_k + ex_p};
+    walker_sum<WPW>(sums, red, wave, lane);
+    sR1 = sums[0];
+    sI1 = sums[1];
+    sR1I1 = sums[2];
+    sR2 = sums[3];
+    sI2 = sums[4];
+    sr2 = sums[5];
+    vext = sums[6];
+    vpot = sums[7];
+    double ex = sums[8];
+    if (tid == 0 && active) {
         if (sy.one_body) {
             out.O_store(w, sy.P, 0, -sr2);
             ex += tb.uR[0] * (-sr2);
@@ -649,10 +675,12 @@ template <typename T, int D, bool PP, bool PER, typename TO>
 __global__ __launch_bounds__(kThreads, 4) void eval_kernel(KSys sy, KTables<T> tb, const double* __restrict__ Rg,
                                                          int64_t W, EvalOut<TO> out, const T* __restrict__ rho_in)
 {
-    eval_body<T, D, PP, PER, TO>(sy, tb, Rg, W, out, rho_in, blockIdx.x);
+    eval_body<T, D, PP, PER, TO, 4>(sy, tb, Rg, W, out, rho_in, blockIdx.x, 0, 0);
 }
 
-// Batch of independent systems in one launch, one workgroup per walker.
+// Batch of independent systems in one launch.  Each system has its own
+// walker shape: 1, 2 or 4 wavefronts per walker (4, 2 or 1 walkers per
+// workgroup); a workgroup belongs to one system.
 template <typename T, int D, bool PER>
 __global__ __launch_bounds__(kThreads, 4) void eval_batch_kernel(EvalBatch<T> bt)
 {
@@ -666,11 +694,30 @@ __global__ __launch_bounds__(kThreads, 4) void eval_batch_kernel(EvalBatch<T> bt
     }
     const EvalBatchItem<T>& it = bt.item[lo];
     EvalOut<double> o{it.expo, it.ER, it.EI, it.O, it.other, nullptr, nullptr, it.outer, it.phiR};
-    eval_body<T, D, false, PER, double>(bt.sys[lo], bt.tab[lo], it.R, it.W, o, it.rho,
-                                        (int64_t)blockIdx.x - bt.block_start[lo]);
+    const int64_t b = (int64_t)blockIdx.x - bt.block_start[lo];
+    const size_t sb = (size_t)it.slot_bytes;
+    switch (it.wpw) {
+    case 1: {
+        const int slot = threadIdx.x >> 6;
+        eval_body<T, D, false, PER, double, 1>(bt.sys[lo], bt.tab[lo], it.R, it.W, o, it.rho, b * 4 + slot, slot, sb);
+        break;
+    }
+    case 2: {
+        const int slot = threadIdx.x >> 7;
+        eval_body<T, D, false, PER, double, 2>(bt.sys[lo], bt.tab[lo], it.R, it.W, o, it.rho, b * 2 + slot, slot, sb);
+        break;
+    }
+    default:
+        eval_body<T, D, false, PER, double, 4>(bt.sys[lo], bt.tab[lo], it.R, it.W, o, it.rho, b, 0, sb);
+        break;
+    }
 }
 
 }  // namespace
+
+// One wavefront per walker up to N = 32 (a single 64-row pair tile), two up
+// to 64, four above.
+int eval_default_wpw(int N) { return N <= 32 ? 1 : (N <= 64 ? 2 : 4); }
 
 template <typename T>
 size_t eval_lds_bytes(const KSys& sy, bool per_particle)
@@ -680,7 +727,7 @@ size_t eval_lds_bytes(const KSys& sy, bool per_particle)
     // coordinates (Coord storage has the size of T) and rho
     const size_t nt = (size_t)sy.N * sy.D + 2 * (size_t)sy.Kh;
     // the k-vector table and the derivative table start 16-byte aligned
-    const size_t head = (64 + nd * sizeof(double) + nt * sizeof(T) + 15) & ~(size_t)15;
+    const size_t head = (kRedDoubles * sizeof(double) + nd * sizeof(double) + nt * sizeof(T) + 15) & ~(size_t)15;
     return head + (4 * (size_t)sy.Kh + (size_t)sy.P_s * kDsplStride) * sizeof(T);
 }
 

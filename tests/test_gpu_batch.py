@@ -123,3 +123,34 @@ def test_batch_graph_replays_match_eager():
     for a, g in zip(res[0], res[1]):
         assert np.all(np.isfinite(a))
         assert np.allclose(a, g, rtol=1e-11, atol=1e-13)
+
+
+def test_batch_autotune_keeps_results():
+    """The measurement's walker shape (1, 2 or 4 wavefronts per walker, per
+    system) changes the work split only: FP64 results before and after the
+    autotuner's choice agree, and match the oracle."""
+    ns = (3, 10, 33, 64, 100)                    # default shapes 1, 1, 2, 2, 4
+    cfgs = _cfgs(ns=ns)
+    print("batch autotune", flush=True)
+    with tdvmc.Batch(cfgs) as b:
+        walkers = []
+        for i, c in enumerate(b.systems):
+            s, uR, uI = _params(cfgs[i], i)
+            c.set_params(uR, uI)
+            walkers.append((s, c.get_walkers(), uR, uI))
+        before = b.measure(precision=tdvmc.FP64)
+        waves = b.autotune(tdvmc.FP64)
+        assert len(waves) == len(ns) and all(w in (1, 2, 4) for w in waves)
+        after = b.measure(precision=tdvmc.FP64)
+        assert all(w in (1, 2, 4) for w in b.autotune(tdvmc.MIXED))
+        mixed = b.measure(precision=tdvmc.MIXED)
+    print("waves per walker", waves)
+    for i, (s, R, uR, uI) in enumerate(walkers):
+        out = o.eval_configs(s, R, uR, uI)
+        m = o.moments(out["O"], out["ER"], out["EI"])
+        for k in ("mO", "mE", "S", "FR", "FI"):
+            assert np.allclose(after[i][k], before[i][k], rtol=1e-11, atol=1e-14), (i, k)
+        assert rel(after[i]["mO"], m["mO"]) < 1e-10
+        assert rel(after[i]["S"], m["S"]) < 1e-10
+        assert rel(after[i]["FR"], m["FR"]) < 1e-9
+        assert rel(mixed[i]["mO"], m["mO"]) < 2e-6
