@@ -214,7 +214,7 @@ def run_batch_c4(steps, warmup):
     cfgs = trap2d_batch64()
     with t.Batch(cfgs) as b:
         b.sweep(0, seed=3, precision=t.MIXED)         # thermalise (each system N moves)
-        waves = b.autotune(t.MIXED)                   # measurement walker shape per system
+        waves, lanes = b.autotune(t.MIXED)            # measurement and sweep shapes per system
         for _ in range(warmup):
             b.tdvp_step(1e-3, ode_type=2, precision=t.MIXED, use_graph=True)
         t0 = time.perf_counter()
@@ -223,11 +223,13 @@ def run_batch_c4(steps, warmup):
         el = time.perf_counter() - t0
         W = sum(c.info.n_walkers for c in b.systems)
         Ns = [c.N for c in b.systems]
-    by_n = {}
-    for n, wv in zip(Ns, waves):
+    by_n, by_l = {}, {}
+    for n, wv, ln in zip(Ns, waves, lanes):
         by_n.setdefault(n, set()).add(wv)
+        by_l.setdefault(n, set()).add(ln)
     return {"systems": len(cfgs), "walkers": W, "N_range": [min(Ns), max(Ns)],
             "measure_waves_per_walker": {str(n): sorted(v) for n, v in sorted(by_n.items())},
+            "sweep_lanes_per_walker": {str(n): sorted(v) for n, v in sorted(by_l.items())},
             "walker_samples_per_s": 4 * steps * W / el, "tdvp_steps_per_s": steps / el,
             "ms_per_step": el / steps * 1e3,
             "step": "RK4: 4 x (one batched sweep of N moves per system + batched measurement + "

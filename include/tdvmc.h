@@ -225,13 +225,17 @@ int tdvmc_batch_destroy(tdvmc_batch* b);
 const char* tdvmc_batch_last_error(const tdvmc_batch* b);
 int tdvmc_batch_size(tdvmc_batch* b, int* n_systems);
 tdvmc_ctx* tdvmc_batch_system(tdvmc_batch* b, int s);
-/* lanes per walker of the batched sweep: 16, 32, 64, or 0 for the default */
+/* lanes per walker of the batched sweep, shared by every system: 16, 32, 64,
+ * or 0 for the default (per system in batches without k-space terms) */
 int tdvmc_batch_set_sweep_lanes(tdvmc_batch* b, int lanes);
 /* Times each system's measurement with 1, 2 and 4 wavefronts per walker (4,
- * 2 or 1 walkers per workgroup) and keeps the fastest per system;
- * waves_out [n_systems] (may be NULL) receives the choice.  No counterpart in
- * the reference (the reference evaluates one walker at a time). */
-int tdvmc_batch_autotune(tdvmc_batch* b, int precision, int* waves_out);
+ * 2 or 1 walkers per workgroup) and, in batches without k-space terms and
+ * without a shared lane setting, each system's sweep with 8, 16, 32 and 64
+ * lanes per walker (one sweep of a scratch copy of its walkers); keeps the
+ * fastest per system.  waves_out, lanes_out [n_systems] (may be NULL) receive
+ * the choices.  No counterpart in the reference (the reference evaluates one
+ * walker at a time). */
+int tdvmc_batch_autotune(tdvmc_batch* b, int precision, int* waves_out, int* lanes_out);
 /* n_moves <= 0: each system's MC_NTHERMSTEPS; system s draws from seed
  * mixed with s (walker w of system s follows the same random stream as walker
  * w of a single context seeded with that mix). */

@@ -98,7 +98,7 @@ SYMBOLS = {
     "tdvmc_batch_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "tdvmc_batch_system": (C.c_void_p, [C.c_void_p, C.c_int]),
     "tdvmc_batch_set_sweep_lanes": (C.c_int, [C.c_void_p, C.c_int]),
-    "tdvmc_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "tdvmc_batch_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "tdvmc_batch_sweep": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.c_int]),
     "tdvmc_batch_measure": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "tdvmc_batch_tdvp_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int]),
@@ -391,10 +391,13 @@ class Batch:
 
     def autotune(self, precision: int = DEFAULT):
         """Times every system's measurement with 1, 2 and 4 wavefronts per
-        walker and keeps the fastest; returns the choice per system."""
-        out = (C.c_int * len(self.systems))()
-        self._check(lib().tdvmc_batch_autotune(self._h, int(precision), C.cast(out, C.c_void_p)))
-        return list(out)
+        walker and (no k-space terms) its sweep with 8 to 64 lanes per walker,
+        and keeps the fastest; returns (waves, lanes) per system."""
+        waves = (C.c_int * len(self.systems))()
+        lanes = (C.c_int * len(self.systems))()
+        self._check(lib().tdvmc_batch_autotune(self._h, int(precision), C.cast(waves, C.c_void_p),
+                                               C.cast(lanes, C.c_void_p)))
+        return list(waves), list(lanes)
 
     def sweep(self, n_moves: int = 0, seed: int = 1, precision: int = DEFAULT):
         self._check(lib().tdvmc_batch_sweep(self._h, int(n_moves), int(seed), int(precision)))

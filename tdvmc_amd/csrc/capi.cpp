@@ -903,6 +903,7 @@ struct tdvmc_batch {
     int D = 0, periodic = 0, kh_max = 0, P_max = 0, N_max = 0;
     int lanes = 0;                    // lanes per walker of the batched sweep (0: default)
     std::vector<int> eval_wpw;        // per system: wavefronts per walker of the measurement (0: default)
+    std::vector<int> sweep_lanes;     // per system: lanes per walker of the sweep (0: default; no k-space terms)
     int tune_gen = 0;                 // changes whenever eval_wpw does
     std::vector<int64_t> xoff;
     int64_t xtotal = 0;
@@ -956,9 +957,24 @@ int upload_array(tdvmc_batch* b, T** dst, const std::vector<T>& src)
     return TDVMC_OK;
 }
 
+// Lanes per walker of the batched sweep shared by every system, or 0 when
+// each system has its own (batches without k-space terms, no override).
 int batch_lanes(const tdvmc_batch* b)
 {
-    return b->lanes ? b->lanes : (b->N_max <= 48 ? 16 : sweep_default_lanes(b->N_max, b->kh_max));
+    if (b->lanes) return b->lanes;
+    if (b->kh_max > 0) return b->N_max <= 48 ? 16 : sweep_default_lanes(b->N_max, b->kh_max);
+    return 0;
+}
+
+// lanes per walker of system s: shared, tuned, or the smallest of 8, 16, 32
+// that covers N (32 above)
+int batch_sys_lanes(const tdvmc_batch* b, int s)
+{
+    const int u = batch_lanes(b);
+    if (u) return u;
+    if (s < (int)b->sweep_lanes.size() && b->sweep_lanes[s]) return b->sweep_lanes[s];
+    const int N = b->sys[s]->sy.N;
+    return N <= 8 ? 8 : (N <= 16 ? 16 : 32);
 }
 
 uint64_t batch_system_seed(uint64_t seed, int s) { return seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(s + 1)); }
@@ -983,8 +999,6 @@ int batch_upload(tdvmc_batch* b, int64_t n_moves, uint64_t seed, bool want_rho)
                   want_rho ? 1 : 0, b->lanes, b->tune_gen);
     if (b->desc_key == key) return TDVMC_OK;
     const int n = (int)b->sys.size();
-    const int lanes = batch_lanes(b);
-    const int wpb = sweep_walkers_per_block(lanes);
     std::vector<KSys> sy(n);
     std::vector<KTables<float>> tf(n);
     std::vector<KTables<double>> td(n);
@@ -1007,6 +1021,8 @@ int batch_upload(tdvmc_batch* b, int64_t n_moves, uint64_t seed, bool want_rho)
                                     b->xoff[s]};
         bpd[s] = BatchParams<double>{c->sy, c->dUR, c->dUI, c->dKmap, c->td.spl, c->td.dspl, c->td.alpha, c->dX,
                                      b->xoff[s]};
+        const int lanes = batch_sys_lanes(b, s);
+        const int wpb = sweep_walkers_per_block(lanes);
         sstart[s] = sb;
         sb += (c->W + wpb - 1) / wpb;
         const int wpw = batch_eval_wpw(b, s);
@@ -1204,6 +1220,64 @@ void batch_release(tdvmc_batch* b)
 }  // namespace
 
 namespace {
+
+// Average time of one sweep (N moves per walker) of system s alone with
+// `lanes` lanes per walker, on a scratch copy of its walkers (the chains and
+// acceptance counts are left as they are).
+template <typename T>
+int batch_time_sweep(tdvmc_batch* b, int s, int lanes, const KTables<T>* tabs, double* dRs,
+                     unsigned long long* dAccS, float* ms)
+{
+    tdvmc_ctx* c = b->sys[s];
+    SweepArgs a;
+    BHIPC(hipMemcpy(&a, b->dSweepArgs + s, sizeof(a), hipMemcpyDeviceToHost));
+    a.R = dRs;
+    a.acc = dAccS;
+    a.lanes_per_walker = lanes;
+    a.n_moves = c->sy.N;
+    a.rho_out = nullptr;
+    SweepArgs* dA = nullptr;
+    int64_t* dStart = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = TDVMC_OK;
+    auto fail_hip = [&](hipError_t e) {
+        b->err = std::string("HIP error: ") + hipGetErrorString(e);
+        rc = TDVMC_ERR_HIP;
+    };
+    const int64_t zero = 0;
+    const size_t rbytes = (size_t)c->W * c->sy.N * c->sy.D * sizeof(double);
+    hipError_t e;
+    if ((e = dalloc(&dA, 1)) != hipSuccess || (e = dalloc(&dStart, 1)) != hipSuccess ||
+        (e = hipMemcpy(dA, &a, sizeof(a), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(dStart, &zero, sizeof(zero), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
+        fail_hip(e);
+    } else {
+        const SweepBatch<T> bt{1, dStart, b->dSys + s, tabs + s, dA};
+        const int wpb = sweep_walkers_per_block(lanes);
+        const int64_t blocks = (c->W + wpb - 1) / wpb;
+        const size_t lds = sweep_lds_bytes<T>(c->sy, lanes);
+        constexpr int kReps = 2;
+        float tot = 0.0f;
+        for (int r = 0; r <= kReps && rc == TDVMC_OK; r++) {   // r = 0: warm-up
+            float t = 0.0f;
+            if ((e = hipMemcpyAsync(dRs, c->dR, rbytes, hipMemcpyDeviceToDevice, b->stream)) != hipSuccess ||
+                (e = hipEventRecord(e0, b->stream)) != hipSuccess ||
+                (e = launch_sweep_batch<T>(bt, b->D, b->periodic != 0, 0, 0, blocks, lds, b->stream)) != hipSuccess ||
+                (e = hipEventRecord(e1, b->stream)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess ||
+                (e = hipEventElapsedTime(&t, e0, e1)) != hipSuccess)
+                fail_hip(e);
+            if (r > 0) tot += t;
+        }
+        *ms = tot / kReps;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipStreamSynchronize(b->stream);
+    dfree(dA);
+    dfree(dStart);
+    return rc;
+}
 
 // Average time of system s's measurement alone with `wpw` wavefronts per
 // walker: a one-system launch whose descriptors point into the batch's
@@ -2107,7 +2181,7 @@ int tdvmc_batch_set_sweep_lanes(tdvmc_batch* b, int lanes)
     return TDVMC_OK;
 }
 
-int tdvmc_batch_autotune(tdvmc_batch* b, int precision, int* waves_out)
+int tdvmc_batch_autotune(tdvmc_batch* b, int precision, int* waves_out, int* lanes_out)
 {
     if (!b) return TDVMC_ERR_BAD_ARG;
     int prec, rc = batch_precision(b, precision, &prec);
@@ -2116,6 +2190,7 @@ int tdvmc_batch_autotune(tdvmc_batch* b, int precision, int* waves_out)
     if ((rc = batch_upload(b, 0, b->seed, true))) return rc;
     if ((rc = batch_prep(b, prec))) return rc;
     const int n = (int)b->sys.size();
+    // measurement: wavefronts per walker
     std::vector<int> best(n, 4);
     for (int s = 0; s < n; s++) {
         const tdvmc_ctx* c = b->sys[s];
@@ -2135,12 +2210,54 @@ int tdvmc_batch_autotune(tdvmc_batch* b, int precision, int* waves_out)
             }
         }
     }
+    // sweep: lanes per walker of each system (batches without k-space terms
+    // and without a shared setting), timed on scratch copies of the walkers
+    std::vector<int> lanes(n, 0);
+    if (batch_lanes(b) == 0) {
+        size_t rmax = 0, wmax = 0;
+        for (auto* c : b->sys) {
+            rmax = std::max(rmax, (size_t)c->W * c->sy.N * c->sy.D);
+            wmax = std::max(wmax, (size_t)c->W);
+        }
+        double* dRs = nullptr;
+        unsigned long long* dAccS = nullptr;
+        if (dalloc(&dRs, rmax) != hipSuccess || dalloc(&dAccS, wmax) != hipSuccess) {
+            dfree(dRs);
+            dfree(dAccS);
+            b->err = "out of device memory";
+            return TDVMC_ERR_HIP;
+        }
+        for (int s = 0; s < n && rc == TDVMC_OK; s++) {
+            float tb = 0.0f;
+            bool have = false;
+            for (int l : {8, 16, 32, 64}) {
+                const size_t lds = std::max(sweep_lds_bytes<float>(b->sys[s]->sy, l),
+                                            sweep_lds_bytes<double>(b->sys[s]->sy, l));
+                if (lds > 160 * 1024) continue;
+                float t = 0.0f;
+                rc = prec == TDVMC_MIXED ? batch_time_sweep<float>(b, s, l, b->dTabF, dRs, dAccS, &t)
+                                         : batch_time_sweep<double>(b, s, l, b->dTabD, dRs, dAccS, &t);
+                if (rc) break;
+                if (!have || t < tb) {
+                    tb = t;
+                    lanes[s] = l;
+                    have = true;
+                }
+            }
+        }
+        dfree(dRs);
+        dfree(dAccS);
+        if (rc) return rc;
+        b->sweep_lanes = lanes;
+    }
     b->eval_wpw = best;
     b->tune_gen++;
     batch_drop_graph(b);
     if ((rc = batch_upload(b, 0, b->seed, true))) return rc;
-    if (waves_out)
-        for (int s = 0; s < n; s++) waves_out[s] = batch_eval_wpw(b, s);
+    for (int s = 0; s < n; s++) {
+        if (waves_out) waves_out[s] = batch_eval_wpw(b, s);
+        if (lanes_out) lanes_out[s] = batch_sys_lanes(b, s);
+    }
     return TDVMC_OK;
 }
 

@@ -265,7 +265,9 @@ __device__ __forceinline__ T group_sum(T v)
 {
     v += dpp_f<kDppXor1>(v);
     v += dpp_f<kDppXor2>(v);
+    if (G == 4) return v;
     v += dpp_f<kDppHalfMirror>(v);
+    if (G == 8) return v;
     v += dpp_f<kDppRowMirror>(v);
     if (G == 16) return v;
     const T r0 = readlane_f(v, 0), r1 = readlane_f(v, 16), r2 = readlane_f(v, 32), r3 = readlane_f(v, 48);

@@ -31,7 +31,7 @@ struct SweepArgs {
     uint64_t seed;
     const uint32_t* epoch;     // device counter selecting the random stream of this call
     unsigned long long* acc;   // [W] accepted moves (accumulated)
-    int lanes_per_walker;      // 16, 32 or 64 (0 = choose)
+    int lanes_per_walker;      // 16, 32 or 64 (0 = choose); 8 also in batches without k-space terms
     void* rho_out;             // [W][K][2] (T) final rho_k of every walker, or null
 };
 template <typename T>
@@ -39,7 +39,9 @@ hipError_t launch_sweep(const KSys& sy, const KTables<T>& tb, const SweepArgs& a
 
 // A batch of independent systems in one launch (device arrays, n systems;
 // block_start [n] is the first workgroup of each system, ascending).  All
-// systems share DIM, the box kind and the lanes per walker.
+// systems share DIM and the box kind.  lanes > 0: every system uses that many
+// lanes per walker; lanes = 0 (batches without k-space terms): each system's
+// SweepArgs::lanes_per_walker (8, 16, 32 or 64).
 template <typename T>
 struct SweepBatch {
     int n;

@@ -418,6 +418,40 @@ static hipError_t launch_sweep_d(const KSys& sy, const KTables<T>& tb, const Swe
     }
 }
 
+// Batch without k-space terms, each system with its own lanes per walker
+// (8, 16, 32 or 64, from its SweepArgs); a workgroup belongs to one system.
+template <typename T, int D, bool PER>
+__global__ __launch_bounds__(kThreads) void sweep_batch_lanes_kernel(SweepBatch<T> bt)
+{
+    int lo = 0, hi = bt.n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (bt.block_start[mid] <= (int64_t)blockIdx.x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const int64_t blk = (int64_t)blockIdx.x - bt.block_start[lo];
+    switch (bt.args[lo].lanes_per_walker) {
+    case 8: sweep_body<T, D, 8, 0, PER>(bt.sys[lo], bt.tab[lo], bt.args[lo], blk); break;
+    case 16: sweep_body<T, D, 16, 0, PER>(bt.sys[lo], bt.tab[lo], bt.args[lo], blk); break;
+    case 64: sweep_body<T, D, 64, 0, PER>(bt.sys[lo], bt.tab[lo], bt.args[lo], blk); break;
+    default: sweep_body<T, D, 32, 0, PER>(bt.sys[lo], bt.tab[lo], bt.args[lo], blk); break;
+    }
+}
+
+template <typename T, int D, bool PER>
+static hipError_t launch_sweep_batch_lanes(const SweepBatch<T>& bt, int64_t blocks, size_t lds, hipStream_t st)
+{
+    auto kern = sweep_batch_lanes_kernel<T, D, PER>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kThreads), lds, st, bt);
+    return hipGetLastError();
+}
+
 template <typename T, int D, int LPW, int KPL, bool PER>
 static hipError_t launch_sweep_batch_k(const SweepBatch<T>& bt, int64_t blocks, size_t lds, hipStream_t st)
 {
@@ -451,6 +485,19 @@ hipError_t launch_sweep_batch(const SweepBatch<T>& bt, int D, bool periodic, int
                               size_t lds, hipStream_t st)
 {
     if (blocks <= 0) return hipSuccess;
+    if (lanes == 0) {
+        // per-system lanes (no k-space terms)
+        if (kh_max > 0) return hipErrorInvalidValue;
+        switch (D) {
+        case 1: return periodic ? launch_sweep_batch_lanes<T, 1, true>(bt, blocks, lds, st)
+                                : launch_sweep_batch_lanes<T, 1, false>(bt, blocks, lds, st);
+        case 2: return periodic ? launch_sweep_batch_lanes<T, 2, true>(bt, blocks, lds, st)
+                                : launch_sweep_batch_lanes<T, 2, false>(bt, blocks, lds, st);
+        case 3: return periodic ? launch_sweep_batch_lanes<T, 3, true>(bt, blocks, lds, st)
+                                : launch_sweep_batch_lanes<T, 3, false>(bt, blocks, lds, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
 #define TDVMC_SWEEP_BATCH_D(DD)                                                                              \
     switch (lanes) {                                                                                         \
     case 16: return periodic ? launch_sweep_batch_l<T, DD, 16, true>(bt, kh_max, blocks, lds, st)             \

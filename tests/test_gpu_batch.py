@@ -58,6 +58,33 @@ def test_batch_sweep_matches_single_contexts(prec):
             assert ctx.acceptance() == acc[i]
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_sweep_per_system_lanes_match_single_contexts(prec):
+    """Without a shared lane setting each system sweeps with the smallest of
+    8, 16, 32 lanes that covers N (32 above).  A walker's random stream does
+    not depend on the lane count, and lanes past N add exact zeros to the
+    group sum, so every system matches a single context with 16 or 32 lanes."""
+    cfgs = _cfgs()
+    seed = 91
+    print("batch sweep per-system lanes", prec, flush=True)
+    with tdvmc.Batch(cfgs) as b:
+        starts = []
+        for i, c in enumerate(b.systems):
+            s, uR, uI = _params(cfgs[i], i)
+            c.set_params(uR, uI)
+            starts.append((c.get_walkers(), uR, uI))
+        b.sweep(40, seed=seed, precision=prec)
+        got = [c.get_walkers() for c in b.systems]
+    for i, cfg in enumerate(cfgs):
+        R0, uR, uI = starts[i]
+        with tdvmc.Context(cfg) as ctx:
+            ctx.set_params(uR, uI)
+            ctx.set_walkers(R0)
+            ctx.set_sweep_lanes(16 if NS[i] <= 16 else 32)
+            ctx.sweep(40, seed=tdvmc.batch_system_seed(seed, i), precision=prec)
+            assert np.array_equal(ctx.get_walkers(), got[i]), f"system {i} (N={NS[i]})"
+
+
 def test_batch_measure_matches_single_and_oracle():
     cfgs = _cfgs()
     print("batch measure", flush=True)
@@ -138,13 +165,19 @@ def test_batch_autotune_keeps_results():
             s, uR, uI = _params(cfgs[i], i)
             c.set_params(uR, uI)
             walkers.append((s, c.get_walkers(), uR, uI))
+        R0 = [c.get_walkers() for c in b.systems]
         before = b.measure(precision=tdvmc.FP64)
-        waves = b.autotune(tdvmc.FP64)
+        waves, lanes = b.autotune(tdvmc.FP64)
         assert len(waves) == len(ns) and all(w in (1, 2, 4) for w in waves)
+        assert all(l in (8, 16, 32, 64) for l in lanes)
+        # the sweep timings ran on scratch copies: walkers and counters untouched
+        for c, r in zip(b.systems, R0):
+            assert np.array_equal(c.get_walkers(), r) and c.acceptance() == (0, 0)
         after = b.measure(precision=tdvmc.FP64)
-        assert all(w in (1, 2, 4) for w in b.autotune(tdvmc.MIXED))
+        w2, l2 = b.autotune(tdvmc.MIXED)
+        assert all(w in (1, 2, 4) for w in w2) and all(l in (8, 16, 32, 64) for l in l2)
         mixed = b.measure(precision=tdvmc.MIXED)
-    print("waves per walker", waves)
+    print("waves per walker", waves, "lanes per walker", lanes)
     for i, (s, R, uR, uI) in enumerate(walkers):
         out = o.eval_configs(s, R, uR, uI)
         m = o.moments(out["O"], out["ER"], out["EI"])
