@@ -18,7 +18,7 @@
 //   k terms  : lane = row n, the k range split over the walker's waves;
 //   E_L      : thread = particle, FP64 block reduction.
 // The pair's position t in its knot interval goes to an LDS histogram of the
-// moments 1, t, t^2, t^3 per interval ([interval][4][8], 8 lanes per column);
+// moments 1, t, t^2, t^3 per interval ([interval][4][16 or 8 columns]);
 // the B-spline basis sums are rebuilt from these moments at the end, and the
 // count of pairs beyond the cutoff follows from the histogram's total.
 // Coordinates and per-particle sums are staged in LDS as structure-of-arrays
@@ -35,7 +35,11 @@ namespace tdvmc {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kHistCols = 8;    // histogram columns (LDS footprint vs bank conflicts: occupancy wins)
+// Histogram columns: 16 with real parameters (no bank conflicts among the 16
+// lanes of one LDS pass), 8 otherwise, where the imaginary gradient sums take
+// the LDS the extra columns would need.
+constexpr int kHistColsC = 8;
+template <bool CPLX> constexpr int hist_cols() { return CPLX ? kHistColsC : 2 * kHistColsC; }
 
 template <typename TO>
 struct EvalOut {
@@ -171,7 +175,8 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
         pc.a2 = (T)(au * au);
         pc.c4 = (T)(4.0 * sy.pot_b);
     }
-    double* const hbase = hist + (lane & (kHistCols - 1));
+    constexpr int HC = hist_cols<CPLX>();
+    double* const hbase = hist + (lane & (HC - 1));
 
     // Pairs in 64 x 64 tiles (A, B), A <= B, of particle blocks.  Lane l owns
     // row particle 64 A + l; column particles ride a register ring that
@@ -241,11 +246,11 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
                     // interval j (FP64); the basis sums B_s are rebuilt from
                     // them at the end
                     const double td = (double)t, t2 = td * td, t3 = t2 * td;
-                    double* hb = hbase + j * (4 * kHistCols);
+                    double* hb = hbase + j * (4 * HC);
                     lds_add(hb, 1.0);
-                    lds_add(hb + kHistCols, td);
-                    lds_add(hb + 2 * kHistCols, t2);
-                    lds_add(hb + 3 * kHistCols, t3);
+                    lds_add(hb + HC, td);
+                    lds_add(hb + 2 * HC, t2);
+                    lds_add(hb + 3 * HC, t3);
                     // u'(r), u''(r) per unit length on interval j
                     const T* c = dsp + j * kDsplStride;
                     const T upR = c[0] + t * (c[1] + t * c[2]);
@@ -440,14 +445,23 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     const int tid = threadIdx.x - slot * NT, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (scalar registers)
 
+    // any nonzero imaginary parameter?  If none, the imaginary parts of the
+    // pair and k-space sums are exact zeros and are skipped (and the LDS
+    // layout trades the imaginary gradient sums for histogram columns)
+    int nz = 0;
+    if (!PP)
+        for (int i = tid; i < sy.P; i += NT) nz |= tb.uI[i] != 0.0;
+    const bool cplx = PP || __syncthreads_or(nz) != 0;
+    const int hc = cplx ? hist_cols<true>() : hist_cols<false>();
+
     extern __shared__ __align__(16) unsigned char smem_all[];
     unsigned char* smem = smem_all + (size_t)slot * slot_bytes;
     // LDS carve-up (sizes mirrored by eval_lds_bytes); per-particle arrays are
     // structure-of-arrays so consecutive particles sit in consecutive banks
     double* red = reinterpret_cast<double*>(smem);             // kRedDoubles
-    double* hist = red + kRedDoubles;                          // P_s * 4 * kHistCols
-    double* gsum = hist + (size_t)Ps * 4 * kHistCols;          // [2D][N]: Re g, Im g
-    double* lsum = gsum + (size_t)N * 2 * D;                   // [2][N] (PP only)
+    double* hist = red + kRedDoubles;                          // P_s * 4 * hc
+    double* gsum = hist + (size_t)Ps * 4 * hc;                 // [D][N] Re g, then (cplx) [D][N] Im g
+    double* lsum = gsum + (size_t)N * (cplx ? 2 : 1) * D;      // [2][N] (PP only)
     S* pos = reinterpret_cast<S*>(lsum + (PP ? (size_t)N * 2 : 0));   // [D][N]
     T* rho = reinterpret_cast<T*>(pos + (size_t)N * D);        // 2K
     // k-vectors in revolutions per coordinate unit ([K][4]), then the
@@ -464,14 +478,13 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     for (int i = tid; i < N; i += NT) {
 #pragma unroll
         for (int a = 0; a < D; a++) pos[a * N + i] = C::load(Rw[(size_t)i * D + a] * scale);
-#pragma unroll
-        for (int c = 0; c < 2 * D; c++) gsum[c * N + i] = 0.0;
+        for (int c = 0; c < (cplx ? 2 * D : D); c++) gsum[c * N + i] = 0.0;
         if (PP) {
             lsum[i] = 0.0;
             lsum[N + i] = 0.0;
         }
     }
-    for (int i = tid; i < Ps * 4 * kHistCols; i += NT) hist[i] = 0.0;
+    for (int i = tid; i < Ps * 4 * hc; i += NT) hist[i] = 0.0;
     for (int i = tid; i < Ps * kDsplStride; i += NT) dsp[i] = tb.dspl[i];
     for (int k = tid; k < K; k += NT) {
         // periodic: kv L is the integer lattice vector n (rounded to remove
@@ -483,11 +496,7 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
         }
         for (int a = D; a < 4; a++) kq[k * 4 + a] = (T)0;
     }
-    // any nonzero imaginary parameter?  If none, the imaginary parts of the
-    // pair and k-space sums are exact zeros and are skipped
-    int nz = 0;
-    for (int i = tid; i < sy.P; i += NT) nz |= tb.uI[i] != 0.0;
-    const bool cplx = __syncthreads_or(nz) != 0;
+    __syncthreads();
 
     // ------------------------------------------------------------ rho_k
     // phase in revolutions = kq . raw coordinate, |phase| <= sum_a |n_a| / 2
@@ -544,15 +553,15 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     // histogram columns -> column 0 (interval moments M[j][p])
     for (int e = tid; e < Ps * 4; e += NT) {
         double m = 0.0;
-        for (int c = 0; c < kHistCols; c++) m += hist[e * kHistCols + c];
-        hist[e * kHistCols] = m;
+        for (int c = 0; c < hc; c++) m += hist[e * hc + c];
+        hist[e * hc] = m;
     }
     __syncthreads();
     // pairs beyond the cutoff: all pairs minus those in the histogram (exact counts)
     double outer = 0.0;
     if (tid == 0) {
         double inner = 0.0;
-        for (int j = 0; j < Ps; j++) inner += hist[j * 4 * kHistCols];
+        for (int j = 0; j < Ps; j++) inner += hist[j * 4 * hc];
         outer = 0.5 * (double)N * (double)(N - 1) - inner;
     }
 
@@ -567,7 +576,7 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
             // one-body factor uses the raw (unwrapped, open-space) coordinate
             const double x = Rw[(size_t)n * D + a];
             gr[a] = (double)gsum[a * N + n] - 2.0 * obR * x;
-            gi[a] = (double)gsum[(D + a) * N + n] - 2.0 * obI * x;
+            gi[a] = (cplx ? (double)gsum[(D + a) * N + n] : 0.0) - 2.0 * obI * x;
             r2n += x * x;
         }
         sr2 += r2n;
@@ -619,7 +628,7 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
                 const int j = s - m;
                 if (j < 0 || j > Ps - 1) continue;
 #pragma unroll
-                for (int p = 0; p < 4; p++) v += beta[m][p] * hist[(j * 4 + p) * kHistCols];
+                for (int p = 0; p < 4; p++) v += beta[m][p] * hist[(j * 4 + p) * hc];
             }
             return v;
         };
@@ -722,8 +731,11 @@ int eval_default_wpw(int N) { return N <= 32 ? 1 : (N <= 64 ? 2 : 4); }
 template <typename T>
 size_t eval_lds_bytes(const KSys& sy, bool per_particle)
 {
-    const size_t nd = (size_t)sy.P_s * 4 * kHistCols + (size_t)sy.N * 2 * sy.D +
-                      (per_particle ? 2 * (size_t)sy.N : 0);
+    // the larger of the two layouts: histogram + real gradient sums (real
+    // parameters), narrower histogram + real and imaginary sums (otherwise)
+    const size_t nd_real = (size_t)sy.P_s * 4 * hist_cols<false>() + (size_t)sy.N * sy.D;
+    const size_t nd_cplx = (size_t)sy.P_s * 4 * hist_cols<true>() + (size_t)sy.N * 2 * sy.D;
+    const size_t nd = (per_particle ? nd_cplx + 2 * (size_t)sy.N : std::max(nd_real, nd_cplx));
     // coordinates (Coord storage has the size of T) and rho
     const size_t nt = (size_t)sy.N * sy.D + 2 * (size_t)sy.Kh;
     // the k-vector table and the derivative table start 16-byte aligned
