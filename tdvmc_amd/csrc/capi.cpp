@@ -985,8 +985,10 @@ int batch_eval_wpw(const tdvmc_batch* b, int s)
 {
     const tdvmc_ctx* c = b->sys[s];
     int wpw = (s < (int)b->eval_wpw.size() && b->eval_wpw[s]) ? b->eval_wpw[s] : eval_default_wpw(c->sy.N);
-    const size_t slot = std::max(eval_lds_bytes<float>(c->sy, false), eval_lds_bytes<double>(c->sy, false));
-    while (wpw < 4 && (size_t)(4 / wpw) * slot > 160 * 1024) wpw *= 2;
+    auto slot = [&](int v) {
+        return std::max(eval_lds_bytes<float>(c->sy, false, v), eval_lds_bytes<double>(c->sy, false, v));
+    };
+    while (wpw < 4 && (size_t)(4 / wpw) * slot(wpw) > 160 * 1024) wpw *= 2;
     return wpw;
 }
 
@@ -1026,7 +1028,8 @@ int batch_upload(tdvmc_batch* b, int64_t n_moves, uint64_t seed, bool want_rho)
         sstart[s] = sb;
         sb += (c->W + wpb - 1) / wpb;
         const int wpw = batch_eval_wpw(b, s);
-        const size_t slot_f = eval_lds_bytes<float>(c->sy, false), slot_d = eval_lds_bytes<double>(c->sy, false);
+        const size_t slot_f = eval_lds_bytes<float>(c->sy, false, wpw);
+        const size_t slot_d = eval_lds_bytes<double>(c->sy, false, wpw);
         estart[s] = eb;
         eb += (c->W + 4 / wpw - 1) / (4 / wpw);
         slds = std::max(slds, std::max(sweep_lds_bytes<float>(c->sy, lanes), sweep_lds_bytes<double>(c->sy, lanes)));
@@ -1287,7 +1290,7 @@ int batch_time_eval(tdvmc_batch* b, int s, int wpw, const EvalBatchItem<T>* item
                     float* ms)
 {
     tdvmc_ctx* c = b->sys[s];
-    const size_t slot = eval_lds_bytes<T>(c->sy, false);
+    const size_t slot = eval_lds_bytes<T>(c->sy, false, wpw);
     EvalBatchItem<T> it;
     BHIPC(hipMemcpy(&it, items + s, sizeof(it), hipMemcpyDeviceToHost));
     it.wpw = wpw;
@@ -2194,10 +2197,11 @@ int tdvmc_batch_autotune(tdvmc_batch* b, int precision, int* waves_out, int* lan
     std::vector<int> best(n, 4);
     for (int s = 0; s < n; s++) {
         const tdvmc_ctx* c = b->sys[s];
-        const size_t slot = std::max(eval_lds_bytes<float>(c->sy, false), eval_lds_bytes<double>(c->sy, false));
         float tb = 0.0f;
         bool have = false;
         for (int wpw : {1, 2, 4}) {
+            const size_t slot =
+                std::max(eval_lds_bytes<float>(c->sy, false, wpw), eval_lds_bytes<double>(c->sy, false, wpw));
             if ((size_t)(4 / wpw) * slot > 160 * 1024) continue;
             float t = 0.0f;
             rc = prec == TDVMC_MIXED ? batch_time_eval<float>(b, s, wpw, b->dEvalF, b->dTabF, &t)

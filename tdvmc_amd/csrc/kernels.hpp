@@ -14,8 +14,9 @@ hipError_t launch_prep_tables(const KSys& sy, const double* uR, const double* uI
                               T* alpha, hipStream_t st);
 
 // ---- evaluation (kernels_eval.hip); grad/lap non-null selects the per-particle variant
+// LDS of one walker with wpw wavefronts (single contexts: 4)
 template <typename T>
-size_t eval_lds_bytes(const KSys& sy, bool per_particle);
+size_t eval_lds_bytes(const KSys& sy, bool per_particle, int wpw = 4);
 // rho_in: [W][K][2] rho_k of the walkers as left by the sweep (skips the
 // rho_k pass), or null to compute it from the coordinates
 template <typename T, typename TO>

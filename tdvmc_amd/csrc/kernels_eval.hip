@@ -35,11 +35,12 @@ namespace tdvmc {
 namespace {
 
 constexpr int kThreads = 256;
-// Histogram columns: 16 with real parameters (no bank conflicts among the 16
-// lanes of one LDS pass), 8 otherwise, where the imaginary gradient sums take
-// the LDS the extra columns would need.
+// Histogram columns: 16 for four-wavefront walkers with real parameters (no
+// bank conflicts among the 16 lanes of one LDS pass; the imaginary gradient
+// sums would take that LDS otherwise), 8 otherwise (smaller walkers spend
+// more on clearing and reducing the histogram than on its conflicts).
 constexpr int kHistColsC = 8;
-template <bool CPLX> constexpr int hist_cols() { return CPLX ? kHistColsC : 2 * kHistColsC; }
+__host__ __device__ constexpr int hist_cols(bool cplx, int nw) { return (cplx || nw < 4) ? kHistColsC : 2 * kHistColsC; }
 
 template <typename TO>
 struct EvalOut {
@@ -175,7 +176,7 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
         pc.a2 = (T)(au * au);
         pc.c4 = (T)(4.0 * sy.pot_b);
     }
-    constexpr int HC = hist_cols<CPLX>();
+    constexpr int HC = hist_cols(CPLX, NW);
     double* const hbase = hist + (lane & (HC - 1));
 
     // Pairs in 64 x 64 tiles (A, B), A <= B, of particle blocks.  Lane l owns
@@ -452,7 +453,7 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     if (!PP)
         for (int i = tid; i < sy.P; i += NT) nz |= tb.uI[i] != 0.0;
     const bool cplx = PP || __syncthreads_or(nz) != 0;
-    const int hc = cplx ? hist_cols<true>() : hist_cols<false>();
+    const int hc = hist_cols(cplx, WPW);
 
     extern __shared__ __align__(16) unsigned char smem_all[];
     unsigned char* smem = smem_all + (size_t)slot * slot_bytes;
@@ -729,12 +730,12 @@ __global__ __launch_bounds__(kThreads, 4) void eval_batch_kernel(EvalBatch<T> bt
 int eval_default_wpw(int N) { return N <= 32 ? 1 : (N <= 64 ? 2 : 4); }
 
 template <typename T>
-size_t eval_lds_bytes(const KSys& sy, bool per_particle)
+size_t eval_lds_bytes(const KSys& sy, bool per_particle, int wpw)
 {
     // the larger of the two layouts: histogram + real gradient sums (real
     // parameters), narrower histogram + real and imaginary sums (otherwise)
-    const size_t nd_real = (size_t)sy.P_s * 4 * hist_cols<false>() + (size_t)sy.N * sy.D;
-    const size_t nd_cplx = (size_t)sy.P_s * 4 * hist_cols<true>() + (size_t)sy.N * 2 * sy.D;
+    const size_t nd_real = (size_t)sy.P_s * 4 * hist_cols(false, wpw) + (size_t)sy.N * sy.D;
+    const size_t nd_cplx = (size_t)sy.P_s * 4 * hist_cols(true, wpw) + (size_t)sy.N * 2 * sy.D;
     const size_t nd = (per_particle ? nd_cplx + 2 * (size_t)sy.N : std::max(nd_real, nd_cplx));
     // coordinates (Coord storage has the size of T) and rho
     const size_t nt = (size_t)sy.N * sy.D + 2 * (size_t)sy.Kh;
@@ -813,8 +814,8 @@ hipError_t launch_eval_batch(const EvalBatch<T>& bt, int D, bool periodic, int64
 template hipError_t launch_eval_batch<float>(const EvalBatch<float>&, int, bool, int64_t, size_t, hipStream_t);
 template hipError_t launch_eval_batch<double>(const EvalBatch<double>&, int, bool, int64_t, size_t, hipStream_t);
 
-template size_t eval_lds_bytes<float>(const KSys&, bool);
-template size_t eval_lds_bytes<double>(const KSys&, bool);
+template size_t eval_lds_bytes<float>(const KSys&, bool, int);
+template size_t eval_lds_bytes<double>(const KSys&, bool, int);
 template hipError_t launch_eval<double, double>(const KSys&, const KTables<double>&, const double*, int64_t, double*,
                                                 double*, double*, double*, double*, double*, double*, double*, double,
                                                 const double*, hipStream_t);
