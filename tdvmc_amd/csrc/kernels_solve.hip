@@ -10,12 +10,15 @@
 // in :1749-1758).  Solver type 1 is the full-pivot QR kernel further down.
 //
 // Blocked right-looking Cholesky: panels of NB columns are staged in LDS
-// (transposed, so consecutive threads read consecutive rows), factored there
-// with two barriers per column, written back, and the trailing lower
-// triangle is updated by all 1024 threads (a rank-NB update with the panel
-// read from LDS).  The triangular solves are blocked the same way: one
-// wavefront solves each NB x NB diagonal block with lane exchanges, the rest
-// of the right-hand side is updated by the whole workgroup.  The factor lives
+// (transposed, so consecutive threads read consecutive rows).  One wavefront
+// factors the NB x NB diagonal block (wave-level ordering, no workgroup
+// barrier), one thread per row solves the rows below it against that block,
+// the panel is written back, and the trailing lower triangle is updated by
+// all 1024 threads in 4 x 4 register tiles (a rank-NB update with the panel
+// read from LDS; every tile's loads are issued before its stores).  The
+// triangular solves are blocked the same way: each NB x NB diagonal block is
+// staged in LDS and solved by one wavefront with lane exchanges, the rest of
+// the right-hand side is updated by the whole workgroup.  The factor lives
 // column-major in the work buffer (L2-resident for P <= 600).
 
 #include <algorithm>
@@ -45,15 +48,17 @@ template <int NB>
 __device__ __forceinline__ void solve_body(const SolveArgs& a)
 {
     const int P = a.P, tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
     double* Lm = a.work;                 // column-major P x P: Lm[j * P + i] = L[i][j]
     double* bR = a.work + (size_t)P * P;
     double* bI = bR + P;
     double* sc = bI + P;
     double* dorig = sc + P;              // diagonal after scaling and the eps shift
     extern __shared__ __align__(16) unsigned char smem[];
-    double* pan = reinterpret_cast<double*>(smem);   // [NB][rows] panel, transposed
+    double* pan = reinterpret_cast<double*>(smem);   // [NB][rows] panel, transposed; [NB][NB] blocks in the solves
     __shared__ double red[kThreads / 64];
-    __shared__ double piv;
+    __shared__ double ipiv[NB];
+    __shared__ double ldc[NB * NB];      // factored diagonal block, column-major: ldc[c * NB + k] = L[k][c]
     __shared__ int nbad;
     const double* mO = a.mean;
     const double eR = a.mean[P], eI = a.mean[P + 1];
@@ -72,15 +77,28 @@ __device__ __forceinline__ void solve_body(const SolveArgs& a)
         sc[i] = a.precondition ? sqrt(a.S[(size_t)i * P + i]) : 1.0;
     }
     __syncthreads();
-    for (size_t e = tid; e < (size_t)P * P; e += kThreads) {
-        const int i = (int)(e % P), j = (int)(e / P);
-        double v = a.S[(size_t)i * P + j];
-        if (a.precondition) v /= (sc[i] * sc[j]);
-        if (i == j) {
-            v += a.eps;
-            dorig[i] = v;
+    // S is symmetric: read it column by column (coalesced), four entries in flight per thread
+    const size_t PP = (size_t)P * P;
+    for (size_t e0 = tid; e0 < PP; e0 += 4 * kThreads) {
+        double v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const size_t e = e0 + (size_t)q * kThreads;
+            v[q] = e < PP ? a.S[e] : 0.0;
         }
-        Lm[e] = v;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const size_t e = e0 + (size_t)q * kThreads;
+            if (e >= PP) continue;
+            const int i = (int)(e % P), j = (int)(e / P);
+            double x = v[q];
+            if (a.precondition) x /= (sc[i] * sc[j]);
+            if (i == j) {
+                x += a.eps;
+                dorig[i] = x;
+            }
+            Lm[e] = x;
+        }
     }
     for (int i = tid; i < P; i += kThreads) {
         if (a.precondition) {
@@ -99,70 +117,144 @@ __device__ __forceinline__ void solve_body(const SolveArgs& a)
             pan[c * rows + r] = Lm[(size_t)(p0 + c) * P + p0 + r];
         }
         __syncthreads();
-        for (int c = 0; c < nb; c++) {
-            double* col = pan + c * rows;
-            if (tid == 0) {
-                const double d = col[c];
-                if (d > 0.0) {
-                    piv = sqrt(d);
-                } else {
-                    piv = dorig[p0 + c];
-                    nbad++;
+        // diagonal block by one wavefront in registers (lane l holds row l;
+        // column entries reach the other lanes by v_readlane), then a
+        // column-major copy of it (ldc) for the rows below
+        if (wv == 0) {
+            double av[NB];
+#pragma unroll
+            for (int k = 0; k < NB; k++) av[k] = (k < nb && lane < nb) ? pan[k * rows + lane] : 0.0;
+#pragma unroll
+            for (int c = 0; c < NB; c++) {
+                if (c < nb) {
+                    const double d = readlane_f(av[c], c);
+                    double pj;
+                    if (d > 0.0) {
+                        pj = sqrt(d);
+                    } else {
+                        pj = dorig[p0 + c];
+                        if (lane == 0) nbad++;
+                    }
+                    const double ipj = 1.0 / pj;
+                    if (lane > c)
+                        av[c] *= ipj;
+                    else if (lane == c)
+                        av[c] = pj;
+                    if (lane == 0) ipiv[c] = ipj;
+#pragma unroll
+                    for (int k = c + 1; k < NB; k++) {
+                        if (k < nb) {
+                            const double lkc = readlane_f(av[c], k);
+                            if (lane >= k) av[k] -= av[c] * lkc;
+                        }
+                    }
                 }
             }
-            __syncthreads();
-            const double pj = piv, ipj = 1.0 / pj;
-            // update the panel's later columns with the (still unscaled) column c
-            const int ncols = nb - c - 1;
-            if (ncols > 0) {
-                const int span = rows - c - 1;     // rows below the pivot
-                for (int e = tid; e < ncols * span; e += kThreads) {
-                    const int cc = c + 1 + e / span;
-                    const int r = c + 1 + e % span;
-                    if (r >= cc) pan[cc * rows + r] -= (col[r] * ipj) * (col[cc] * ipj);
+            if (lane < nb) {
+#pragma unroll
+                for (int k = 0; k < NB; k++) {
+                    if (k <= lane && k < nb) {
+                        pan[k * rows + lane] = av[k];
+                        ldc[k * NB + lane] = av[k];
+                    }
                 }
             }
-            __syncthreads();
-            for (int r = c + 1 + tid; r < rows; r += kThreads) col[r] *= ipj;
-            if (tid == 0) col[c] = pj;
-            // (the next column's pivot thread reads only its own diagonal entry)
+        }
+        __syncthreads();
+        // rows below the block, one thread per row, right-looking within the
+        // row: L[r][c] = v_c / L[c][c], then v_k -= L[r][c] L[k][c] for k > c
+        for (int r = nb + tid; r < rows; r += kThreads) {
+            double v[NB];
+#pragma unroll
+            for (int k = 0; k < NB; k++) v[k] = k < nb ? pan[k * rows + r] : 0.0;
+#pragma unroll
+            for (int c = 0; c < NB; c++) {
+                if (c < nb) {
+                    const double xc = v[c] * ipiv[c];
+                    pan[c * rows + r] = xc;
+                    const double* lcol = ldc + c * NB;   // L[k][c]
+#pragma unroll
+                    for (int k = c + 1; k < NB; k++)
+                        if (k < nb) v[k] -= xc * lcol[k];
+                }
+                // keeps the 32 columns' table reads from being hoisted together (register spills)
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
         __syncthreads();
         for (int e = tid; e < nb * rows; e += kThreads) {
             const int c = e / rows, r = e % rows;
             if (r >= c) Lm[(size_t)(p0 + c) * P + p0 + r] = pan[c * rows + r];
         }
-        // trailing update of the lower triangle: A[i][k] -= sum_c L[i][c] L[k][c]
-        const int T = P - p0 - nb;
+        // trailing update of the lower triangle, A[i][k] -= sum_c L[i][c] L[k][c],
+        // in 4 x 4 tiles (lower-triangle tiles only), loads issued before stores
+        const int T = rows - nb;
         if (T > 0) {
-            for (int e = tid; e < T * T; e += kThreads) {
-                const int kk = e / T, ii = e % T;
-                if (ii < kk) continue;
-                const int rk = nb + kk, ri = nb + ii;    // row offsets inside the panel
-                double s = 0.0;
-#pragma unroll 8
-                for (int c = 0; c < nb; c++) s += pan[c * rows + ri] * pan[c * rows + rk];
-                Lm[(size_t)(p0 + nb + kk) * P + p0 + nb + ii] -= s;
+            const int nt = (T + 3) / 4;
+            const int ntiles = nt * (nt + 1) / 2;
+            for (int t = tid; t < ntiles; t += kThreads) {
+                int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+                while (ti * (ti + 1) / 2 > t) ti--;
+                while ((ti + 1) * (ti + 2) / 2 <= t) ti++;
+                const int tk = t - ti * (ti + 1) / 2;
+                const int i0 = nb + 4 * ti, k0 = nb + 4 * tk;   // row offsets inside the panel
+                double acc[4][4];
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+#pragma unroll
+                    for (int u = 0; u < 4; u++) acc[q][u] = 0.0;
+                for (int c = 0; c < nb; c++) {
+                    double li[4], lk[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        li[q] = i0 + q < rows ? pan[c * rows + i0 + q] : 0.0;
+                        lk[q] = k0 + q < rows ? pan[c * rows + k0 + q] : 0.0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+#pragma unroll
+                        for (int u = 0; u < 4; u++) acc[q][u] += li[q] * lk[u];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int ri = i0 + q;
+                    double old[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int rk = k0 + u;
+                        old[u] = (ri < rows && ri >= rk) ? Lm[(size_t)(p0 + rk) * P + p0 + ri] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int rk = k0 + u;
+                        if (ri < rows && ri >= rk) Lm[(size_t)(p0 + rk) * P + p0 + ri] = old[u] - acc[q][u];
+                    }
+                }
             }
         }
         __syncthreads();
     }
 
     // ---------------------------------------------------------------- L t = b
-    const int lane = tid & 63, wv = tid >> 6;
     for (int p0 = 0; p0 < P; p0 += NB) {
         const int nb = min(NB, P - p0);
+        // diagonal block into LDS: pan[j * NB + l] = L[p0 + l][p0 + j]
+        for (int e = tid; e < nb * nb; e += kThreads) {
+            const int j = e / nb, l = e % nb;
+            pan[j * NB + l] = Lm[(size_t)(p0 + j) * P + p0 + l];
+        }
+        __syncthreads();
         if (wv == 0) {
-            // diagonal block by one wavefront: lane l owns row p0 + l
+            // lane l owns row p0 + l
             double tR = lane < nb ? bR[p0 + lane] : 0.0, tI = lane < nb ? bI[p0 + lane] : 0.0;
             for (int j = 0; j < nb; j++) {
-                const double ljj = Lm[(size_t)(p0 + j) * P + p0 + j];
+                const double ljj = pan[j * NB + j];
                 const double xR = __shfl(tR, j, 64) / ljj, xI = __shfl(tI, j, 64) / ljj;
                 if (lane == j) {
                     tR = xR;
                     tI = xI;
                 } else if (lane > j && lane < nb) {
-                    const double l = Lm[(size_t)(p0 + j) * P + p0 + lane];
+                    const double l = pan[j * NB + lane];
                     tR -= l * xR;
                     tI -= l * xI;
                 }
@@ -175,6 +267,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs& a)
         __syncthreads();
         for (int i = p0 + nb + tid; i < P; i += kThreads) {
             double sR = 0.0, sI = 0.0;
+#pragma unroll 8
             for (int j = 0; j < nb; j++) {
                 const double l = Lm[(size_t)(p0 + j) * P + i];
                 sR += l * bR[p0 + j];
@@ -207,17 +300,22 @@ __device__ __forceinline__ void solve_body(const SolveArgs& a)
                 bI[i] -= sI;
             }
         }
+        // diagonal block into LDS: pan[j * NB + l] = L[p0 + l][p0 + j]
+        for (int e = tid; e < nb * nb; e += kThreads) {
+            const int j = e / nb, l = e % nb;
+            pan[j * NB + l] = Lm[(size_t)(p0 + j) * P + p0 + l];
+        }
         __syncthreads();
         if (wv == 0) {
             double tR = lane < nb ? bR[p0 + lane] : 0.0, tI = lane < nb ? bI[p0 + lane] : 0.0;
             for (int j = nb - 1; j >= 0; j--) {
-                const double ljj = Lm[(size_t)(p0 + j) * P + p0 + j];
+                const double ljj = pan[j * NB + j];
                 const double xR = __shfl(tR, j, 64) / ljj, xI = __shfl(tI, j, 64) / ljj;
                 if (lane == j) {
                     tR = xR;
                     tI = xI;
                 } else if (lane < j) {
-                    const double l = Lm[(size_t)(p0 + lane) * P + p0 + j];   // L[j][lane]
+                    const double l = pan[lane * NB + j];   // L[j][lane]
                     tR -= l * xR;
                     tI -= l * xI;
                 }
