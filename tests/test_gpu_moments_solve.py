@@ -79,7 +79,7 @@ def test_moments_bf16x3_small_sample():
 
 
 @pytest.mark.parametrize("imag,pre", [(0, 0), (1, 0), (0, 1), (1, 1)])
-@pytest.mark.parametrize("P", [12, 296])
+@pytest.mark.parametrize("P", [12, 33, 296, 560])       # last panel of 12, 1, 8, 16 columns
 def test_solve_vs_oracle(imag, pre, P):
     rng = np.random.default_rng(P + 10 * imag + pre)
     A = rng.normal(size=(P, P))
@@ -94,6 +94,25 @@ def test_solve_vs_oracle(imag, pre, P):
     assert fro(r["xI"], ref["xI"]) < 1e-10
     assert abs(r["phiDotR"] - ref["phiDotR"]) < 1e-10 * max(1, abs(ref["phiDotR"]))
     assert abs(r["phiDotI"] - ref["phiDotI"]) < 1e-10 * max(1, abs(ref["phiDotI"]))
+
+
+def test_solve_nonpositive_pivot_in_later_panel():
+    """A non-positive pivot in the second 32-column panel (and in the
+    partial last one) is counted and handled as in the reference."""
+    P = 70
+    rng = np.random.default_rng(7)
+    A = rng.normal(size=(P, P))
+    S = A @ A.T / P + 0.5 * np.eye(P)
+    for i in (40, 66):
+        S[i, :] = 0.0
+        S[:, i] = 0.0
+        S[i, i] = -0.5
+    m = dict(mO=rng.normal(size=P), mE=np.array([0.3, 0.0]), S=S, FR=rng.normal(size=P), FI=np.zeros(P))
+    with _ctx() as ctx:
+        r = ctx.solve(m["mO"], m["mE"], S, m["FR"], m["FI"], imag_time=1, eps=1e-3, check=False)
+    ref = o.solve_param_dot(m, imag_time=1, eps=1e-3)
+    assert r["bad"] == ref["bad"] == 2
+    assert np.allclose(r["xR"], ref["xR"], rtol=1e-10, atol=1e-12)
 
 
 def test_solve_nonpositive_pivot_flagged():
