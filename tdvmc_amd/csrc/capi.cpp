@@ -34,6 +34,7 @@ struct DevTables {
     T* dspl = nullptr;
     T* kv = nullptr;
     T* alpha = nullptr;
+    T* kag = nullptr;
     bool dirty = true;
 };
 
@@ -417,6 +418,7 @@ KTables<T> tables_of(tdvmc_ctx* ctx, DevTables<T>& t)
     k.dspl = t.dspl;
     k.kv = t.kv;
     k.alpha = t.alpha;
+    k.kag = t.kag;
     k.kmap = ctx->dKmap;
     k.uR = ctx->dUR;
     k.uI = ctx->dUI;
@@ -427,7 +429,8 @@ template <typename T>
 int ensure_tables(tdvmc_ctx* ctx, DevTables<T>& t, bool force)
 {
     if (!t.dirty && !force) return TDVMC_OK;
-    HIPC(launch_prep_tables<T>(ctx->sy, ctx->dUR, ctx->dUI, ctx->dKmap, t.spl, t.dspl, t.alpha, ctx->stream));
+    HIPC(launch_prep_tables<T>(ctx->sy, ctx->dUR, ctx->dUI, ctx->dKmap, t.kv, t.spl, t.dspl, t.alpha, t.kag,
+                               ctx->stream));
     t.dirty = false;
     return TDVMC_OK;
 }
@@ -821,9 +824,11 @@ int alloc_ctx(tdvmc_ctx* ctx)
     HIPC(dalloc(&ctx->td.dspl, (size_t)Ps * kDsplStride));
     HIPC(dalloc(&ctx->tf.kv, 4 * (size_t)K));
     HIPC(dalloc(&ctx->tf.alpha, 2 * (size_t)K));
+    HIPC(dalloc(&ctx->tf.kag, 8 * (size_t)K));
     HIPC(dalloc(&ctx->td.spl, (size_t)(Ps - 1) * 8));
     HIPC(dalloc(&ctx->td.kv, 4 * (size_t)K));
     HIPC(dalloc(&ctx->td.alpha, 2 * (size_t)K));
+    HIPC(dalloc(&ctx->td.kag, 8 * (size_t)K));
     HIPC(hipMemcpyAsync(ctx->tf.kv, ctx->kv_f.data(), ctx->kv_f.size() * sizeof(float), hipMemcpyHostToDevice,
                         ctx->stream));
     HIPC(hipMemcpyAsync(ctx->td.kv, ctx->kv_d.data(), ctx->kv_d.size() * sizeof(double), hipMemcpyHostToDevice,
@@ -866,9 +871,11 @@ void release(tdvmc_ctx* ctx)
     dfree(ctx->td.dspl);
     dfree(ctx->tf.kv);
     dfree(ctx->tf.alpha);
+    dfree(ctx->tf.kag);
     dfree(ctx->td.spl);
     dfree(ctx->td.kv);
     dfree(ctx->td.alpha);
+    dfree(ctx->td.kag);
     dfree(ctx->dO64);
     dfree(ctx->dO32);
     dfree(ctx->dER);
@@ -1019,9 +1026,11 @@ int batch_upload(tdvmc_batch* b, int64_t n_moves, uint64_t seed, bool want_rho)
         sy[s] = c->sy;
         tf[s] = tables_of(c, c->tf);
         td[s] = tables_of(c, c->td);
-        bpf[s] = BatchParams<float>{c->sy, c->dUR, c->dUI, c->dKmap, c->tf.spl, c->tf.dspl, c->tf.alpha, c->dX,
+        bpf[s] = BatchParams<float>{c->sy, c->dUR, c->dUI, c->dKmap, c->tf.kv, c->tf.spl, c->tf.dspl, c->tf.alpha,
+                                    c->tf.kag, c->dX,
                                     b->xoff[s]};
-        bpd[s] = BatchParams<double>{c->sy, c->dUR, c->dUI, c->dKmap, c->td.spl, c->td.dspl, c->td.alpha, c->dX,
+        bpd[s] = BatchParams<double>{c->sy, c->dUR, c->dUI, c->dKmap, c->td.kv, c->td.spl, c->td.dspl, c->td.alpha,
+                                     c->td.kag, c->dX,
                                      b->xoff[s]};
         const int lanes = batch_sys_lanes(b, s);
         const int wpb = sweep_walkers_per_block(lanes);

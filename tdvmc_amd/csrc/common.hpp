@@ -41,6 +41,7 @@ struct KTables {
                       //   and u''(rc) / hinv as d1, d2 (used at t = 0 only)
     const T* kv;      // [Kh][4]: n/L of the representative (phase in revolutions = kv . r), |k|^2
     const T* alpha;   // [Kh][2]: (Re, Im) of alpha_k (+ alpha_{-k} when paired)
+    const T* kag;     // [Kh][8]: Re alpha_k times (kv, |k|^2), then Im alpha_k times the same
     const int* kmap;  // [Kh][2]: parameter slots (relative to off_k) of k and of -k (-1: unpaired)
     const double* uR; // [P] parameters (FP64 master copy)
     const double* uI;

@@ -10,8 +10,8 @@ namespace tdvmc {
 
 // ---- parameter-derived tables (kernels_misc.hip)
 template <typename T>
-hipError_t launch_prep_tables(const KSys& sy, const double* uR, const double* uI, const int* kmap, T* spl, T* dspl,
-                              T* alpha, hipStream_t st);
+hipError_t launch_prep_tables(const KSys& sy, const double* uR, const double* uI, const int* kmap, const T* kv, T* spl,
+                              T* dspl, T* alpha, T* kag, hipStream_t st);
 
 // ---- evaluation (kernels_eval.hip); grad/lap non-null selects the per-particle variant
 // LDS of one walker with wpw wavefronts (single contexts: 4)
@@ -130,9 +130,11 @@ struct BatchParams {
     double* uR;                // [P] this system's parameters (device)
     double* uI;
     const int* kmap;
+    const T* kv;
     T* spl;
     T* dspl;
     T* alpha;
+    T* kag;
     double* xsys;              // [2P+2] this system's packed parameters
     int64_t xoff;              // offset of this system in the concatenated vector
 };

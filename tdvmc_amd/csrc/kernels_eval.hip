@@ -349,6 +349,11 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
 
 // k-space gradient and Laplacian of sum_k alpha_k |rho_k|^2 for every particle
 // (lane = particle, the k range split over the walker's NW waves).
+//   grad_n |rho_k|^2 = -2 k Im(e^{ikr} conj rho),  k = 2 pi kv,
+//   lap_n  |rho_k|^2 = 2 |k|^2 (1 - Re(e^{ikr} conj rho)),
+// with the amplitudes folded into kag (alpha kv, alpha |k|^2): per (particle,
+// k) one phase, one sincos, the two products with conj rho and four FMAs; the
+// constant 2 sum_k alpha |k|^2 of the Laplacian is added once.
 template <typename T, int D, bool PP, bool PER, bool CPLX, int NW>
 __device__ __forceinline__ void eval_kgrad(const KSys& sy, const KTables<T>& tb, const typename Coord<T, PER>::S* pos,
                                            const T* rho, const T* kq, double* gsum, double* lsum, int wave, int lane,
@@ -358,6 +363,12 @@ __device__ __forceinline__ void eval_kgrad(const KSys& sy, const KTables<T>& tb,
     const int N = sy.N, K = sy.Kh;
     const int kchunk = (K + NW - 1) / NW;
     const int k_lo = wave * kchunk, k_hi = min(K, (wave + 1) * kchunk);
+    // sum over this wave's k of alpha |k|^2 (the constant part of the Laplacian)
+    T cR = (T)0, cI = (T)0;
+    for (int k = k_lo; k < k_hi; k++) {
+        cR += tb.kag[8 * k + 3];
+        if (CPLX) cI += tb.kag[8 * k + 7];
+    }
     for (int rb = 0; rb < N; rb += 64) {
         const int n = rb + lane;
         if (n >= N) continue;
@@ -367,12 +378,9 @@ __device__ __forceinline__ void eval_kgrad(const KSys& sy, const KTables<T>& tb,
         T gR[D], gI[D];
 #pragma unroll
         for (int a = 0; a < D; a++) gR[a] = gI[a] = (T)0;
-        T lR = (T)0, lI = (T)0;
-        // grad |rho_k|^2 = -2 k Im(e^{ikr} conj rho), k = 2 pi kv;
-        // lap |rho_k|^2 = 2 |k|^2 (1 - Re(e^{ikr} conj rho)); the
-        // constant factors are applied after the loop
+        T lR = (T)0, lI = (T)0;   // - sum alpha |k|^2 Re(e^{ikr} conj rho)
         for (int k = k_lo; k < k_hi; k++) {
-            const T* q = tb.kv + k * 4;
+            const T* ka = tb.kag + 8 * k;
             const T* qs = kq + k * 4;
             T ph = (T)0;
 #pragma unroll
@@ -382,18 +390,13 @@ __device__ __forceinline__ void eval_kgrad(const KSys& sy, const KTables<T>& tb,
             const T rr = rho[2 * k], ri = rho[2 * k + 1];
             const T pre = cs * rr + sn * ri;    // Re(e^{ikr} conj(rho))
             const T pim = sn * rr - cs * ri;    // Im(e^{ikr} conj(rho))
-            const T aR = tb.alpha[2 * k];
-            const T sR = aR * pim;
 #pragma unroll
-            for (int a = 0; a < D; a++) gR[a] += sR * q[a];
-            const T lk = q[3] * ((T)1 - pre);
-            lR += aR * lk;
+            for (int a = 0; a < D; a++) gR[a] += pim * ka[a];
+            lR -= pre * ka[3];
             if constexpr (CPLX) {
-                const T aI = tb.alpha[2 * k + 1];
-                const T sI = aI * pim;
 #pragma unroll
-                for (int a = 0; a < D; a++) gI[a] += sI * q[a];
-                lI += aI * lk;
+                for (int a = 0; a < D; a++) gI[a] += pim * ka[4 + a];
+                lI -= pre * ka[7];
             }
         }
 #pragma unroll
@@ -401,12 +404,13 @@ __device__ __forceinline__ void eval_kgrad(const KSys& sy, const KTables<T>& tb,
             lds_add(gsum + a * N + n, (double)(gR[a] * (T)(-2.0 * 6.283185307179586)));
             if (CPLX) lds_add(gsum + (D + a) * N + n, (double)(gI[a] * (T)(-2.0 * 6.283185307179586)));
         }
+        const double lapR = 2.0 * ((double)cR + (double)lR), lapI = 2.0 * ((double)cI + (double)lI);
         if (PP) {
-            lds_add(lsum + n, (double)(lR * (T)2));
-            if (CPLX) lds_add(lsum + N + n, (double)(lI * (T)2));
+            lds_add(lsum + n, lapR);
+            if (CPLX) lds_add(lsum + N + n, lapI);
         } else {
-            lap_tot_R += (double)(lR * (T)2);
-            if (CPLX) lap_tot_I += (double)(lI * (T)2);
+            lap_tot_R += lapR;
+            if (CPLX) lap_tot_I += lapI;
         }
     }
 }

@@ -22,8 +22,8 @@ __device__ double spline_coeff(const double* u, int Ps, int s)
 
 template <typename T>
 __device__ void prep_tables_body(const KSys& sy, const double* __restrict__ uR, const double* __restrict__ uI,
-                                 const int* __restrict__ kmap, T* __restrict__ spl, T* __restrict__ dspl,
-                                 T* __restrict__ alpha, int j0, int jstep)
+                                 const int* __restrict__ kmap, const T* __restrict__ kv, T* __restrict__ spl,
+                                 T* __restrict__ dspl, T* __restrict__ alpha, T* __restrict__ kag, int j0, int jstep)
 {
     const int Ps = sy.P_s;
     for (int j = j0; j < Ps - 1 + sy.Kh; j += jstep) {
@@ -68,16 +68,21 @@ __device__ void prep_tables_body(const KSys& sy, const double* __restrict__ uR, 
             }
             alpha[2 * k] = (T)aR;
             alpha[2 * k + 1] = (T)aI;
+            // amplitude times (n / L, |k|^2) for the k-space gradient and Laplacian
+            for (int m = 0; m < 4; m++) {
+                kag[8 * k + m] = (T)(aR * (double)kv[4 * k + m]);
+                kag[8 * k + 4 + m] = (T)(aI * (double)kv[4 * k + m]);
+            }
         }
     }
 }
 
 template <typename T>
 __global__ void prep_tables_kernel(KSys sy, const double* __restrict__ uR, const double* __restrict__ uI,
-                                   const int* __restrict__ kmap, T* __restrict__ spl, T* __restrict__ dspl,
-                                   T* __restrict__ alpha)
+                                   const int* __restrict__ kmap, const T* __restrict__ kv, T* __restrict__ spl,
+                                   T* __restrict__ dspl, T* __restrict__ alpha, T* __restrict__ kag)
 {
-    prep_tables_body(sy, uR, uI, kmap, spl, dspl, alpha, threadIdx.x + blockIdx.x * blockDim.x,
+    prep_tables_body(sy, uR, uI, kmap, kv, spl, dspl, alpha, kag, threadIdx.x + blockIdx.x * blockDim.x,
                      blockDim.x * gridDim.x);
 }
 
@@ -86,7 +91,7 @@ template <typename T>
 __global__ void prep_tables_batch_kernel(const BatchParams<T>* __restrict__ bp)
 {
     const BatchParams<T> b = bp[blockIdx.x];
-    prep_tables_body(b.sy, b.uR, b.uI, b.kmap, b.spl, b.dspl, b.alpha, threadIdx.x, blockDim.x);
+    prep_tables_body(b.sy, b.uR, b.uI, b.kmap, b.kv, b.spl, b.dspl, b.alpha, b.kag, threadIdx.x, blockDim.x);
 }
 
 // x (concatenated packed parameters) -> per-system uR, uI
@@ -306,18 +311,19 @@ hipError_t launch_adjoint_outer(const double* outer, const double* gbar, int64_t
 }
 
 template <typename T>
-hipError_t launch_prep_tables(const KSys& sy, const double* uR, const double* uI, const int* kmap, T* spl, T* dspl,
-                              T* alpha, hipStream_t st)
+hipError_t launch_prep_tables(const KSys& sy, const double* uR, const double* uI, const int* kmap, const T* kv, T* spl,
+                              T* dspl, T* alpha, T* kag, hipStream_t st)
 {
     const int n = sy.P_s - 1 + sy.Kh;
     const int blocks = (n + 255) / 256;
-    hipLaunchKernelGGL(prep_tables_kernel<T>, dim3(blocks), dim3(256), 0, st, sy, uR, uI, kmap, spl, dspl, alpha);
+    hipLaunchKernelGGL(prep_tables_kernel<T>, dim3(blocks), dim3(256), 0, st, sy, uR, uI, kmap, kv, spl, dspl, alpha,
+                       kag);
     return hipGetLastError();
 }
-template hipError_t launch_prep_tables<float>(const KSys&, const double*, const double*, const int*, float*, float*,
-                                              float*, hipStream_t);
-template hipError_t launch_prep_tables<double>(const KSys&, const double*, const double*, const int*, double*,
-                                               double*, double*, hipStream_t);
+template hipError_t launch_prep_tables<float>(const KSys&, const double*, const double*, const int*, const float*,
+                                              float*, float*, float*, float*, hipStream_t);
+template hipError_t launch_prep_tables<double>(const KSys&, const double*, const double*, const int*, const double*,
+                                               double*, double*, double*, double*, hipStream_t);
 
 hipError_t launch_axpy_params(int P, const double* x0, const double* xdot, double c, double* y, hipStream_t st)
 {
