@@ -235,13 +235,13 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
             S old[D], nw[D];
 #pragma unroll
             for (int c = 0; c < D; c++) {
-                old[c] = pos[p * D + c];
+                old[c] = pos[__umul24((uint32_t)p, (uint32_t)D) + c];
                 nw[c] = C::add(old[c], bcast<LPW>(dl[c], tt));
             }
             const T lu = bcast<LPW>(logu, tt);
             const T us = bcast<LPW>(uself, tt);
             T delta = (T)0;
-#pragma unroll 2
+#pragma unroll 1
             for (int i = gl; i < N; i += LPW) {
                 T ro2 = (T)0, rn2 = (T)0;
 #pragma unroll
@@ -282,16 +282,14 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
                 // |rho + d|^2 - |rho|^2 = 2 Re(conj(rho) d) + |d|^2
                 delta += qa[m] * ((T)2 * (rhoR[m] * dR[m] + rhoI[m] * dI[m]) + dR[m] * dR[m] + dI[m] * dI[m]);
             }
-            if (sy.one_body) {
-                if (gl == 0) {
-                    T ro2 = (T)0, rn2 = (T)0;
+            if (sy.one_body && gl == 0) {
+                T ro2 = (T)0, rn2 = (T)0;
 #pragma unroll
-                    for (int c = 0; c < D; c++) {
-                        ro2 += oc[c] * oc[c];
-                        rn2 += ncd[c] * ncd[c];
-                    }
-                    delta += -ob * (rn2 - ro2);
+                for (int c = 0; c < D; c++) {
+                    ro2 += oc[c] * oc[c];
+                    rn2 += ncd[c] * ncd[c];
                 }
+                delta += -ob * (rn2 - ro2);
             }
             delta = group_sum<LPW>(delta) - us;
             const bool accept = ((T)2 * delta >= lu);
