@@ -29,7 +29,8 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef double double4v __attribute__((ext_vector_type(4)));
 
 constexpr int kColBlock = 256;
-constexpr int kMaxChunks = 256;
+constexpr int kChunkRows = 64;     // walkers per column-sum chunk (fewer only past kMaxChunks chunks)
+constexpr int kMaxChunks = 1024;
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
@@ -58,7 +59,7 @@ Geometry geometry(int64_t W, int P, int mode)
     if (groups > cap) groups = cap;
     if (groups < 1) groups = 1;
     g.groups = (int)groups;
-    int64_t ch = ceil_div(W, 256);
+    int64_t ch = ceil_div(W, kChunkRows);
     g.chunks = (int)(ch > kMaxChunks ? kMaxChunks : (ch < 1 ? 1 : ch));
     return g;
 }
@@ -75,22 +76,30 @@ __global__ void colsum_kernel(const TO* __restrict__ O, const double* __restrict
     const int64_t w0 = ch * rows, w1 = min(W, w0 + rows);
     if (c >= Pa) return;
     double s = 0.0;
-    if (c < P)
+    if (c < P) {
+        // independent loads in flight; the sum keeps walker order
+#pragma unroll 8
         for (int64_t w = w0; w < w1; w++) s += (double)O[w * P + c];
-    else if (c == P)
+    } else if (c == P)
         for (int64_t w = w0; w < w1; w++) s += ER[w];
     else
         for (int64_t w = w0; w < w1; w++) s += EI[w];
     part[(size_t)ch * Pa + c] = s;
 }
 
-__global__ void colmean_kernel(const double* __restrict__ part, int chunks, int Pa, int64_t W, double* __restrict__ mean)
+// one workgroup of 256 threads per column: strided partial sums, then a fixed
+// reduction tree (wavefront butterflies, then the 4 wavefronts in order)
+__global__ __launch_bounds__(256) void colmean_kernel(const double* __restrict__ part, int chunks, int Pa, int64_t W,
+                                                      double* __restrict__ mean)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= Pa) return;
+    __shared__ double red[4];
+    const int c = blockIdx.x, tid = threadIdx.x;
     double s = 0.0;
-    for (int ch = 0; ch < chunks; ch++) s += part[(size_t)ch * Pa + c];
-    mean[c] = s / (double)W;
+    for (int ch = tid; ch < chunks; ch += 256) s += part[(size_t)ch * Pa + c];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) mean[c] = (((red[0] + red[1]) + red[2]) + red[3]) / (double)W;
 }
 
 __device__ __forceinline__ double xval(const float* O, const double* ER, const double* EI, int64_t W, int P,
@@ -150,17 +159,18 @@ __global__ void pack_bf16_kernel(const TO* __restrict__ O, const double* __restr
         const int cb = (int)(rest / S16);
         const int c = 32 * cb + l32;
         __bf16* base = xt + ((((size_t)cb * S16 + s) * 2) * 64) * 8;
-        __bf16* hi = base + ((size_t)h * 32 + l32) * 8;
-        __bf16* lo = base + ((size_t)64 + h * 32 + l32) * 8;
+        // 8 walkers of hi and of lo: one 16-byte store each
+        bf16x8 vh, vl;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int64_t w = 16 * s + 8 * h + j;
             const float x = (float)xval(O, ER, EI, W, P, mean, w, c);
             const __bf16 xh = (__bf16)x;
-            const __bf16 xl = (__bf16)(x - (float)xh);
-            hi[j] = xh;
-            lo[j] = xl;
+            vh[j] = xh;
+            vl[j] = (__bf16)(x - (float)xh);
         }
+        *reinterpret_cast<bf16x8*>(base + ((size_t)h * 32 + l32) * 8) = vh;
+        *reinterpret_cast<bf16x8*>(base + ((size_t)64 + h * 32 + l32) * 8) = vl;
     }
 }
 
@@ -288,8 +298,7 @@ hipError_t launch_moments(const TO* O, const double* ER, const double* EI, int64
 
     hipLaunchKernelGGL(colsum_kernel<TO>, dim3((unsigned)ceil_div(g.Pa, kColBlock), g.chunks), dim3(kColBlock), 0, st,
                        O, ER, EI, W, P, g.chunks, wk.colsum);
-    hipLaunchKernelGGL(colmean_kernel, dim3((unsigned)ceil_div(g.Pa, 256)), dim3(256), 0, st, wk.colsum, g.chunks,
-                       g.Pa, W, mean);
+    hipLaunchKernelGGL(colmean_kernel, dim3((unsigned)g.Pa), dim3(256), 0, st, wk.colsum, g.chunks, g.Pa, W, mean);
     if (mode == 0) {
         const int64_t total = g.Wp * g.Pp;
         int64_t blocks = ceil_div(total, 256);
