@@ -9,8 +9,8 @@
 // (phi-dot, computed on the scaled solution before the scaling is undone, as
 // in :1749-1758).  Solver type 1 is the full-pivot QR kernel further down.
 //
-// Blocked right-looking Cholesky: panels of NB columns are staged in LDS
-// (transposed, so consecutive threads read consecutive rows).  One wavefront
+// Blocked right-looking Cholesky: the rows below each NB-column diagonal
+// block are staged in LDS (row-major at a padded stride).  One wavefront
 // factors the NB x NB diagonal block (wave-level ordering, no workgroup
 // barrier), one thread per row solves the rows below it against that block,
 // the panel is written back, and the trailing lower triangle is updated by
@@ -19,7 +19,7 @@
 // triangular solves are blocked the same way: each NB x NB diagonal block is
 // staged in LDS and solved by one wavefront with lane exchanges, the rest of
 // the right-hand side is updated by the whole workgroup.  The factor lives
-// column-major in the work buffer (L2-resident for P <= 600).
+// column-major in the work buffer (L2-resident for P <= 581 with NB = 32).
 
 #include <algorithm>
 
@@ -55,10 +55,11 @@ __device__ __forceinline__ void solve_body(const SolveArgs& a)
     double* sc = bI + P;
     double* dorig = sc + P;              // diagonal after scaling and the eps shift
     extern __shared__ __align__(16) unsigned char smem[];
-    double* pan = reinterpret_cast<double*>(smem);   // [NB][rows] panel, transposed; [NB][NB] blocks in the solves
+    double* pan = reinterpret_cast<double*>(smem);   // [rows below][NB + 1] panel; [NB][NB] blocks in the solves
     __shared__ double red[kThreads / 64];
-    __shared__ double ipiv[NB];
-    __shared__ double ldc[NB * NB];      // factored diagonal block, column-major: ldc[c * NB + k] = L[k][c]
+    __shared__ double ipiv[NB];          // 1 / L[c][c]
+    __shared__ double dval[NB];          // L[c][c]
+    __shared__ double ldc[NB * NB];      // diagonal block, column-major: ldc[c * NB + k] = L[k][c]
     __shared__ int nbad;
     const double* mO = a.mean;
     const double eR = a.mean[P], eI = a.mean[P + 1];
@@ -109,106 +110,117 @@ __device__ __forceinline__ void solve_body(const SolveArgs& a)
     __syncthreads();
 
     // ---------------------------------------------------------------- Cholesky
+    // The rows below the diagonal block sit in LDS row-major with a padded
+    // stride (pan[rb * LDP + c], LDP = NB + 1): a thread's row is read with
+    // immediate offsets, and only threads rb and rb + 32 share a bank.  The
+    // diagonal block is staged on its own, column-major and padded with the
+    // identity to NB x NB (ldc[c * NB + l] = A[l][c]): padded columns have
+    // pivot 1 and zero off-diagonal entries and leave the real rows unchanged,
+    // so no step below has a run-time bound on the block.  Only the last
+    // block can be partial, and it has no rows below it.
+    constexpr int LDP = NB + 1;
     for (int p0 = 0; p0 < P; p0 += NB) {
         const int nb = min(NB, P - p0);
-        const int rows = P - p0;
-        for (int e = tid; e < nb * rows; e += kThreads) {
-            const int c = e / rows, r = e % rows;
-            pan[c * rows + r] = Lm[(size_t)(p0 + c) * P + p0 + r];
+        const int below = P - p0 - nb;
+        for (int e = tid; e < nb * below; e += kThreads) {
+            const int c = e / below, rb = e % below;
+            pan[rb * LDP + c] = Lm[(size_t)(p0 + c) * P + p0 + nb + rb];
+        }
+        for (int e = tid; e < NB * NB; e += kThreads) {
+            const int c = e / NB, l = e % NB;
+            ldc[e] = (c < nb && l < nb) ? Lm[(size_t)(p0 + c) * P + p0 + l] : (c == l ? 1.0 : 0.0);
         }
         __syncthreads();
-        // diagonal block by one wavefront in registers (lane l holds row l;
-        // column entries reach the other lanes by v_readlane), then a
-        // column-major copy of it (ldc) for the rows below
+        // diagonal block by one wavefront in registers: lane l holds row
+        // l mod NB, column entries reach the other lanes by v_readlane.  Every
+        // lane updates its whole row (the entries above the diagonal are
+        // never read), and L[c][c] is written from dval at the end.
         if (wv == 0) {
+            const int l = lane & (NB - 1);
             double av[NB];
 #pragma unroll
-            for (int k = 0; k < NB; k++) av[k] = (k < nb && lane < nb) ? pan[k * rows + lane] : 0.0;
+            for (int k = 0; k < NB; k++) av[k] = ldc[k * NB + l];
 #pragma unroll
             for (int c = 0; c < NB; c++) {
-                if (c < nb) {
-                    const double d = readlane_f(av[c], c);
-                    double pj;
-                    if (d > 0.0) {
-                        pj = sqrt(d);
-                    } else {
-                        pj = dorig[p0 + c];
-                        if (lane == 0) nbad++;
-                    }
-                    const double ipj = 1.0 / pj;
-                    if (lane > c)
-                        av[c] *= ipj;
-                    else if (lane == c)
-                        av[c] = pj;
-                    if (lane == 0) ipiv[c] = ipj;
-#pragma unroll
-                    for (int k = c + 1; k < NB; k++) {
-                        if (k < nb) {
-                            const double lkc = readlane_f(av[c], k);
-                            if (lane >= k) av[k] -= av[c] * lkc;
-                        }
-                    }
+                const double d = readlane_f(av[c], c);
+                double pj;
+                if (d > 0.0) {
+                    pj = sqrt(d);
+                } else {
+                    pj = dorig[p0 + c];
+                    if (lane == 0) nbad++;
                 }
+                const double ipj = 1.0 / pj;
+                av[c] *= ipj;
+                if (lane == 0) {
+                    ipiv[c] = ipj;
+                    dval[c] = pj;
+                }
+#pragma unroll
+                for (int k = c + 1; k < NB; k++) av[k] -= av[c] * readlane_f(av[c], k);
+                // one column's lane reads at a time (scalar registers)
+                __builtin_amdgcn_sched_barrier(0);
             }
-            if (lane < nb) {
+            if (lane < NB) {
 #pragma unroll
-                for (int k = 0; k < NB; k++) {
-                    if (k <= lane && k < nb) {
-                        pan[k * rows + lane] = av[k];
-                        ldc[k * NB + lane] = av[k];
-                    }
-                }
+                for (int k = 0; k < NB; k++) ldc[k * NB + lane] = av[k];
+                // same wavefront, after the writes above and lane 0's dval writes
+                ldc[lane * NB + lane] = dval[lane];
             }
         }
         __syncthreads();
         // rows below the block, one thread per row, right-looking within the
         // row: L[r][c] = v_c / L[c][c], then v_k -= L[r][c] L[k][c] for k > c
-        for (int r = nb + tid; r < rows; r += kThreads) {
+        for (int rb = tid; rb < below; rb += kThreads) {
+            double* pr = pan + rb * LDP;
             double v[NB];
 #pragma unroll
-            for (int k = 0; k < NB; k++) v[k] = k < nb ? pan[k * rows + r] : 0.0;
+            for (int k = 0; k < NB; k++) v[k] = pr[k];
 #pragma unroll
             for (int c = 0; c < NB; c++) {
-                if (c < nb) {
-                    const double xc = v[c] * ipiv[c];
-                    pan[c * rows + r] = xc;
-                    const double* lcol = ldc + c * NB;   // L[k][c]
+                const double xc = v[c] * ipiv[c];
+                pr[c] = xc;
+                const double* lcol = ldc + c * NB;   // L[k][c]
 #pragma unroll
-                    for (int k = c + 1; k < NB; k++)
-                        if (k < nb) v[k] -= xc * lcol[k];
-                }
+                for (int k = c + 1; k < NB; k++) v[k] -= xc * lcol[k];
                 // keeps the 32 columns' table reads from being hoisted together (register spills)
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
         __syncthreads();
-        for (int e = tid; e < nb * rows; e += kThreads) {
-            const int c = e / rows, r = e % rows;
-            if (r >= c) Lm[(size_t)(p0 + c) * P + p0 + r] = pan[c * rows + r];
+        for (int e = tid; e < nb * (nb + below); e += kThreads) {
+            const int c = e / (nb + below), r = e % (nb + below);
+            if (r >= c)
+                Lm[(size_t)(p0 + c) * P + p0 + r] = r < nb ? ldc[c * NB + r] : pan[(r - nb) * LDP + c];
         }
         // trailing update of the lower triangle, A[i][k] -= sum_c L[i][c] L[k][c],
-        // in 4 x 4 tiles (lower-triangle tiles only), loads issued before stores
-        const int T = rows - nb;
-        if (T > 0) {
-            const int nt = (T + 3) / 4;
+        // in 4 x 4 tiles (lower-triangle tiles only), loads issued before
+        // stores.  Rows past the end of the panel are read (inside the LDS
+        // allocation) but their sums are not stored.
+        if (below > 0) {
+            const int nt = (below + 3) / 4;
             const int ntiles = nt * (nt + 1) / 2;
+            const size_t q0 = (size_t)(p0 + nb);   // first trailing row / column
             for (int t = tid; t < ntiles; t += kThreads) {
                 int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
                 while (ti * (ti + 1) / 2 > t) ti--;
                 while ((ti + 1) * (ti + 2) / 2 <= t) ti++;
                 const int tk = t - ti * (ti + 1) / 2;
-                const int i0 = nb + 4 * ti, k0 = nb + 4 * tk;   // row offsets inside the panel
+                const int i0 = 4 * ti, k0 = 4 * tk;   // rows below the block
+                const double* pi = pan + i0 * LDP;
+                const double* pk = pan + k0 * LDP;
                 double acc[4][4];
 #pragma unroll
                 for (int q = 0; q < 4; q++)
 #pragma unroll
                     for (int u = 0; u < 4; u++) acc[q][u] = 0.0;
-                for (int c = 0; c < nb; c++) {
+#pragma unroll 4
+                for (int c = 0; c < NB; c++) {
                     double li[4], lk[4];
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
-                        li[q] = i0 + q < rows ? pan[c * rows + i0 + q] : 0.0;
-                        lk[q] = k0 + q < rows ? pan[c * rows + k0 + q] : 0.0;
+                        li[q] = pi[q * LDP + c];
+                        lk[q] = pk[q * LDP + c];
                     }
 #pragma unroll
                     for (int q = 0; q < 4; q++)
@@ -222,12 +234,12 @@ __device__ __forceinline__ void solve_body(const SolveArgs& a)
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
                         const int rk = k0 + u;
-                        old[u] = (ri < rows && ri >= rk) ? Lm[(size_t)(p0 + rk) * P + p0 + ri] : 0.0;
+                        old[u] = (ri < below && ri >= rk) ? Lm[(q0 + rk) * P + q0 + ri] : 0.0;
                     }
 #pragma unroll
                     for (int u = 0; u < 4; u++) {
                         const int rk = k0 + u;
-                        if (ri < rows && ri >= rk) Lm[(size_t)(p0 + rk) * P + p0 + ri] = old[u] - acc[q][u];
+                        if (ri < below && ri >= rk) Lm[(q0 + rk) * P + q0 + ri] = old[u] - acc[q][u];
                     }
                 }
             }
@@ -622,10 +634,15 @@ __global__ __launch_bounds__(kThreads) void solve_batch_kernel(const SolveArgs* 
         solve_body<NB>(a);
 }
 
+// dynamic LDS of the Cholesky: the rows below a block at stride NB + 1 (also
+// holds the NB x NB blocks of the triangular solves)
+size_t solve_lds_bytes(int NB, int P) { return (size_t)(NB + 1) * std::max(P, NB) * sizeof(double); }
+constexpr size_t kMaxPanelLds = 150 * 1024;   // beside the static LDS (NB x NB block, reductions)
+
 template <int NB>
 hipError_t launch_nb(const SolveArgs& a, hipStream_t st)
 {
-    const size_t lds = (size_t)NB * a.P * sizeof(double);
+    const size_t lds = solve_lds_bytes(NB, a.P);
     auto kern = solve_kernel<NB>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -641,8 +658,8 @@ hipError_t launch_solve_batch(const SolveArgs* args_dev, int n, int P_max, hipSt
 {
     if (n <= 0) return hipSuccess;
     // LDS: Cholesky panels (NB x P) or the QR transpositions (2 P ints)
-    const size_t lds = std::max((size_t)32 * P_max * sizeof(double), 2 * (size_t)P_max * sizeof(int));
-    if ((size_t)32 * P_max * sizeof(double) > 150 * 1024) return hipErrorInvalidValue;
+    const size_t lds = std::max(solve_lds_bytes(32, P_max), 2 * (size_t)P_max * sizeof(int));
+    if (solve_lds_bytes(32, P_max) > kMaxPanelLds) return hipErrorInvalidValue;
     auto kern = solve_batch_kernel<32>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -659,8 +676,8 @@ hipError_t launch_solve(const SolveArgs& a, hipStream_t st)
         hipLaunchKernelGGL(solve_qr_kernel, dim3(1), dim3(kThreads), 2 * (size_t)a.P * sizeof(int), st, a);
         return hipGetLastError();
     }
-    if ((size_t)32 * a.P * sizeof(double) <= 150 * 1024) return launch_nb<32>(a, st);
-    if ((size_t)16 * a.P * sizeof(double) <= 150 * 1024) return launch_nb<16>(a, st);
+    if (solve_lds_bytes(32, a.P) <= kMaxPanelLds) return launch_nb<32>(a, st);
+    if (solve_lds_bytes(16, a.P) <= kMaxPanelLds) return launch_nb<16>(a, st);
     return hipErrorInvalidValue;
 }
 
