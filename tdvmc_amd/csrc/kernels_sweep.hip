@@ -57,6 +57,43 @@ __device__ __forceinline__ uint32_t bcast(uint32_t v, int src)
     return (uint32_t)bcast_i<LPW>((int)v, src);
 }
 
+// Metropolis decision 2 (sum over the group of d - us) >= lu, the same in
+// every lane of each aligned group of LPW lanes.  Groups of at most 16 lanes
+// sum within their row (every lane gets the same bits).  Wider groups add the
+// row sums with row broadcasts: rows 1 and 3 receive r0 + r1 and r2 + r3
+// (LPW = 32), row 3 then r0 + r1 + r2 + r3 (LPW = 64); the last lane of the
+// group decides and a ballot hands its decision to the group.
+constexpr int kDppRowBcast15 = 0x142;   // row r + 1 <- lane 15 of row r
+constexpr int kDppRowBcast31 = 0x143;   // rows 2, 3 <- lane 31
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_rows(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_rows(double v)
+{
+    const int2 x = *reinterpret_cast<int2*>(&v);
+    int2 y;
+    y.x = __builtin_amdgcn_update_dpp(0, x.x, CTRL, ROWS, 0xF, false);
+    y.y = __builtin_amdgcn_update_dpp(0, x.y, CTRL, ROWS, 0xF, false);
+    return *reinterpret_cast<double*>(&y);
+}
+template <int LPW, typename T>
+__device__ __forceinline__ bool group_accept(T d, T us, T lu)
+{
+    if constexpr (LPW <= 16) {
+        return (T)2 * (group_sum<LPW>(d) - us) >= lu;
+    } else {
+        d = group_sum<16>(d);
+        d += dpp_rows<kDppRowBcast15, 0xA>(d);
+        if constexpr (LPW == 64) d += dpp_rows<kDppRowBcast31, 0x8>(d);
+        const uint64_t m = __builtin_amdgcn_ballot_w64((T)2 * (d - us) >= lu);
+        if constexpr (LPW == 64) return (m >> 63) & 1;
+        return ((threadIdx.x & 63) < 32 ? (m >> 31) : (m >> 63)) & 1;
+    }
+}
+
 // u(r) from the spline table.  Periodic box: the table extends over every
 // possible minimum-image distance, entries past the cutoff hold the
 // outer-pair value, so the lookup needs no clamp or select.  Open space:
@@ -147,12 +184,13 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
     const T ob = sy.one_body ? (T)(tb.uR[0] / (scale * scale) * C::kUnit * C::kUnit) : (T)0;
 
     // this lane's k-vectors (revolutions per raw coordinate unit) and
-    // amplitudes, and their rho_k
-    T rhoR[KPL > 0 ? KPL : 1], rhoI[KPL > 0 ? KPL : 1];
+    // amplitudes, and twice their rho_k (sR + i sI = 2 rho_k: scaling by 2 is
+    // exact, and it saves a multiply per k and trial)
+    T sR[KPL > 0 ? KPL : 1], sI[KPL > 0 ? KPL : 1];
     T qx[KPL > 0 ? KPL : 1], qy[KPL > 0 ? KPL : 1], qz[KPL > 0 ? KPL : 1], qa[KPL > 0 ? KPL : 1];
 #pragma unroll
     for (int m = 0; m < KPL; m++) {
-        rhoR[m] = rhoI[m] = (T)0;
+        sR[m] = sI[m] = (T)0;
         const int k = gl + m * LPW;
         qx[m] = qy[m] = qz[m] = qa[m] = (T)0;   // lanes past K: zero amplitude, no effect
         if (k < K) {
@@ -170,9 +208,11 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
                 if (D > 2) ph += qz[m] * C::raw(pos[n * D + 2]);
                 T s, c;
                 Prec<T>::sincos_small(ph, &s, &c);
-                rhoR[m] += c;
-                rhoI[m] += s;
+                sR[m] += c;
+                sI[m] += s;
             }
+            sR[m] *= (T)2;
+            sI[m] *= (T)2;
         }
     }
     // u at distance 0: the moved particle meets itself in the partner loop
@@ -232,10 +272,11 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
         const int nb = (int)((ntr - base) < LPW ? (ntr - base) : LPW);
         for (int tt = 0; tt < nb; tt++) {
             const int p = bcast_i<LPW>(pm, tt);
+            S* const pp = pos + __umul24((uint32_t)p, (uint32_t)D);   // the moved particle's slot
             S old[D], nw[D];
 #pragma unroll
             for (int c = 0; c < D; c++) {
-                old[c] = pos[__umul24((uint32_t)p, (uint32_t)D) + c];
+                old[c] = pp[c];
                 nw[c] = C::add(old[c], bcast<LPW>(dl[c], tt));
             }
             const T lu = bcast<LPW>(logu, tt);
@@ -279,29 +320,31 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
                 Prec<T>::sincos_small(pn, &sn, &cn);
                 dR[m] = cn - co;
                 dI[m] = sn - so;
-                // |rho + d|^2 - |rho|^2 = 2 Re(conj(rho) d) + |d|^2
-                delta += qa[m] * ((T)2 * (rhoR[m] * dR[m] + rhoI[m] * dI[m]) + dR[m] * dR[m] + dI[m] * dI[m]);
+                // |rho + d|^2 - |rho|^2 = 2 Re(conj(rho) d) + |d|^2 = Re((2 rho + d) conj(d))
+                delta += qa[m] * ((sR[m] + dR[m]) * dR[m] + (sI[m] + dI[m]) * dI[m]);
             }
-            if (sy.one_body && gl == 0) {
+            if (sy.one_body) {
+                // a branch on the (uniform) flag, not a select: systems without
+                // the one-body factor skip these instructions
+                asm volatile("" ::: "memory");
                 T ro2 = (T)0, rn2 = (T)0;
 #pragma unroll
                 for (int c = 0; c < D; c++) {
                     ro2 += oc[c] * oc[c];
                     rn2 += ncd[c] * ncd[c];
                 }
-                delta += -ob * (rn2 - ro2);
+                if (gl == 0) delta += -ob * (rn2 - ro2);
             }
-            delta = group_sum<LPW>(delta) - us;
-            const bool accept = ((T)2 * delta >= lu);
+            const bool accept = group_accept<LPW>(delta, us, lu);
             if (accept) {
                 if (gl == 0) {
 #pragma unroll
-                    for (int c = 0; c < D; c++) pos[p * D + c] = nw[c];
+                    for (int c = 0; c < D; c++) pp[c] = nw[c];
                 }
 #pragma unroll
                 for (int m = 0; m < KPL; m++) {
-                    rhoR[m] += dR[m];
-                    rhoI[m] += dI[m];
+                    sR[m] += (T)2 * dR[m];
+                    sI[m] += (T)2 * dI[m];
                 }
                 nacc++;
             }
@@ -319,8 +362,8 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
         for (int m = 0; m < KPL; m++) {
             const int k = gl + m * LPW;
             if (k < K) {
-                ro[2 * k] = rhoR[m];
-                ro[2 * k + 1] = rhoI[m];
+                ro[2 * k] = (T)0.5 * sR[m];
+                ro[2 * k + 1] = (T)0.5 * sI[m];
             }
         }
     }
