@@ -186,12 +186,28 @@ __device__ __forceinline__ void tile_of(int t, int nb, int* I, int* J)
     *J = i + t;
 }
 
+// Workgroup -> (tile t, walker group g).  Consecutive workgroups go to the 8
+// XCDs in turn (workgroup b runs on XCD b mod 8), each with its own L2: the
+// blocks of XCD x are given a contiguous range of the group-major order, so
+// every tile of a walker group reads that group's rows through one L2 and
+// the rows come from HBM about once instead of once per XCD.
+__device__ __forceinline__ void tile_group_of(int b, int ntiles, int groups, int* t, int* g)
+{
+    const int nblk = ntiles * groups;
+    const int x = b & 7, k = b >> 3;
+    const int per = nblk >> 3, extra = nblk & 7;
+    const int lin = x * per + min(x, extra) + k;
+    *g = lin / ntiles;
+    *t = lin - *g * ntiles;
+}
+
 // ---- pass 3a (mode 0): FP64 MFMA over walker slices, one wave per (tile, group)
 __global__ __launch_bounds__(64) void gemm64_kernel(const double* __restrict__ xt, int64_t ksteps, int nb, int groups,
                                                     double* __restrict__ partial)
 {
-    const int t = blockIdx.x, g = blockIdx.y, lane = threadIdx.x;
-    int I, J;
+    const int lane = threadIdx.x, ntiles = nb * (nb + 1) / 2;
+    int t, g, I, J;
+    tile_group_of(blockIdx.x, ntiles, groups, &t, &g);
     tile_of(t, nb, &I, &J);
     const int64_t per = (ksteps + groups - 1) / groups;
     const int64_t s0 = g * per, s1 = min(ksteps, s0 + per);
@@ -204,7 +220,7 @@ __global__ __launch_bounds__(64) void gemm64_kernel(const double* __restrict__ x
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
     // D[i][j]: j = lane % 16, i = 4 v + lane / 16
-    double* out = partial + (((size_t)g * gridDim.x + t) * 256);
+    double* out = partial + (((size_t)g * ntiles + t) * 256);
 #pragma unroll
     for (int v = 0; v < 4; v++) out[(4 * v + lane / 16) * 16 + (lane % 16)] = acc[v];
 }
@@ -213,8 +229,9 @@ __global__ __launch_bounds__(64) void gemm64_kernel(const double* __restrict__ x
 __global__ __launch_bounds__(64) void gemm_bf16x3_kernel(const __bf16* __restrict__ xt, int64_t ksteps, int nb,
                                                          int groups, double* __restrict__ partial)
 {
-    const int t = blockIdx.x, g = blockIdx.y, lane = threadIdx.x;
-    int I, J;
+    const int lane = threadIdx.x, ntiles = nb * (nb + 1) / 2;
+    int t, g, I, J;
+    tile_group_of(blockIdx.x, ntiles, groups, &t, &g);
     tile_of(t, nb, &I, &J);
     // per k-step block: 2 parts x 64 lanes x 8 bf16
     const size_t step_elems = 2 * 64 * 8;
@@ -243,7 +260,7 @@ __global__ __launch_bounds__(64) void gemm_bf16x3_kernel(const __bf16* __restric
         for (int v = 0; v < 16; v++) acc64[v] += (double)acc[v];
     }
     // D[i][j]: j = lane % 32, i = 8 (v / 4) + 4 (lane / 32) + (v % 4)
-    double* out = partial + (((size_t)g * gridDim.x + t) * 1024);
+    double* out = partial + (((size_t)g * ntiles + t) * 1024);
 #pragma unroll
     for (int v = 0; v < 16; v++) out[(8 * (v / 4) + 4 * (lane / 32) + (v % 4)) * 32 + (lane % 32)] = acc64[v];
 }
@@ -305,7 +322,7 @@ hipError_t launch_moments(const TO* O, const double* ER, const double* EI, int64
         if (blocks > 16384) blocks = 16384;
         hipLaunchKernelGGL(pack64_kernel<TO>, dim3((unsigned)blocks), dim3(256), 0, st, O, ER, EI, W, P, mean, g.Wp,
                            g.Pp, static_cast<double*>(wk.xt));
-        hipLaunchKernelGGL(gemm64_kernel, dim3(g.ntiles, g.groups), dim3(64), 0, st, static_cast<const double*>(wk.xt),
+        hipLaunchKernelGGL(gemm64_kernel, dim3(g.ntiles * g.groups), dim3(64), 0, st, static_cast<const double*>(wk.xt),
                            g.ksteps, g.nb, g.groups, wk.partial);
     } else {
         const int64_t total = (int64_t)(g.Pp / 32) * (g.Wp / 16) * 64;
@@ -313,7 +330,7 @@ hipError_t launch_moments(const TO* O, const double* ER, const double* EI, int64
         if (blocks > 16384) blocks = 16384;
         hipLaunchKernelGGL(pack_bf16_kernel<TO>, dim3((unsigned)blocks), dim3(256), 0, st, O, ER, EI, W, P, mean,
                            g.Wp, g.Pp, static_cast<__bf16*>(wk.xt));
-        hipLaunchKernelGGL(gemm_bf16x3_kernel, dim3(g.ntiles, g.groups), dim3(64), 0, st,
+        hipLaunchKernelGGL(gemm_bf16x3_kernel, dim3(g.ntiles * g.groups), dim3(64), 0, st,
                            static_cast<const __bf16*>(wk.xt), g.ksteps, g.nb, g.groups, wk.partial);
     }
     hipLaunchKernelGGL(reduce_tiles_kernel, dim3(g.ntiles), dim3(256), 0, st, wk.partial, g.ntiles, g.nb, g.tile,
