@@ -489,7 +489,8 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
             lsum[N + i] = 0.0;
         }
     }
-    for (int i = tid; i < Ps * 4 * hc; i += NT) hist[i] = 0.0;
+    // hc is even and hist 16-byte aligned: cleared two doubles per store
+    for (int i = tid; i < Ps * 2 * hc; i += NT) reinterpret_cast<double2*>(hist)[i] = make_double2(0.0, 0.0);
     for (int i = tid; i < Ps * kDsplStride; i += NT) dsp[i] = tb.dspl[i];
     for (int k = tid; k < K; k += NT) {
         // periodic: kv L is the integer lattice vector n (rounded to remove
@@ -557,8 +558,13 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
 
     // histogram columns -> column 0 (interval moments M[j][p])
     for (int e = tid; e < Ps * 4; e += NT) {
+        const double2* h2 = reinterpret_cast<const double2*>(hist + e * hc);
         double m = 0.0;
-        for (int c = 0; c < hc; c++) m += hist[e * hc + c];
+        for (int c = 0; c < hc / 2; c++) {
+            const double2 v = h2[c];
+            m += v.x;
+            m += v.y;
+        }
         hist[e * hc] = m;
     }
     __syncthreads();
