@@ -39,7 +39,7 @@ PEAK_FP64_TFLOPS = 78.6
 # sweep = coordinates in + out (2 W N D 8) + rho_k out (W K 8) = 1.213 GB,
 # measurement = coordinates + rho_k in + O_k (FP32) and energies out = 0.753 GB.
 MEASURED_TRAFFIC = {
-    ("bulk3d_lj343_k256", 65536): {"sweep": 1.1288e9, "eval": 7.072e8,
+    ("bulk3d_lj343_k256", 65536): {"sweep": 1.1290e9, "eval": 7.073e8,
                                    "source": "profiles/traffic_bulk3d_lj343_k256.md"},
 }
 
