@@ -162,7 +162,10 @@ int tdvmc_moments(tdvmc_ctx* ctx, const double* O, const double* ER, const doubl
  * S += eps I, Cholesky, :1733-1762) or type 1 (:1763-1822: S += 0.002 I only
  * with the scaling, solve, subtract the mean of each solution; eps is
  * ignored).  xdot [2P+2] = (alpha-dot R, alpha-dot I, phi-dot R, phi-dot I);
- * *n_bad = count of non-positive Cholesky pivots (TDVMC_ERR_NOT_PD if > 0). */
+ * *n_bad = count of non-positive Cholesky pivots (TDVMC_ERR_NOT_PD if > 0).
+ * Sizes: P <= 1129 for type 0, P <= 19200 for type 1 (TDVMC_ERR_BAD_ARG
+ * beyond; tdvmc_tdvp_step checks the configured type the same way, and a
+ * batch takes systems up to P = 1129). */
 int tdvmc_solve(tdvmc_ctx* ctx, const double* mean, const double* S, const double* F, int P, int imag_time,
                 int precondition, double eps, int solver_type, double* xdot, int* n_bad);
 

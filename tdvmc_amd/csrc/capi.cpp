@@ -1769,6 +1769,8 @@ int tdvmc_solve(tdvmc_ctx* ctx, const double* mean, const double* S, const doubl
     if (!ctx) return TDVMC_ERR_BAD_ARG;
     CHECK_ARG(mean && S && F && xdot && P >= 1, "bad solve arguments");
     CHECK_ARG(solver_type == 0 || solver_type == 1, "solver type must be 0 or 1");
+    CHECK_ARG(P <= solve_max_params(solver_type),
+              "P = " + std::to_string(P) + " exceeds the solver's limit of " + std::to_string(solve_max_params(solver_type)));
     double *dM = nullptr, *dS = nullptr, *dF = nullptr, *dW = nullptr, *dX = nullptr;
     int* dSt = nullptr;
     hipError_t e = hipSuccess;
@@ -1815,6 +1817,9 @@ int tdvmc_tdvp_step(tdvmc_ctx* ctx, double dt, int ode_type, int precision, int 
     if (rc) return rc;
     const int ode = ode_type < 0 ? ctx->ode_type : ode_type;
     CHECK_ARG(ode == 0 || ode == 2, "ODE type must be 0 (Euler) or 2 (RK4)");
+    CHECK_ARG(P_of(ctx) <= solve_max_params(ctx->solver_type),
+              "P = " + std::to_string(P_of(ctx)) + " exceeds the solver's limit of " +
+                  std::to_string(solve_max_params(ctx->solver_type)));
     if (use_graph) {
         char key[256];
         std::snprintf(key, sizeof(key), "%d/%d/%.17g/%lld/%.17g/%.17g/%d/%d/%d/%d", prec, ode, dt, (long long)ctx->W,
@@ -2114,6 +2119,11 @@ int tdvmc_batch_create(const char* const* configs, int n_systems, tdvmc_batch** 
             return fail(TDVMC_ERR_BAD_CONFIG, "USE_NORMALIZE_WF, USE_ADJUST_PARAMETERS and the parameter acceptance "
                                               "check are not available for batches");
         if (c->spw != 1) return fail(TDVMC_ERR_BAD_CONFIG, "batches take one sample per walker and stage");
+        // the batched solve sizes its LDS for the Cholesky panel of the largest system
+        if (c->sy.P > solve_max_params(0))
+            return fail(TDVMC_ERR_BAD_CONFIG, "system " + std::to_string(s) + ": P = " + std::to_string(c->sy.P) +
+                                                  " exceeds the batched solver's limit of " +
+                                                  std::to_string(solve_max_params(0)));
         b->kh_max = std::max(b->kh_max, c->sy.Kh);
         b->P_max = std::max(b->P_max, c->sy.P);
         b->N_max = std::max(b->N_max, c->sy.N);

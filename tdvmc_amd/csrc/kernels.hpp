@@ -120,6 +120,9 @@ struct SolveArgs {
     int mean_sub;              // 1: subtract the mean of each solution vector (solver type 1)
 };
 hipError_t launch_solve(const SolveArgs& a, hipStream_t st);
+// largest P the solve kernels take: Cholesky (solver type 0; LDS panel of
+// NB = 16 columns) or full-pivot QR (type 1)
+int solve_max_params(int solver_type);
 // one workgroup per system; args_dev [n] in device memory
 hipError_t launch_solve_batch(const SolveArgs* args_dev, int n, int P_max, hipStream_t st);
 

@@ -654,19 +654,32 @@ hipError_t launch_nb(const SolveArgs& a, hipStream_t st)
 
 }  // namespace
 
-hipError_t launch_solve_batch(const SolveArgs* args_dev, int n, int P_max, hipStream_t st)
+template <int NB>
+hipError_t launch_batch_nb(const SolveArgs* args_dev, int n, int P_max, hipStream_t st)
 {
-    if (n <= 0) return hipSuccess;
-    // LDS: Cholesky panels (NB x P) or the QR transpositions (2 P ints)
-    const size_t lds = std::max(solve_lds_bytes(32, P_max), 2 * (size_t)P_max * sizeof(int));
-    if (solve_lds_bytes(32, P_max) > kMaxPanelLds) return hipErrorInvalidValue;
-    auto kern = solve_batch_kernel<32>;
+    // LDS: Cholesky panels or the QR transpositions (2 P ints)
+    const size_t lds = std::max(solve_lds_bytes(NB, P_max), 2 * (size_t)P_max * sizeof(int));
+    auto kern = solve_batch_kernel<NB>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(kern, dim3(n), dim3(kThreads), lds, st, args_dev);
     return hipGetLastError();
+}
+
+hipError_t launch_solve_batch(const SolveArgs* args_dev, int n, int P_max, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    if (solve_lds_bytes(32, P_max) <= kMaxPanelLds) return launch_batch_nb<32>(args_dev, n, P_max, st);
+    if (solve_lds_bytes(16, P_max) <= kMaxPanelLds) return launch_batch_nb<16>(args_dev, n, P_max, st);
+    return hipErrorInvalidValue;
+}
+
+int solve_max_params(int solver_type)
+{
+    if (solver_type == 1) return (int)(kMaxPanelLds / (2 * sizeof(int)));
+    return (int)(kMaxPanelLds / (17 * sizeof(double)));
 }
 
 hipError_t launch_solve(const SolveArgs& a, hipStream_t st)

@@ -79,7 +79,9 @@ def test_moments_bf16x3_small_sample():
 
 
 @pytest.mark.parametrize("imag,pre", [(0, 0), (1, 0), (0, 1), (1, 1)])
-@pytest.mark.parametrize("P", [12, 33, 296, 560])       # last panel of 12, 1, 8, 16 columns
+# 32-column blocks up to P = 581, 16-column blocks beyond; last block of
+# 12, 1, 8, 16 (NB = 32) and 8, 12 (NB = 16) columns
+@pytest.mark.parametrize("P", [12, 33, 296, 560, 600, 1100])
 def test_solve_vs_oracle(imag, pre, P):
     rng = np.random.default_rng(P + 10 * imag + pre)
     A = rng.normal(size=(P, P))
@@ -94,6 +96,18 @@ def test_solve_vs_oracle(imag, pre, P):
     assert fro(r["xI"], ref["xI"]) < 1e-10
     assert abs(r["phiDotR"] - ref["phiDotR"]) < 1e-10 * max(1, abs(ref["phiDotR"]))
     assert abs(r["phiDotI"] - ref["phiDotI"]) < 1e-10 * max(1, abs(ref["phiDotI"]))
+
+
+def test_solve_too_large_is_an_error():
+    """Past the largest system the LDS panel holds (P = 1129), the solve
+    reports an error instead of launching."""
+    P = 1200
+    rng = np.random.default_rng(3)
+    S = np.eye(P)
+    m = dict(mO=rng.normal(size=P), mE=np.array([0.3, 0.0]), S=S, FR=rng.normal(size=P), FI=np.zeros(P))
+    with _ctx() as ctx:
+        with pytest.raises(tdvmc.TdvmcError):
+            ctx.solve(m["mO"], m["mE"], S, m["FR"], m["FI"], imag_time=1, eps=1e-3)
 
 
 def test_solve_nonpositive_pivot_in_later_panel():
