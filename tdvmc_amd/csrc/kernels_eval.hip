@@ -306,6 +306,18 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
                     pair();
                     rotate();
                 }
+            } else if (diag && (A + 1) * 64 <= N) {
+                // full diagonal block: every lane is live at the steps s < 32,
+                // lanes < 32 only at s = 32 (s runs from s_first >= 1 up to 32)
+                const int n1 = min(nsteps, max(0, 32 - s_first));
+                for (int u = 0; u < n1; u++) {
+                    pair();
+                    rotate();
+                }
+                for (int u = n1; u < nsteps; u++) {
+                    if (lane < 32) pair();
+                    rotate();
+                }
             } else {
                 for (int u = 0; u < nsteps; u++) {
                     const int s = s_first + u;
