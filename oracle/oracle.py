@@ -123,6 +123,14 @@ def lib():
                                              _dp, _dp, _dp, _dp]
         L.tdo_fpqr_solve.restype = C.c_int
         L.tdo_fpqr_solve.argtypes = [C.c_int, _dp, _dp, _dp]
+        _ip = C.POINTER(C.c_int)
+        L.tdo_gr_grid.restype = C.c_int
+        L.tdo_gr_grid.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp, _dp, _dp]
+        L.tdo_kshells.restype = C.c_int
+        L.tdo_kshells.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _ip]
+        L.tdo_observables.restype = C.c_int
+        L.tdo_observables.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int,
+                                      _dp, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -150,6 +158,9 @@ def ref():
         L.ref_bosonsbulk_sample.restype = C.c_int64
         L.ref_bosonsbulk_sample.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _dp, C.c_int, _dp, _dp,
                                             C.c_double, C.c_uint64, C.c_int64, C.c_int64, _dp]
+        L.ref_bosonsbulk_observables.restype = C.c_int
+        L.ref_bosonsbulk_observables.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _dp, C.c_int, C.c_int64,
+                                                 _dp, C.c_int, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp, _dp]
         _REF = L
     return _REF
 
@@ -257,6 +268,48 @@ def fpqr_solve(A, b):
     x = np.zeros(n)
     rank = lib().tdo_fpqr_solve(n, _ptr(A), _ptr(_f64(b).copy()), _ptr(x))
     return x, rank
+
+
+# ---------------------------------------------------------------- g(r), S(k)
+
+def gr_grid(D, P_s, rc, bins):
+    """Pair-distribution grid (tdo_gr_grid): (gmax, spacing, centres, scaling)."""
+    gmax, sp = C.c_double(), C.c_double()
+    n = lib().tdo_gr_grid(D, P_s, rc, bins, C.byref(gmax), C.byref(sp), None, None)
+    cen, scl = np.zeros(max(n, 0)), np.zeros(max(n, 0))
+    if n > 0:
+        lib().tdo_gr_grid(D, P_s, rc, bins, C.byref(gmax), C.byref(sp), _ptr(cen), _ptr(scl))
+    return gmax.value, sp.value, cen, scl
+
+
+def kshells(D, nshell=50):
+    """First nshell wave-vector shells (tdo_kshells): (vectors [V][D], counts, |n|^2)."""
+    cap = 8192
+    vec = np.zeros(cap * D, dtype=np.int32)
+    cnt = np.zeros(nshell, dtype=np.int32)
+    n2 = np.zeros(nshell, dtype=np.int32)
+    ip = C.POINTER(C.c_int)
+    V = lib().tdo_kshells(D, nshell, cap, vec.ctypes.data_as(ip), cnt.ctypes.data_as(ip), n2.ctypes.data_as(ip))
+    if V < 0:
+        raise ValueError("too many vectors")
+    return vec[:V * D].reshape(V, D), cnt, n2
+
+
+def observables(R, L, P_s, rc, bins, nshell=50):
+    """g(r) [W][count] and S(k) [W][nshell] of configurations R [W][N][D]
+    (tdo_observables, BosonsBulk::CalculateAdditionalSystemProperties)."""
+    R = _f64(R)
+    W, N, D = R.shape
+    _, _, cen, _ = gr_grid(D, P_s, rc, bins)
+    gr = np.zeros((W, len(cen)))
+    sk = np.zeros((W, nshell))
+    for w in range(W):
+        Rw = _f64(R[w])
+        g = np.zeros(len(cen))
+        s = np.zeros(nshell)
+        lib().tdo_observables(N, D, L, P_s, rc, bins, nshell, _ptr(Rw), _ptr(g), _ptr(s))
+        gr[w], sk[w] = g, s
+    return gr, sk
 
 
 # ---------------------------------------------------------------- helpers

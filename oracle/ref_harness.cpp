@@ -16,6 +16,7 @@
 #include "CSDataBulkSplines.h"
 #include "Utils.h"
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <fstream>
@@ -74,7 +75,10 @@ void bind_dim_functions(int dim)
     }
 }
 
-// 50 shells of one vector each: n = (s, 0, ...), |n| = s.
+// The first 50 shells of wave vectors in the format of kVectors<D>D.json:
+// every integer vector n != 0 with non-negative components, grouped by |n|^2
+// in increasing order, lexicographic within a shell (the order of the
+// reference's config/kVectors<D>D.json), and kNorm<D>D.csv = |n|.
 string write_kvector_dir(int dim)
 {
     char tmpl[] = "/tmp/tdvmc_refk_XXXXXX";
@@ -82,16 +86,32 @@ string write_kvector_dir(int dim)
     if (!d) return "";
     string dir = string(d) + "/";
     ofstream kv(dir + "kVectors" + to_string(dim) + "D.json");
+    ofstream kn(dir + "kNorm" + to_string(dim) + "D.csv");
+    kn.precision(17);
     kv << "{\"data\": [";
-    for (int s = 1; s <= 50; s++) {
-        kv << (s > 1 ? "," : "") << "[[" << s;
-        for (int a = 1; a < dim; a++) kv << ",0";
-        kv << "]]";
+    int shells = 0;
+    const int M = 64;
+    for (int n2 = 1; shells < 50; n2++) {
+        vector<vector<int>> sh;
+        int v[3] = {0, 0, 0};
+        const int lim[3] = {M, dim > 1 ? M : 1, dim > 2 ? M : 1};
+        for (v[0] = 0; v[0] < lim[0]; v[0]++)
+            for (v[1] = 0; v[1] < lim[1]; v[1]++)
+                for (v[2] = 0; v[2] < lim[2]; v[2]++)
+                    if (v[0] * v[0] + v[1] * v[1] + v[2] * v[2] == n2) sh.push_back(vector<int>(v, v + dim));
+        if (sh.empty()) continue;
+        kv << (shells ? "," : "") << "[";
+        for (size_t q = 0; q < sh.size(); q++) {
+            kv << (q ? "," : "") << "[";
+            for (int a = 0; a < dim; a++) kv << (a ? "," : "") << sh[q][a];
+            kv << "]";
+        }
+        kv << "]";
+        kn << sqrt((double)n2) << "\n";
+        shells++;
     }
     kv << "]}";
     kv.close();
-    ofstream kn(dir + "kNorm" + to_string(dim) + "D.csv");
-    for (int s = 1; s <= 50; s++) kn << s << "\n";
     kn.close();
     return dir;
 }
@@ -101,7 +121,8 @@ struct Setup {
     string dir;
 };
 
-Setup make_system(int n, int dim, double L, int nparam, const double* sysparams, int nsys, double time)
+Setup make_system(int n, int dim, double L, int nparam, const double* sysparams, int nsys, double time,
+                  int gr_bins = 50)
 {
     N = n;
     DIM = dim;
@@ -115,7 +136,7 @@ Setup make_system(int n, int dim, double L, int nparam, const double* sysparams,
     s.dir = write_kvector_dir(dim);
     vector<double> p(sysparams, sysparams + nsys);
     s.sys = new PhysicalSystems::BosonsBulk(p, s.dir);
-    s.sys->SetPairDistributionBinCount(50);
+    s.sys->SetPairDistributionBinCount(gr_bins);
     s.sys->InitSystem();
     s.sys->SetTime(time);
     return s;
@@ -256,6 +277,41 @@ int64_t ref_bosonsbulk_sample(int n, int dim, double L, int nparam, const double
     *mean_E = n_samples > 0 ? esum / (double)n_samples : 0.0;
     free_system(s);
     return acc;
+}
+
+// Additional observables of W fed configurations
+// (BosonsBulk::CalculateAdditionalSystemProperties, BosonsBulk.cpp:474-520):
+// the pair distribution with gr_bins bins requested (GR_BIN_COUNT) and S(k)
+// on the 50 shells written above.  *n_gr receives the grid's bin count as the
+// reference computes it (Grid.cpp:17-31); gr is [W][n_gr], sk [W][50],
+// gr_grid the bin centres [n_gr], sk_grid the shell norms |k| [50].
+int ref_bosonsbulk_observables(int n, int dim, double L, int nparam, const double* sysparams, int nsys,
+                               int64_t W, const double* R, int gr_bins, int gr_cap, int* n_gr, double* gr,
+                               double* sk, double* gr_grid, double* sk_grid)
+{
+    Setup s = make_system(n, dim, L, nparam, sysparams, nsys, 0.0, gr_bins);
+    vector<double> vR(nparam, 0.0), vI(nparam, 0.0);
+    int ng = 0;
+    for (int64_t w = 0; w < W; w++) {
+        auto Rw = to_R(R + w * n * dim);
+        s.sys->CalculateAdditionalSystemProperties(Rw, vR, vI, 0.0, 0.0);
+        auto oc = s.sys->GetAdditionalObservablesClone();
+        auto* g = dynamic_cast<Observables::ObservableVsOnGrid*>(oc.observables[0]);
+        auto* q = dynamic_cast<Observables::ObservableVsOnGrid*>(oc.observables[1]);
+        if (!g || !q) return -1;
+        ng = g->grid.count;
+        if (ng > gr_cap || (int)q->observablesV[0].values.size() != 50) return -2;
+        for (int b = 0; b < ng; b++) gr[w * ng + b] = g->observablesV[0].values[b];
+        for (int k = 0; k < 50; k++) sk[w * 50 + k] = q->observablesV[0].values[k];
+        if (w == 0) {
+            for (int b = 0; b < ng; b++) gr_grid[b] = g->grid.gridCenterPoints[b];
+            for (int k = 0; k < 50; k++) sk_grid[k] = q->grid.grid[k];
+        }
+        oc.Destroy();
+    }
+    *n_gr = ng;
+    free_system(s);
+    return 0;
 }
 
 }  // extern "C"

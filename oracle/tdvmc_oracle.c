@@ -803,3 +803,142 @@ int tdo_solve_param_dot_qr(int P, int imag_time, int precondition, double* S, co
     free(bR); free(bI); free(sc);
     return P - rank;
 }
+
+/* ------------------------------------------------------------------ */
+/* additional observables: pair distribution g(r) and S(k)            */
+/* ------------------------------------------------------------------ */
+
+/* Pair-distribution grid of BosonsBulk.  The grid ends at the last knot,
+ * nodes[P_s + 2] = ((P_s - 1) rc) / (P_s - 1) (src/PhysicalSystems/BosonsBulk.cpp:61-72
+ * with halfLength = rc), spacing = gmax / bins (:145), count = (int)(gmax / spacing)
+ * and lower edges accumulated by repeated addition (src/Observables/Grid.cpp:17-49),
+ * centres as in Grid::InitGridCenterPoints, scaling = bin width (1-D) or the
+ * area / volume between consecutive circles / spheres
+ * (src/Observables/ObservableVsOnGridWithScaling.cpp:19-42).
+ * Returns count; centres and scaling (may be NULL) need count entries. */
+int tdo_gr_grid(int D, int P_s, double rc, int bins, double* gmax, double* spacing, double* centres,
+                double* scaling)
+{
+    const double mx = ((double)(P_s - 1) * rc) / ((double)P_s - 1);
+    const double sp = mx / bins;
+    const int count = (int)((mx - 0.0) / sp);
+    *gmax = mx;
+    *spacing = sp;
+    if (count <= 0) return count;
+    if (centres) {
+        double val = 0.0, prev = 0.0;
+        for (int i = 0; i < count; i++) {
+            if (i > 0) centres[i - 1] = (val + prev) / 2.0;
+            prev = val;
+            val += sp;
+        }
+        centres[count - 1] = (mx + prev) / 2.0;
+    }
+    if (scaling) {
+        for (int i = 0; i < count; i++) {
+            if (D == 1)
+                scaling[i] = sp;
+            else if (D == 3)
+                scaling[i] = 4.0 * M_PI * pow(sp * (i + 1), 3.0) / 3.0;
+            else
+                scaling[i] = M_PI * pow(sp * (i + 1), 2.0);
+        }
+        if (D != 1)
+            for (int i = count - 1; i > 0; i--) scaling[i] = scaling[i] - scaling[i - 1];
+    }
+    return count;
+}
+
+/* The first nshell shells of wave vectors, in the layout of
+ * config/kVectors<D>D.json as read by src/Utils.cpp:1020-1049 and used by
+ * src/PhysicalSystems/BosonsBulk.cpp:124-137 (50 shells, :55): integer n != 0
+ * with non-negative components, grouped by |n|^2 in increasing order,
+ * lexicographic within a shell.  vec receives up to max_vec vectors [D] each,
+ * count[s] the vectors of shell s, n2[s] its |n|^2.  Returns the total number
+ * of vectors (or -1 if max_vec is too small). */
+int tdo_kshells(int D, int nshell, int max_vec, int* vec, int* count, int* n2)
+{
+    int total = 0, s = 0;
+    for (int q = 1; s < nshell; q++) {
+        int lim = 1;
+        while (lim * lim <= q) lim++;
+        int c = 0;
+        const int l1 = lim, l2 = D > 1 ? lim : 1, l3 = D > 2 ? lim : 1;
+        for (int x = 0; x < l1; x++)
+            for (int y = 0; y < l2; y++)
+                for (int z = 0; z < l3; z++) {
+                    if (x * x + y * y + z * z != q) continue;
+                    if (total >= max_vec) return -1;
+                    const int v[3] = {x, y, z};
+                    for (int a = 0; a < D; a++) vec[total * D + a] = v[a];
+                    total++;
+                    c++;
+                }
+        if (c == 0) continue;
+        count[s] = c;
+        n2[s] = q;
+        s++;
+    }
+    return total;
+}
+
+/* BosonsBulk::CalculateAdditionalSystemProperties for one configuration
+ * (src/PhysicalSystems/BosonsBulk.cpp:474-520): every pair i > j with
+ * minimum-image distance r < gmax adds (D / (N - 1)) / scaling[bin] to bin
+ * floor(r / spacing) (ObservableVsOnGridWithScaling::AddToHistogram,
+ * :45-50, with Grid::GetIndex, Grid.cpp:51-60); the reference writes past
+ * the grid if that bin is >= count, which is skipped here.  S(k) per shell:
+ * (C^2 + S^2) / (N n_k) with C, S the sums over particles and the shell's
+ * vectors of cos / sin(k . r_i), k = n 2 pi / L, summed particle-major as
+ * there.  gr needs tdo_gr_grid's count entries, sk nshell.  Returns count. */
+int tdo_observables(int N, int D, double L, int P_s, double rc, int bins, int nshell, const double* R, double* gr,
+                    double* sk)
+{
+    double gmax, sp;
+    const int count = tdo_gr_grid(D, P_s, rc, bins, &gmax, &sp, NULL, NULL);
+    if (count <= 0) return count;
+    double* scaling = (double*)malloc(sizeof(double) * count);
+    tdo_gr_grid(D, P_s, rc, bins, &gmax, &sp, NULL, scaling);
+    for (int b = 0; b < count; b++) gr[b] = 0.0;
+    const double weight = 1.0 / ((double)(N - 1)) * D;
+    double dv[3];
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < i; j++) {
+            const double r = tdo_displacement(R + i * D, R + j * D, D, L, dv);
+            if (r < gmax) {
+                const int bin = (int)floor(r / sp);
+                if (bin < count) gr[bin] += weight / scaling[bin];
+            }
+        }
+    free(scaling);
+
+    const int max_vec = 4096;
+    int* vec = (int*)malloc(sizeof(int) * max_vec * D);
+    int* cnt = (int*)malloc(sizeof(int) * nshell);
+    int* n2 = (int*)malloc(sizeof(int) * nshell);
+    tdo_kshells(D, nshell, max_vec, vec, cnt, n2);
+    double* sc = (double*)calloc(nshell, sizeof(double));
+    double* ss = (double*)calloc(nshell, sizeof(double));
+    const double kf = 2 * M_PI / L;
+    for (int i = 0; i < N; i++) {
+        int v0 = 0;
+        for (int k = 0; k < nshell; k++) {
+            for (int kn = 0; kn < cnt[k]; kn++) {
+                const int* n = vec + (v0 + kn) * D;
+                double ph = (n[0] * kf) * R[i * D];
+                if (D > 1) ph = ph + (n[1] * kf) * R[i * D + 1];
+                if (D > 2) ph = ph + (n[2] * kf) * R[i * D + 2];
+                sc[k] += cos(ph);
+                ss[k] += sin(ph);
+            }
+            v0 += cnt[k];
+        }
+    }
+    for (int k = 0; k < nshell; k++) sk[k] = (sc[k] * sc[k] + ss[k] * ss[k]) / ((double)(N * cnt[k]));
+    free(vec);
+    free(cnt);
+    free(n2);
+    free(sc);
+    free(ss);
+    return count;
+}

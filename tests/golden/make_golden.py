@@ -15,6 +15,13 @@ Outputs (inputs and expected outputs only; no reference source):
                          sD[S][N][D], sD2[S][N]
   move_cases.npz      -- single-particle moves through CalculateWFQuotient
   config_*.json       -- reference config files (data) for the schema loader
+  obs_<name>.npz      -- fed configurations and the reference BosonsBulk's
+                         additional observables: pair distribution g(r) with
+                         its grid, S(k) on the 50 wave-vector shells
+  kshells.json        -- the first 50 shells of config/kVectors<D>D.json and
+                         config/kNorm<D>D.csv (data) for D = 1, 2, 3
+
+    python tests/golden/make_golden.py obs    # only the observables fixtures
 """
 import ctypes as C
 import json
@@ -161,6 +168,48 @@ def move_cases():
         json.dump(rows, fh)
 
 
+
+OBS_CASES = [
+    # name, D, N, L, P (pair-spline knots), GR_BIN_COUNT, W, seed
+    ("bulk1d_n20", 1, 20, 20.0, 16, 50, 4, 21),
+    ("bulk2d_n25", 2, 25, 6.0, 12, 37, 4, 22),
+    ("bulk3d_n64", 3, 64, 5.5971, 32, 100, 4, 23),
+    ("bulk3d_n343", 3, 343, 9.7949, 40, 200, 2, 24),
+]
+
+
+def obs_cases():
+    r = o.ref()
+    for (name, D, N, L, P, bins, W, seed) in OBS_CASES:
+        rng = np.random.default_rng(seed)
+        R = rng.uniform(-L / 2, L / 2, (W, N, D))
+        R[W // 2:] += rng.integers(-2, 3, (W - W // 2, N, D)) * L
+        R = np.ascontiguousarray(R)
+        spa = np.array([0.9, 1.5])
+        cap = 4096
+        ng = C.c_int()
+        gr, sk = np.zeros(W * cap), np.zeros(W * 50)
+        gg, sg = np.zeros(cap), np.zeros(50)
+        rc = r.ref_bosonsbulk_observables(N, D, L, P, f(spa), 2, W, f(R), bins, cap, C.byref(ng), f(gr), f(sk),
+                                          f(gg), f(sg))
+        assert rc == 0 and ng.value == bins, (rc, ng.value)
+        n = ng.value
+        np.savez_compressed(os.path.join(HERE, f"obs_{name}.npz"), D=D, N=N, L=L, P=P, bins=bins, R=R,
+                            gr=gr[:W * n].reshape(W, n), sk=sk.reshape(W, 50), gr_grid=gg[:n], sk_grid=sg)
+
+
+def kshells():
+    out = {}
+    for D in (1, 2, 3):
+        with open(os.path.join(REF, "config", f"kVectors{D}D.json")) as fh:
+            data = json.load(fh)["data"][:50]
+        with open(os.path.join(REF, "config", f"kNorm{D}D.csv")) as fh:
+            norms = [float(x) for x in fh.read().split()][:50]
+        out[str(D)] = dict(shells=[[[int(c) for c in v] for v in sh] for sh in data], norms=norms)
+    with open(os.path.join(HERE, "kshells.json"), "w") as fh:
+        json.dump(dict(source="config/kVectors<D>D.json, config/kNorm<D>D.csv (first 50 shells)", shells=out), fh)
+
+
 def configs():
     for name in ["BosonsBulk1D.config", "InhContactBosonsLinearResponseToPulse.config"]:
         src = os.path.join(REF, "config", name)
@@ -171,8 +220,11 @@ def configs():
 if __name__ == "__main__":
     if not o.ref_available():
         sys.exit("oracle/_ref/libref.so missing: run `make -C oracle ref` in a container with the reference")
-    nic()
-    eval_cases()
-    move_cases()
-    configs()
+    if sys.argv[1:] != ["obs"]:
+        nic()
+        eval_cases()
+        move_cases()
+        configs()
+    obs_cases()
+    kshells()
     print("golden vectors written to", HERE)
