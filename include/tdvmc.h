@@ -152,6 +152,29 @@ int tdvmc_eval_configs_dev(tdvmc_ctx* ctx, const double* R_dev, int64_t W, int p
                            double* ER_dev, double* EI_dev, double* O_dev, double* grad_dev, double* lap_dev,
                            void* stream);
 
+/* Pair distribution g(r) and structure factor S(k), the additional
+ * observables of BosonsBulk (periodic boxes only):
+ *   grid     - src/PhysicalSystems/BosonsBulk.cpp:142-151 with Grid::Init and
+ *              InitGridCenterPoints (src/Observables/Grid.cpp:17-49): GR_BIN_COUNT
+ *              bins requested up to the last knot, n_gr = the grid's count;
+ *              S(k) on the first 50 shells of config/kVectors<D>D.json
+ *              (:55, :124-137), sk_grid = |k| of each shell;
+ *   per configuration - BosonsBulk::CalculateAdditionalSystemProperties
+ *              (src/PhysicalSystems/BosonsBulk.cpp:474-520);
+ *   averaged - CalculateAdditionalSystemProperties + ParallelCalculateAdditional-
+ *              SystemProperties (src/TDVMC.cpp:1332-1365, :1438-1444): the
+ *              context's walkers do MC_NADDITIONALINITIALIZATIONSTEPS moves,
+ *              then n_samples x (MC_NADDITIONALTHERMSTEPS moves + one
+ *              observation); g(r) and S(k) averaged over samples and walkers.
+ *              n_samples <= 0 takes MC_NADDITIONALSTEPS (the driver's end-of-run
+ *              call multiplies it by MC_NFINALSTEPS_MULTIPLICATOR, :3997-3999).
+ * Outputs: gr_grid [n_gr] bin centres, sk_grid [n_sk]; gr [W][n_gr], sk [W][n_sk]
+ * per configuration; gr [n_gr], sk [n_sk] averaged.  Any output may be NULL. */
+int tdvmc_observables_info(tdvmc_ctx* ctx, int* n_gr, int* n_sk, double* gr_grid, double* sk_grid);
+int tdvmc_observables_configs(tdvmc_ctx* ctx, const double* R, int64_t W, int precision, double* gr, double* sk);
+int tdvmc_additional_properties(tdvmc_ctx* ctx, int64_t n_samples, uint64_t seed, int precision, double* gr,
+                                double* sk);
+
 /* TDVP moments of given samples (the accumulation of src/TDVMC.cpp:1103-1109
  * and the centring of :1506-1537).  mean [P+2] = (<O>, <E_R>, <E_I>),
  * S [P][P], F [2][P] = (cov(O, E_R), cov(O, E_I)). */

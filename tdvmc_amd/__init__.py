@@ -81,6 +81,9 @@ SYMBOLS = {
     "tdvmc_eval_configs": (C.c_int, [C.c_void_p, _dp, C.c_int64, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "tdvmc_eval_configs_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tdvmc_observables_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, _dp]),
+    "tdvmc_observables_configs": (C.c_int, [C.c_void_p, _dp, C.c_int64, C.c_int, _dp, _dp]),
+    "tdvmc_additional_properties": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.c_int, _dp, _dp]),
     "tdvmc_moments": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int64, C.c_int, C.c_int, _dp, _dp, _dp]),
     "tdvmc_solve": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp,
                               C.POINTER(C.c_int)]),
@@ -286,6 +289,33 @@ class Context:
                                              _ptr(out["ER"]), _ptr(out["EI"]), _ptr(out["O"]), _ptr(g), _ptr(l),
                                              _ptr(out["other"])))
         return out
+
+    def observables_info(self):
+        """(gr_grid bin centres, sk_grid |k| per shell) of the pair distribution and S(k)."""
+        ng, ns = C.c_int(), C.c_int()
+        self._check(lib().tdvmc_observables_info(self._h, C.byref(ng), C.byref(ns), None, None))
+        gg, sg = np.zeros(ng.value), np.zeros(ns.value)
+        self._check(lib().tdvmc_observables_info(self._h, None, None, _ptr(gg), _ptr(sg)))
+        return gg, sg
+
+    def observables_configs(self, R, precision: int = DEFAULT):
+        """g(r) [W][n_gr] and S(k) [W][n_sk] of fed configurations
+        (BosonsBulk::CalculateAdditionalSystemProperties)."""
+        R = _f64(R).reshape(-1, self.N, self.D)
+        W = R.shape[0]
+        gg, sg = self.observables_info()
+        gr, sk = np.zeros((W, len(gg))), np.zeros((W, len(sg)))
+        self._check(lib().tdvmc_observables_configs(self._h, _ptr(R), W, int(precision), _ptr(gr), _ptr(sk)))
+        return gr, sk
+
+    def additional_properties(self, seed: int, n_samples: int = 0, precision: int = DEFAULT):
+        """g(r) and S(k) averaged over n_samples observations of every walker
+        (CalculateAdditionalSystemProperties + its average over ranks)."""
+        gg, sg = self.observables_info()
+        gr, sk = np.zeros(len(gg)), np.zeros(len(sg))
+        self._check(lib().tdvmc_additional_properties(self._h, int(n_samples), int(seed), int(precision), _ptr(gr),
+                                                      _ptr(sk)))
+        return gr, sk
 
     def moments(self, O, ER, EI, precision: int = DEFAULT):
         O = _f64(O)

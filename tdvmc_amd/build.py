@@ -28,6 +28,7 @@ SOURCES = [
     "kernels_moments.hip",
     "kernels_solve.hip",
     "kernels_misc.hip",
+    "kernels_obs.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

@@ -118,6 +118,21 @@ struct tdvmc_ctx {
     std::vector<Pending> graph_ev;   // event pairs recorded by nodes of the captured graph
     std::map<std::string, Prof> prof_acc;
     int64_t trials_total = 0;
+
+    // pair distribution and S(k) (BosonsBulk's additional observables), set up on first use
+    struct Obs {
+        bool ready = false;
+        int count = 0, nshell = 0, V = 0, nmax = 0;
+        double gmax = 0.0, spacing = 0.0;
+        std::vector<double> scale, centres, kgrid;
+        int* dVec = nullptr;                  // [V][4]
+        int* dOff = nullptr;                  // [nshell + 1]
+        double* dScale = nullptr;             // [count]
+        unsigned long long* dTot = nullptr;   // [count]
+        double* dSkAcc = nullptr;             // [nshell]
+        double* dSkW = nullptr;               // [W][nshell]
+        int64_t skw_rows = 0;
+    } obs;
 };
 
 namespace {
@@ -676,6 +691,156 @@ int enqueue_solve(tdvmc_ctx* ctx, double* xdot)
 }
 
 // one sampling stage with the current device tables: moves, measurements, moments
+// Grid and wave-vector shells of the pair distribution and S(k)
+// (BosonsBulk::InitSystem, src/PhysicalSystems/BosonsBulk.cpp:55, :61-72,
+// :124-155): the g(r) grid ends at the last knot ((P_s - 1) rc) / (P_s - 1)
+// with GR_BIN_COUNT bins requested and count = (int)(gmax / spacing) kept
+// (src/Observables/Grid.cpp:17-49), bin volumes as in
+// src/Observables/ObservableVsOnGridWithScaling.cpp:19-42; S(k) on the first
+// 50 shells of integer vectors with non-negative components, grouped by
+// |n|^2, lexicographic within a shell (the layout of config/kVectors<D>D.json).
+constexpr int kObsShells = 50;
+constexpr int kObsMaxBins = 8192;
+
+int ensure_obs(tdvmc_ctx* ctx)
+{
+    auto& o = ctx->obs;
+    if (o.ready) return TDVMC_OK;
+    if (!ctx->sy.periodic) {
+        ctx->err = "g(r) and S(k) need a periodic box (BosonsBulk)";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    const int bins = ctx->cfg.GR_BIN_COUNT;
+    if (bins <= 0) {
+        ctx->err = "GR_BIN_COUNT must be positive for g(r)";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    const int D = ctx->sy.D, Ps = ctx->sy.P_s, N = ctx->sy.N;
+    o.gmax = ((double)(Ps - 1) * ctx->sy.rc) / ((double)Ps - 1);
+    o.spacing = o.gmax / bins;
+    o.count = (int)((o.gmax - 0.0) / o.spacing);
+    if (o.count <= 0 || o.count > kObsMaxBins) {
+        ctx->err = "g(r) grid of " + std::to_string(o.count) + " bins (1.." + std::to_string(kObsMaxBins) + ")";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    o.centres.assign(o.count, 0.0);
+    {
+        double val = 0.0, prev = 0.0;
+        for (int i = 0; i < o.count; i++) {
+            if (i > 0) o.centres[i - 1] = (val + prev) / 2.0;
+            prev = val;
+            val += o.spacing;
+        }
+        o.centres[o.count - 1] = (o.gmax + prev) / 2.0;
+    }
+    std::vector<double> scaling(o.count);
+    for (int i = 0; i < o.count; i++) {
+        if (D == 1)
+            scaling[i] = o.spacing;
+        else if (D == 3)
+            scaling[i] = 4.0 * M_PI * std::pow(o.spacing * (i + 1), 3.0) / 3.0;
+        else
+            scaling[i] = M_PI * std::pow(o.spacing * (i + 1), 2.0);
+    }
+    if (D != 1)
+        for (int i = o.count - 1; i > 0; i--) scaling[i] = scaling[i] - scaling[i - 1];
+    const double weight = N > 1 ? 1.0 / ((double)(N - 1)) * D : 0.0;
+    o.scale.resize(o.count);
+    for (int i = 0; i < o.count; i++) o.scale[i] = weight / scaling[i];
+
+    std::vector<int> vec, off(1, 0);
+    o.kgrid.clear();
+    o.nmax = 0;
+    for (int q = 1; (int)o.kgrid.size() < kObsShells; q++) {
+        int lim = 1;
+        while (lim * lim <= q) lim++;
+        const int l2 = D > 1 ? lim : 1, l3 = D > 2 ? lim : 1;
+        int c = 0;
+        for (int x = 0; x < lim; x++)
+            for (int y = 0; y < l2; y++)
+                for (int z = 0; z < l3; z++) {
+                    if (x * x + y * y + z * z != q) continue;
+                    vec.insert(vec.end(), {x, y, z, 0});
+                    o.nmax = std::max(o.nmax, std::max(x, std::max(y, z)));
+                    c++;
+                }
+        if (c == 0) continue;
+        off.push_back(off.back() + c);
+        o.kgrid.push_back(std::sqrt((double)q) * (2 * M_PI / ctx->sy.L));
+    }
+    o.nshell = kObsShells;
+    o.V = off.back();
+    if (o.V > 4 * 256) {
+        ctx->err = "too many S(k) wave vectors";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    HIPC(dalloc(&o.dVec, vec.size()));
+    HIPC(dalloc(&o.dOff, off.size()));
+    HIPC(dalloc(&o.dScale, (size_t)o.count));
+    HIPC(dalloc(&o.dTot, (size_t)o.count));
+    HIPC(dalloc(&o.dSkAcc, (size_t)o.nshell));
+    HIPC(hipMemcpyAsync(o.dVec, vec.data(), vec.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(o.dOff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(o.dScale, o.scale.data(), o.count * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    o.ready = true;
+    return TDVMC_OK;
+}
+
+// g(r) counts of W walkers at dR added to obs.dTot, per-walker g(r) into gr_w
+// (may be null) and S(k) into sk_w [W][nshell]
+int enqueue_obs(tdvmc_ctx* ctx, int prec, const double* dR, int64_t W, double* gr_w, double* sk_w)
+{
+    auto& o = ctx->obs;
+    const int D = ctx->sy.D, N = ctx->sy.N;
+    const bool f32 = prec == TDVMC_MIXED;
+    const size_t cx = f32 ? 2 * sizeof(float) : 2 * sizeof(double);
+    const int chunk = (int)std::min<size_t>(64, std::max<size_t>(1, 24576 / ((size_t)D * (o.nmax + 1) * cx)));
+    const size_t lds = obs_lds_bytes(N, D, o.V, chunk, o.nmax, o.count, f32);
+    if (lds > 65536) {
+        ctx->err = "g(r) / S(k) need " + std::to_string(lds) + " bytes of LDS per walker (at most 65536)";
+        return TDVMC_ERR_BAD_ARG;
+    }
+    ObsArgs a{};
+    a.R = dR;
+    a.W = W;
+    a.N = N;
+    a.L = ctx->sy.L;
+    a.Lr = 1.0 / ctx->sy.L;
+    a.gmax = o.gmax;
+    a.spacing = o.spacing;
+    a.count = o.count;
+    a.scale = o.dScale;
+    a.gr_tot = o.dTot;
+    a.gr_w = gr_w;
+    a.kvec = o.dVec;
+    a.shell_off = o.dOff;
+    a.V = o.V;
+    a.nshell = o.nshell;
+    a.nmax = o.nmax;
+    a.chunk = chunk;
+    a.vpt = (o.V + 255) / 256;
+    a.sk_w = sk_w;
+    hipEvent_t ev{};
+    int rc = prof_begin(ctx, "observables", &ev);
+    if (rc) return rc;
+    if (f32) {
+        HIPC(launch_obs<float>(a, D, lds, ctx->stream));
+    } else {
+        HIPC(launch_obs<double>(a, D, lds, ctx->stream));
+    }
+    return prof_end(ctx, "observables", ev);
+}
+
+// zero the g(r) totals and the S(k) accumulator (fill kernels: both may sit in a captured stream)
+int clear_obs(tdvmc_ctx* ctx)
+{
+    auto& o = ctx->obs;
+    HIPC(launch_fill_u32(reinterpret_cast<uint32_t*>(o.dTot), 0u, 2 * (int64_t)o.count, ctx->stream));
+    HIPC(launch_fill_u32(reinterpret_cast<uint32_t*>(o.dSkAcc), 0u, 2 * (int64_t)o.nshell, ctx->stream));
+    return TDVMC_OK;
+}
+
 int enqueue_stage_samples(tdvmc_ctx* ctx, int prec, uint64_t seed)
 {
     int rc = enqueue_sweep(ctx, prec, ctx->init_moves, seed, true);
@@ -847,6 +1012,12 @@ int alloc_ctx(tdvmc_ctx* ctx)
 void release(tdvmc_ctx* ctx)
 {
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    dfree(ctx->obs.dVec);
+    dfree(ctx->obs.dOff);
+    dfree(ctx->obs.dScale);
+    dfree(ctx->obs.dTot);
+    dfree(ctx->obs.dSkAcc);
+    dfree(ctx->obs.dSkW);
     for (auto& p : ctx->pending) {
         if (p.owned) {
             hipEventDestroy(p.a);
@@ -1717,6 +1888,106 @@ int tdvmc_eval_configs(tdvmc_ctx* ctx, const double* R, int64_t W, int precision
     if (ok) ok = hipc(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     cleanup();
     if (!ok) return TDVMC_ERR_HIP;
+    return prof_flush(ctx);
+}
+
+int tdvmc_observables_info(tdvmc_ctx* ctx, int* n_gr, int* n_sk, double* gr_grid, double* sk_grid)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    int rc = ensure_obs(ctx);
+    if (rc) return rc;
+    const auto& o = ctx->obs;
+    if (n_gr) *n_gr = o.count;
+    if (n_sk) *n_sk = o.nshell;
+    if (gr_grid) std::copy(o.centres.begin(), o.centres.end(), gr_grid);
+    if (sk_grid) std::copy(o.kgrid.begin(), o.kgrid.end(), sk_grid);
+    return TDVMC_OK;
+}
+
+int tdvmc_observables_configs(tdvmc_ctx* ctx, const double* R, int64_t W, int precision, double* gr, double* sk)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(R != nullptr && W >= 1 && (gr || sk), "R is null, W < 1, or both outputs are null");
+    int prec, rc = resolve_precision(ctx, precision, &prec);
+    if (rc) return rc;
+    rc = ensure_obs(ctx);
+    if (rc) return rc;
+    const auto& o = ctx->obs;
+    const int N = ctx->sy.N, D = ctx->sy.D;
+    double *dR = nullptr, *dG = nullptr, *dS = nullptr;
+    auto cleanup = [&]() {
+        dfree(dR);
+        dfree(dG);
+        dfree(dS);
+    };
+    auto fail = [&](hipError_t e, const char* what) {
+        if (e == hipSuccess) return false;
+        ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+        cleanup();
+        return true;
+    };
+    if (fail(dalloc(&dR, (size_t)W * N * D), "hipMalloc") || fail(dalloc(&dS, (size_t)W * o.nshell), "hipMalloc") ||
+        (gr && fail(dalloc(&dG, (size_t)W * o.count), "hipMalloc")) ||
+        fail(hipMemcpyAsync(dR, R, (size_t)W * N * D * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
+             "hipMemcpy"))
+        return TDVMC_ERR_HIP;
+    rc = clear_obs(ctx);
+    if (!rc) rc = enqueue_obs(ctx, prec, dR, W, dG, dS);
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    if ((gr && fail(hipMemcpyAsync(gr, dG, (size_t)W * o.count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+                    "hipMemcpy")) ||
+        (sk && fail(hipMemcpyAsync(sk, dS, (size_t)W * o.nshell * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+                    "hipMemcpy")) ||
+        fail(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"))
+        return TDVMC_ERR_HIP;
+    cleanup();
+    return prof_flush(ctx);
+}
+
+int tdvmc_additional_properties(tdvmc_ctx* ctx, int64_t n_samples, uint64_t seed, int precision, double* gr,
+                                double* sk)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    int prec, rc = resolve_precision(ctx, precision, &prec);
+    if (rc) return rc;
+    const int64_t steps = n_samples > 0 ? n_samples : (int64_t)ctx->cfg.MC_NADDITIONALSTEPS;
+    if (steps <= 0) {
+        ctx->err = "MC_NADDITIONALSTEPS must be positive (or pass n_samples > 0)";
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    CHECK_ARG(ctx->W >= 1, "no walkers");
+    rc = prec == TDVMC_MIXED ? ensure_tables(ctx, ctx->tf, false) : ensure_tables(ctx, ctx->td, false);
+    if (rc) return rc;
+    rc = ensure_obs(ctx);
+    if (rc) return rc;
+    auto& o = ctx->obs;
+    if (o.skw_rows < ctx->W) {
+        HIPC(dalloc(&o.dSkW, (size_t)ctx->W * o.nshell));
+        o.skw_rows = ctx->W;
+    }
+    rc = clear_obs(ctx);
+    if (rc) return rc;
+    rc = enqueue_sweep(ctx, prec, (int64_t)ctx->cfg.MC_NADDITIONALINITIALIZATIONSTEPS, seed);
+    for (int64_t i = 0; !rc && i < steps; i++) {
+        rc = enqueue_sweep(ctx, prec, (int64_t)ctx->cfg.MC_NADDITIONALTHERMSTEPS, seed);
+        if (!rc) rc = enqueue_obs(ctx, prec, ctx->dR, ctx->W, nullptr, o.dSkW);
+        if (!rc) HIPC(launch_obs_reduce(o.dSkW, ctx->W, o.nshell, o.dSkAcc, ctx->stream));
+    }
+    if (rc) return rc;
+    std::vector<unsigned long long> tot(o.count);
+    std::vector<double> acc(o.nshell);
+    HIPC(hipMemcpyAsync(tot.data(), o.dTot, tot.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        ctx->stream));
+    HIPC(hipMemcpyAsync(acc.data(), o.dSkAcc, acc.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    const double n = (double)ctx->W * (double)steps;
+    if (gr)
+        for (int b = 0; b < o.count; b++) gr[b] = (double)tot[b] * o.scale[b] / n;
+    if (sk)
+        for (int k = 0; k < o.nshell; k++) sk[k] = acc[k] / n;
     return prof_flush(ctx);
 }
 

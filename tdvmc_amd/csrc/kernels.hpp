@@ -192,4 +192,28 @@ hipError_t launch_adjoint_scale(const double* grad, const double* gbar, int64_t 
 hipError_t launch_adjoint_outer(const double* outer, const double* gbar, int64_t W, int idx, double* dalpha,
                                 hipStream_t st);
 
+// Pair distribution and structure factor of every walker (kernels_obs.hip).
+struct ObsArgs {
+    const double* R;                // [W][N][D] FP64 walkers
+    int64_t W;
+    int N;
+    double L, Lr;                   // box length and 1 / L
+    double gmax, spacing;           // g(r) grid end and bin width
+    int count;                      // g(r) bins
+    const double* scale;            // [count] (D / (N-1)) / scaling[bin]
+    unsigned long long* gr_tot;     // [count] pair counts summed over walkers (atomics)
+    double* gr_w;                   // [W][count] per-walker g(r), or null
+    const int* kvec;                // [V][4] integer wave vectors n (components 0..D-1)
+    const int* shell_off;           // [nshell + 1] first vector of each shell
+    int V, nshell, nmax;            // vectors, shells, largest component
+    int chunk;                      // particles per LDS phase table
+    int vpt;                        // wave vectors per thread (<= 4)
+    double* sk_w;                   // [W][nshell] per-walker S(k)
+};
+size_t obs_lds_bytes(int N, int D, int V, int chunk, int nmax, int count, bool fp32);
+template <typename T>
+hipError_t launch_obs(const ObsArgs& a, int D, size_t lds, hipStream_t st);
+// acc[s] += sum_w sk_w[w][s] (fixed order)
+hipError_t launch_obs_reduce(const double* sk_w, int64_t W, int nshell, double* acc, hipStream_t st);
+
 }  // namespace tdvmc
