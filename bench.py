@@ -54,6 +54,48 @@ def measure_flops(N, D, K):
     return N * (N - 1) / 2 * (8 * D + 29) + N * K * (2 * D + 12) + N * (4 * D + 2)
 
 
+def obs_flops(N, D, V):
+    """Algorithmic flops of one walker's g(r) and S(k): every pair with
+    D subtractions, D minimum images (2 flops each), r^2 (2D - 1), sqrt and the
+    bin (2): 5D + 2; every (particle, wave vector) with the phase (2D - 1),
+    cos, sin and the two sums: 2D + 3."""
+    return N * (N - 1) / 2 * (5 * D + 2) + N * V * (2 * D + 3)
+
+
+def run_observables(cfg, samples=4, bins=200):
+    """Pair distribution and S(k) of every walker (BosonsBulk's additional
+    observables on the 50 wave-vector shells), `samples` observations per
+    precision without moves in between; kernel time from HIP events on the
+    context stream."""
+    import tdvmc_amd as t
+
+    cfg = dict(cfg, GR_BIN_COUNT=bins, MC_NADDITIONALSTEPS=samples, MC_NADDITIONALTHERMSTEPS=0,
+               MC_NADDITIONALINITIALIZATIONSTEPS=0)
+    out = {}
+    with t.Context(cfg) as ctx:
+        info = ctx.get_info()
+        for i in range(2):
+            ctx.sweep(info.N, seed=21 + i, precision=t.MIXED)
+        gg, sg = ctx.observables_info()
+        for name, prec in (("mixed", t.MIXED), ("fp64", t.FP64)):
+            ctx.additional_properties(seed=5, n_samples=1, precision=prec)    # warm-up
+            ctx.profile_enable(True)
+            t0 = time.perf_counter()
+            ctx.additional_properties(seed=5, precision=prec)
+            el = time.perf_counter() - t0
+            tot, n = ctx.profile_get("observables")
+            ctx.profile_enable(False)
+            out[name + "_ms"] = tot / max(n, 1)
+            out[name + "_call_ms"] = el * 1e3
+        N, D, W = info.N, info.DIM, info.n_walkers
+    V = {1: 50, 2: 103, 3: 307}[D]
+    out.update({"gr_bins": len(gg), "sk_shells": len(sg), "wave_vectors": V, "walkers": W,
+                "samples_per_call": samples,
+                "roofline_mixed": kernel_roofline(f"obs_kernel<float, {D}>", out["mixed_ms"], W * obs_flops(N, D, V),
+                                                  PEAK_FP32_TFLOPS)})
+    return out
+
+
 def run_gpu(cfg, steps, warmup, precision, use_graph=True):
     """Times `steps` RK4 TDVP steps.  Kernel durations come from HIP events
     recorded on the context stream around every sweep / measurement /
@@ -258,6 +300,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched trap workload")
+    ap.add_argument("--no-obs", action="store_true", help="skip the g(r) / S(k) observables")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise kernel of this size (counter calibration)")
     ap.add_argument("--drift", type=int, default=0, metavar="STEPS",
@@ -347,6 +390,8 @@ def main():
         line["fp64_kernel_ms"] = {k: (v[0] / max(v[1], 1)) for k, v in fp64["prof"].items()}
     if not args.no_batch:
         line["batch_trap2d_64"] = run_batch_c4(max(1, args.steps // 2), 1)
+    if not args.no_obs:
+        line["observables_gr_sk"] = run_observables(cfg)
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
     print(json.dumps(line))
 

@@ -819,7 +819,7 @@ int enqueue_obs(tdvmc_ctx* ctx, int prec, const double* dR, int64_t W, double* g
     a.nshell = o.nshell;
     a.nmax = o.nmax;
     a.chunk = chunk;
-    a.vpt = (o.V + 255) / 256;
+    a.gr_segs = std::max(1, std::min((N - 1) / 2 + 1, (512 + N - 1) / N));
     a.sk_w = sk_w;
     hipEvent_t ev{};
     int rc = prof_begin(ctx, "observables", &ev);

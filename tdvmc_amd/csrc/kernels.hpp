@@ -207,7 +207,7 @@ struct ObsArgs {
     const int* shell_off;           // [nshell + 1] first vector of each shell
     int V, nshell, nmax;            // vectors, shells, largest component
     int chunk;                      // particles per LDS phase table
-    int vpt;                        // wave vectors per thread (<= 4)
+    int gr_segs;                    // segments per row of the pair loop
     double* sk_w;                   // [W][nshell] per-walker S(k)
 };
 size_t obs_lds_bytes(int N, int D, int V, int chunk, int nmax, int count, bool fp32);
