@@ -144,6 +144,26 @@ public:
         check(tdvmc_eval_configs(ctx_, R.data(), W, precision, expo.data(), ER.data(), EI.data(), O.data(),
                                  nullptr, nullptr, other ? other->data() : nullptr));
     }
+    // grids of the pair distribution (bin centres) and S(k) (|k| per shell)
+    void observables_grid(std::vector<double>& gr_grid, std::vector<double>& sk_grid)
+    {
+        int ng = 0, ns = 0;
+        check(tdvmc_observables_info(ctx_, &ng, &ns, nullptr, nullptr));
+        gr_grid.resize(ng);
+        sk_grid.resize(ns);
+        check(tdvmc_observables_info(ctx_, nullptr, nullptr, gr_grid.data(), sk_grid.data()));
+    }
+    // ParallelCalculateAdditionalSystemProperties (src/TDVMC.cpp:1332-1365, :1438-1444):
+    // g(r) and S(k) averaged over n_samples observations (<= 0: MC_NADDITIONALSTEPS) of every walker
+    void additional_properties(uint64_t seed, std::vector<double>& gr, std::vector<double>& sk,
+                               int64_t n_samples = 0, int precision = TDVMC_PRECISION_DEFAULT)
+    {
+        int ng = 0, ns = 0;
+        check(tdvmc_observables_info(ctx_, &ng, &ns, nullptr, nullptr));
+        gr.resize(ng);
+        sk.resize(ns);
+        check(tdvmc_additional_properties(ctx_, n_samples, seed, precision, gr.data(), sk.data()));
+    }
 
 private:
     template <typename F>

@@ -22,6 +22,11 @@
  *                                         reference's non-positive-pivot behaviour
  *   tdo_build_system / tdo_phi_dot     -> hand-derived small cases (tests)
  *   tdo_metropolis                     -> exact two-particle distribution exp(2u(r))
+ *   tdo_gr_grid / tdo_kshells /        -> golden vectors of the compiled reference's
+ *   tdo_observables                       CalculateAdditionalSystemProperties, the
+ *                                         reference's kVectors / kNorm files, S(k) of a
+ *                                         lattice in closed form, g(r) pair counts
+ *                                         (tests/test_oracle_observables.py)
  */
 #include <math.h>
 #include <stdint.h>
