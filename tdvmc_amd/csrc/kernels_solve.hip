@@ -32,6 +32,30 @@ namespace {
 
 constexpr int kThreads = 1024;
 
+typedef double double4v __attribute__((ext_vector_type(4)));
+
+// (r, c) of the elements e = e0, e0 + kThreads, e0 + 2 kThreads, ... of a
+// column-major range with `rows` rows, stepped without a division per element
+struct Walk2 {
+    int r, c, dr, dc, rows;
+    __device__ Walk2(int e0, int rows_) : rows(rows_)
+    {
+        c = e0 / rows;
+        r = e0 - c * rows;
+        dc = kThreads / rows;
+        dr = kThreads - dc * rows;
+    }
+    __device__ void next()
+    {
+        r += dr;
+        c += dc;
+        if (r >= rows) {
+            r -= rows;
+            c++;
+        }
+    }
+};
+
 __device__ double block_reduce(double v, double* red)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -80,18 +104,23 @@ __device__ __forceinline__ void solve_body(const SolveArgs& a)
     __syncthreads();
     // S is symmetric: read it column by column (coalesced), four entries in flight per thread
     const size_t PP = (size_t)P * P;
+    Walk2 wk(tid, P);   // (row, column) of e0, stepped along with it
     for (size_t e0 = tid; e0 < PP; e0 += 4 * kThreads) {
         double v[4];
+        int ri[4], ci[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const size_t e = e0 + (size_t)q * kThreads;
             v[q] = e < PP ? a.S[e] : 0.0;
+            ri[q] = wk.r;
+            ci[q] = wk.c;
+            wk.next();
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const size_t e = e0 + (size_t)q * kThreads;
             if (e >= PP) continue;
-            const int i = (int)(e % P), j = (int)(e / P);
+            const int i = ri[q], j = ci[q];
             double x = v[q];
             if (a.precondition) x /= (sc[i] * sc[j]);
             if (i == j) {
@@ -122,9 +151,12 @@ __device__ __forceinline__ void solve_body(const SolveArgs& a)
     for (int p0 = 0; p0 < P; p0 += NB) {
         const int nb = min(NB, P - p0);
         const int below = P - p0 - nb;
-        for (int e = tid; e < nb * below; e += kThreads) {
-            const int c = e / below, rb = e % below;
-            pan[rb * LDP + c] = Lm[(size_t)(p0 + c) * P + p0 + nb + rb];
+        if (below > 0) {
+            Walk2 wk2(tid, below);
+            for (int e = tid; e < nb * below; e += kThreads) {
+                pan[wk2.r * LDP + wk2.c] = Lm[(size_t)(p0 + wk2.c) * P + p0 + nb + wk2.r];
+                wk2.next();
+            }
         }
         for (int e = tid; e < NB * NB; e += kThreads) {
             const int c = e / NB, l = e % NB;
@@ -188,59 +220,52 @@ __device__ __forceinline__ void solve_body(const SolveArgs& a)
             }
         }
         __syncthreads();
-        for (int e = tid; e < nb * (nb + below); e += kThreads) {
-            const int c = e / (nb + below), r = e % (nb + below);
-            if (r >= c)
-                Lm[(size_t)(p0 + c) * P + p0 + r] = r < nb ? ldc[c * NB + r] : pan[(r - nb) * LDP + c];
+        {
+            Walk2 wk3(tid, nb + below);
+            for (int e = tid; e < nb * (nb + below); e += kThreads) {
+                const int c = wk3.c, r = wk3.r;
+                if (r >= c)
+                    Lm[(size_t)(p0 + c) * P + p0 + r] = r < nb ? ldc[c * NB + r] : pan[(r - nb) * LDP + c];
+                wk3.next();
+            }
         }
         // trailing update of the lower triangle, A[i][k] -= sum_c L[i][c] L[k][c],
-        // in 4 x 4 tiles (lower-triangle tiles only), loads issued before
-        // stores.  Rows past the end of the panel are read (inside the LDS
-        // allocation) but their sums are not stored.
+        // on FP64 MFMA in 16 x 16 tiles spread over the wavefronts: lane 16 kk + pp
+        // holds L[16 K + pp][4 s + kk] as the A operand and L[16 I + pp][4 s + kk]
+        // as the B operand, so acc[v] sums row i = 16 I + pp, column
+        // k = 16 K + 4 v + kk, and consecutive lanes update consecutive rows of the
+        // column-major factor.  The old entries are loaded before the MFMA chain.
+        // Rows past the end of the panel are read (inside the LDS allocation) but
+        // their sums are not stored.
         if (below > 0) {
-            const int nt = (below + 3) / 4;
-            const int ntiles = nt * (nt + 1) / 2;
+            const int n16 = (below + 15) / 16;
+            const int ntiles = n16 * (n16 + 1) / 2;
             const size_t q0 = (size_t)(p0 + nb);   // first trailing row / column
-            for (int t = tid; t < ntiles; t += kThreads) {
+            const int pp = lane & 15, kk = lane >> 4;
+            for (int t = wv; t < ntiles; t += kThreads / 64) {
                 int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
                 while (ti * (ti + 1) / 2 > t) ti--;
                 while ((ti + 1) * (ti + 2) / 2 <= t) ti++;
-                const int tk = t - ti * (ti + 1) / 2;
-                const int i0 = 4 * ti, k0 = 4 * tk;   // rows below the block
-                const double* pi = pan + i0 * LDP;
-                const double* pk = pan + k0 * LDP;
-                double acc[4][4];
+                const int tk = t - ti * (ti + 1) / 2;   // tile (ti, tk), tk <= ti
+                const int i = 16 * ti + pp;
+                double old[4];
+                bool ok[4];
 #pragma unroll
-                for (int q = 0; q < 4; q++)
-#pragma unroll
-                    for (int u = 0; u < 4; u++) acc[q][u] = 0.0;
-#pragma unroll 4
-                for (int c = 0; c < NB; c++) {
-                    double li[4], lk[4];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        li[q] = pi[q * LDP + c];
-                        lk[q] = pk[q * LDP + c];
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; q++)
-#pragma unroll
-                        for (int u = 0; u < 4; u++) acc[q][u] += li[q] * lk[u];
+                for (int v = 0; v < 4; v++) {
+                    const int k = 16 * tk + 4 * v + kk;
+                    ok[v] = i < below && k <= i;
+                    old[v] = ok[v] ? Lm[(q0 + k) * P + q0 + i] : 0.0;
                 }
+                const double* pa = pan + (16 * tk + pp) * LDP + kk;   // rows k
+                const double* pb = pan + (16 * ti + pp) * LDP + kk;   // rows i
+                double4v acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int ri = i0 + q;
-                    double old[4];
+                for (int s4 = 0; s4 < NB / 4; s4++)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[4 * s4], pb[4 * s4], acc, 0, 0, 0);
 #pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        const int rk = k0 + u;
-                        old[u] = (ri < below && ri >= rk) ? Lm[(q0 + rk) * P + q0 + ri] : 0.0;
-                    }
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        const int rk = k0 + u;
-                        if (ri < below && ri >= rk) Lm[(q0 + rk) * P + q0 + ri] = old[u] - acc[q][u];
-                    }
+                for (int v = 0; v < 4; v++) {
+                    const int k = 16 * tk + 4 * v + kk;
+                    if (ok[v]) Lm[(q0 + k) * P + q0 + i] = old[v] - acc[v];
                 }
             }
         }
@@ -433,8 +458,10 @@ __device__ __forceinline__ void solve_qr_body(const SolveArgs& a)
         sc[i] = a.precondition ? sqrt(a.S[(size_t)i * P + i]) : 1.0;
     }
     __syncthreads();
+    Walk2 wk(tid, P);
     for (size_t e = tid; e < (size_t)P * P; e += kThreads) {
-        const int i = (int)(e % P), j = (int)(e / P);
+        const int i = wk.r, j = wk.c;
+        wk.next();
         double v = a.S[(size_t)i * P + j];
         if (a.precondition) v /= (sc[i] * sc[j]);
         if (i == j) v += a.eps;
