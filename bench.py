@@ -96,7 +96,7 @@ def run_observables(cfg, samples=4, bins=200):
     return out
 
 
-def run_gpu(cfg, steps, warmup, precision, use_graph=True):
+def run_gpu(cfg, steps, warmup, precision, use_graph=True, ode=2):
     """Times `steps` RK4 TDVP steps.  Kernel durations come from HIP events
     recorded on the context stream around every sweep / measurement /
     moments / solve launch of the timed steps themselves (inside the captured
@@ -115,12 +115,12 @@ def run_gpu(cfg, steps, warmup, precision, use_graph=True):
     dt = info.timestep
     ctx.profile_enable(True)
     for _ in range(warmup):
-        ctx.tdvp_step(dt, ode_type=2, precision=precision, use_graph=use_graph)
+        ctx.tdvp_step(dt, ode_type=ode, precision=precision, use_graph=use_graph)
     ctx.synchronize()
     ctx.profile_enable(True)          # clears the counters; the graph is kept
     t0 = time.perf_counter()
     for _ in range(steps):
-        ctx.tdvp_step(dt, ode_type=2, precision=precision, use_graph=use_graph)
+        ctx.tdvp_step(dt, ode_type=ode, precision=precision, use_graph=use_graph)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
     prof = {k: ctx.profile_get(k) for k in ("sweep", "eval", "moments", "solve")}
@@ -301,6 +301,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched trap workload")
     ap.add_argument("--no-obs", action="store_true", help="skip the g(r) / S(k) observables")
+    ap.add_argument("--no-reuse", action="store_true", help="skip the RK4 step that reuses the first stage's samples")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain elementwise kernel of this size (counter calibration)")
     ap.add_argument("--drift", type=int, default=0, metavar="STEPS",
@@ -392,6 +393,13 @@ def main():
         line["batch_trap2d_64"] = run_batch_c4(max(1, args.steps // 2), 1)
     if not args.no_obs:
         line["observables_gr_sk"] = run_observables(cfg)
+    if not args.no_reuse:
+        # ODE type 3: one sampling stage and three measurements of the same samples per step
+        r3 = run_gpu(cfg, max(1, args.steps // 2), 1, precision=1, use_graph=not args.no_graph, ode=3)
+        line["rk4_reuse_samples"] = {"tdvp_steps_per_s": r3["steps"] / r3["elapsed"],
+                                     "ms_per_step": r3["elapsed"] / r3["steps"] * 1e3,
+                                     "kernel_ms": {k: (v[0] / max(v[1], 1)) for k, v in r3["prof"].items()},
+                                     "kernel_launches": {k: v[1] for k, v in r3["prof"].items()}}
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
     print(json.dumps(line))
 

@@ -192,9 +192,18 @@ int tdvmc_moments(tdvmc_ctx* ctx, const double* O, const double* ER, const doubl
 int tdvmc_solve(tdvmc_ctx* ctx, const double* mean, const double* S, const double* F, int P, int imag_time,
                 int precondition, double eps, int solver_type, double* xdot, int* n_bad);
 
-/* One TDVP time step on the device: CalculateNextParametersEuler (type 0,
- * src/TDVMC.cpp:1834-1853) or CalculateNextParametersRK4 (type 2,
- * src/TDVMC.cpp:1969-2035, fresh samples per stage, chain continued).
+/* One TDVP time step on the device (CalculateNextParameters, src/TDVMC.cpp:2454-2512):
+ *   type 0  CalculateNextParametersEuler (:1834-1853);
+ *   type 1  CalculateNextParametersPC (:1855-1910): predictor x + dt k1, one
+ *           corrector x + dt/2 (k1 + k2) on fresh samples (chain continued);
+ *   type 2  CalculateNextParametersRK4 (:1969-2035, fresh samples per stage,
+ *           chain continued);
+ *   type 3  CalculateNextParametersRK4ReuseSamples (:2037-2103): k2..k4 on the
+ *           samples of k1 (UpdateExpectationValuesForGivenSamples, :1222-1303,
+ *           unit weights as there);
+ *   type 5  CalculateNextParametersPCReuseSamples (:1912-1967): six correctors
+ *           on the samples of k1.
+ * Types 4 and 6 (implicit Euler, Crank-Nicolson) return TDVMC_ERR_BAD_ARG.
  * ode_type < 0 uses the configured ODE_SOLVER_TYPE.  With use_graph != 0 the
  * step is captured once into a hipGraph and replayed.  After the step
  * USE_NORMALIZE_WF sets phi R = -exponent of walker 0's last sample and

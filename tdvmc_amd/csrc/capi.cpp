@@ -84,6 +84,8 @@ struct tdvmc_ctx {
     int* dKmap = nullptr;
 
     int64_t cap_samples = 0;
+    double* dSamp = nullptr;          // [spw][W][N][D] sample configurations kept for reuse (spw > 1)
+    int64_t samp_elems = 0;
     double* dO64 = nullptr;
     float* dO32 = nullptr;
     double* dER = nullptr;
@@ -630,20 +632,21 @@ int enqueue_sweep(tdvmc_ctx* ctx, int prec, int64_t n_moves, uint64_t seed, bool
 
 // eval of the context walkers into sample rows [row0, row0 + W); use_rho:
 // take rho_k from the sweep that just ran instead of recomputing it
-int enqueue_eval(tdvmc_ctx* ctx, int prec, int64_t row0, bool use_rho = false)
+int enqueue_eval(tdvmc_ctx* ctx, int prec, int64_t row0, bool use_rho = false, const double* R = nullptr)
 {
     const bool rho = use_rho && ctx->sy.Kh > 0;
+    if (!R) R = ctx->dR;
     const int P = P_of(ctx);
     hipEvent_t ev{};
     int rc = prof_begin(ctx, "eval", &ev);
     if (rc) return rc;
     if (prec == TDVMC_MIXED) {
-        HIPC((launch_eval<float, float>(ctx->sy, tables_of(ctx, ctx->tf), ctx->dR, ctx->W, ctx->dExpo + row0,
+        HIPC((launch_eval<float, float>(ctx->sy, tables_of(ctx, ctx->tf), R, ctx->W, ctx->dExpo + row0,
                                         ctx->dER + row0, ctx->dEI + row0, ctx->dO32 + row0 * P,
                                         ctx->dOther + row0 * 9, nullptr, nullptr, nullptr, ctx->phiR,
                                         rho ? reinterpret_cast<const float*>(ctx->dRho) : nullptr, ctx->stream)));
     } else {
-        HIPC((launch_eval<double, double>(ctx->sy, tables_of(ctx, ctx->td), ctx->dR, ctx->W, ctx->dExpo + row0,
+        HIPC((launch_eval<double, double>(ctx->sy, tables_of(ctx, ctx->td), R, ctx->W, ctx->dExpo + row0,
                                           ctx->dER + row0, ctx->dEI + row0, ctx->dO64 + row0 * P,
                                           ctx->dOther + row0 * 9, nullptr, nullptr, nullptr, ctx->phiR,
                                           rho ? ctx->dRho : nullptr, ctx->stream)));
@@ -841,15 +844,39 @@ int clear_obs(tdvmc_ctx* ctx)
     return TDVMC_OK;
 }
 
-int enqueue_stage_samples(tdvmc_ctx* ctx, int prec, uint64_t seed)
+// keep: copy each sample's configurations for later reuse (more than one
+// sample per walker; with one, the walkers themselves are the samples)
+int enqueue_stage_samples(tdvmc_ctx* ctx, int prec, uint64_t seed, bool keep = false)
 {
     int rc = enqueue_sweep(ctx, prec, ctx->init_moves, seed, true);
     if (rc) return rc;
+    const size_t per = (size_t)ctx->W * ctx->sy.N * ctx->sy.D;
     for (int s = 0; s < ctx->spw; s++) {
         rc = enqueue_sweep(ctx, prec, ctx->therm_moves, seed, true);
         if (rc) return rc;
         const bool swept = ctx->therm_moves > 0 || (s == 0 && ctx->init_moves > 0);
         rc = enqueue_eval(ctx, prec, (int64_t)s * ctx->W, swept);
+        if (rc) return rc;
+        if (keep && ctx->spw > 1)
+            HIPC(hipMemcpyAsync(ctx->dSamp + s * per, ctx->dR, per * sizeof(double), hipMemcpyDeviceToDevice,
+                                ctx->stream));
+    }
+    return enqueue_moments(ctx, prec, (int64_t)ctx->spw * ctx->W);
+}
+
+// UpdateExpectationValuesForGivenSamples (src/TDVMC.cpp:1222-1303): the kept
+// samples evaluated at the current parameters, unit weights (the reweighting
+// there is commented out), then the moments.  One sample per walker: the
+// walkers, with the rho_k their last sweep left when there was one.
+int enqueue_reuse_samples(tdvmc_ctx* ctx, int prec)
+{
+    const size_t per = (size_t)ctx->W * ctx->sy.N * ctx->sy.D;
+    for (int s = 0; s < ctx->spw; s++) {
+        int rc;
+        if (ctx->spw == 1)
+            rc = enqueue_eval(ctx, prec, 0, ctx->therm_moves > 0 || ctx->init_moves > 0);
+        else
+            rc = enqueue_eval(ctx, prec, (int64_t)s * ctx->W, false, ctx->dSamp + s * per);
         if (rc) return rc;
     }
     return enqueue_moments(ctx, prec, (int64_t)ctx->spw * ctx->W);
@@ -861,14 +888,15 @@ int enqueue_prep(tdvmc_ctx* ctx, int prec)
     return ensure_tables(ctx, ctx->td, true);
 }
 
-// f(x): parameters x (packed, device) -> xdot
-int enqueue_rhs(tdvmc_ctx* ctx, int prec, const double* x, double* xdot)
+// f(x): parameters x (packed, device) -> xdot.  reuse: evaluate the kept
+// samples instead of sampling; keep: keep this stage's samples for reuse
+int enqueue_rhs(tdvmc_ctx* ctx, int prec, const double* x, double* xdot, bool reuse = false, bool keep = false)
 {
     const int P = P_of(ctx);
     HIPC(launch_unpack_params(P, x, ctx->dUR, ctx->dUI, ctx->stream));
     int rc = enqueue_prep(ctx, prec);
     if (rc) return rc;
-    rc = enqueue_stage_samples(ctx, prec, ctx->seed);
+    rc = reuse ? enqueue_reuse_samples(ctx, prec) : enqueue_stage_samples(ctx, prec, ctx->seed, keep);
     if (rc) return rc;
     return enqueue_solve(ctx, xdot);
 }
@@ -896,8 +924,41 @@ int enqueue_tdvp_step(tdvmc_ctx* ctx, int prec, int ode, double dt)
         rc = enqueue_rhs(ctx, prec, ctx->dXtmp, ctx->dK[3]);
         if (rc) return rc;
         HIPC(launch_rk4_combine(P, ctx->dX, ctx->dK[0], ctx->dK[1], ctx->dK[2], ctx->dK[3], dt, ctx->stream));
+    } else if (ode == 3) {
+        // CalculateNextParametersRK4ReuseSamples (src/TDVMC.cpp:2037-2103):
+        // fresh samples for k1, the same samples for k2..k4
+        rc = enqueue_rhs(ctx, prec, ctx->dX, ctx->dK[0], false, true);
+        if (rc) return rc;
+        HIPC(launch_axpy_params(P, ctx->dX, ctx->dK[0], dt / 2.0, ctx->dXtmp, ctx->stream));
+        rc = enqueue_rhs(ctx, prec, ctx->dXtmp, ctx->dK[1], true);
+        if (rc) return rc;
+        HIPC(launch_axpy_params(P, ctx->dX, ctx->dK[1], dt / 2.0, ctx->dXtmp, ctx->stream));
+        rc = enqueue_rhs(ctx, prec, ctx->dXtmp, ctx->dK[2], true);
+        if (rc) return rc;
+        HIPC(launch_axpy_params(P, ctx->dX, ctx->dK[2], dt, ctx->dXtmp, ctx->stream));
+        rc = enqueue_rhs(ctx, prec, ctx->dXtmp, ctx->dK[3], true);
+        if (rc) return rc;
+        HIPC(launch_rk4_combine(P, ctx->dX, ctx->dK[0], ctx->dK[1], ctx->dK[2], ctx->dK[3], dt, ctx->stream));
+    } else if (ode == 1 || ode == 5) {
+        // CalculateNextParametersPC (src/TDVMC.cpp:1855-1910): predictor
+        // x + dt k1, one corrector x + dt/2 (k1 + k2) on fresh samples;
+        // ...PCReuseSamples (:1912-1967): six correctors on the samples of k1
+        const bool reuse = ode == 5;
+        const int passes = reuse ? 6 : 1;
+        rc = enqueue_rhs(ctx, prec, ctx->dX, ctx->dK[0], false, reuse);
+        if (rc) return rc;
+        HIPC(launch_axpy_params(P, ctx->dX, ctx->dK[0], dt, ctx->dXtmp, ctx->stream));
+        for (int c = 0; c < passes; c++) {
+            rc = enqueue_rhs(ctx, prec, ctx->dXtmp, ctx->dK[1], reuse);
+            if (rc) return rc;
+            HIPC(launch_pc_combine(P, ctx->dX, ctx->dK[0], ctx->dK[1], dt, ctx->dXtmp, ctx->stream));
+        }
+        HIPC(hipMemcpyAsync(ctx->dX, ctx->dXtmp, (2 * (size_t)P + 2) * sizeof(double), hipMemcpyDeviceToDevice,
+                            ctx->stream));
     } else {
-        ctx->err = "ODE_SOLVER_TYPE " + std::to_string(ode) + " is not implemented (0 Euler, 2 RK4)";
+        ctx->err = "ODE_SOLVER_TYPE " + std::to_string(ode) +
+                   " is not implemented (0 Euler, 1 predictor-corrector, 2 RK4, 3 RK4 reusing samples, "
+                   "5 predictor-corrector reusing samples)";
         return TDVMC_ERR_BAD_ARG;
     }
     // src/TDVMC.cpp:3757-3775: phi R := -(exponent of the root rank's walker,
@@ -1027,6 +1088,7 @@ void release(tdvmc_ctx* ctx)
     ctx->pending.clear();
     drop_graph(ctx);
     dfree(ctx->dR);
+    dfree(ctx->dSamp);
     dfree(ctx->dRho);
     dfree(ctx->dAcc);
     dfree(ctx->dEpoch);
@@ -2087,7 +2149,18 @@ int tdvmc_tdvp_step(tdvmc_ctx* ctx, double dt, int ode_type, int precision, int 
     int prec, rc = resolve_precision(ctx, precision, &prec);
     if (rc) return rc;
     const int ode = ode_type < 0 ? ctx->ode_type : ode_type;
-    CHECK_ARG(ode == 0 || ode == 2, "ODE type must be 0 (Euler) or 2 (RK4)");
+    CHECK_ARG(ode == 0 || ode == 1 || ode == 2 || ode == 3 || ode == 5,
+              "ODE type must be 0 (Euler), 1 (predictor-corrector), 2 (RK4), 3 (RK4 reusing samples) or 5 "
+              "(predictor-corrector reusing samples)");
+    if ((ode == 3 || ode == 5) && ctx->spw > 1) {
+        // every sample's configurations are kept for the stages that reuse them
+        const int64_t need = (int64_t)ctx->spw * ctx->W * ctx->sy.N * ctx->sy.D;
+        if (ctx->samp_elems < need) {
+            drop_graph(ctx);   // a captured graph holds the old buffer's address
+            HIPC(dalloc(&ctx->dSamp, (size_t)need));
+            ctx->samp_elems = need;
+        }
+    }
     CHECK_ARG(P_of(ctx) <= solve_max_params(ctx->solver_type),
               "P = " + std::to_string(P_of(ctx)) + " exceeds the solver's limit of " +
                   std::to_string(solve_max_params(ctx->solver_type)));

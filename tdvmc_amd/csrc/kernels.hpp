@@ -179,6 +179,9 @@ hipError_t launch_axpy_params(int P, const double* x0, const double* xdot, doubl
 hipError_t launch_rk4_combine(int P, double* x, const double* k1, const double* k2, const double* k3,
                               const double* k4, double dt, hipStream_t st);
 // params (2P+2 packed) -> uR, uI arrays used by the tables
+// y = x + (k1 + k2) dt / 2 on the packed parameter vector [2P+2]
+hipError_t launch_pc_combine(int P, const double* x, const double* k1, const double* k2, double dt, double* y,
+                             hipStream_t st);
 hipError_t launch_unpack_params(int P, const double* packed, double* uR, double* uI, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t st);
 hipError_t launch_add_u32(uint32_t* p, uint32_t v, hipStream_t st);

@@ -227,6 +227,13 @@ __global__ void rk4_combine_kernel(int n, double* x, const double* k1, const dou
         x[i] = x[i] + ((k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]) / 6.0) * dt;
 }
 
+// predictor-corrector combination of src/TDVMC.cpp:1899-1902: y = x + (k1 + k2) dt/2
+__global__ void pc_combine_kernel(int n, const double* x, const double* k1, const double* k2, double dt_2, double* y)
+{
+    for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < n; i += blockDim.x * gridDim.x)
+        y[i] = x[i] + (k1[i] + k2[i]) * dt_2;
+}
+
 __global__ void unpack_params_kernel(int P, const double* packed, double* uR, double* uI)
 {
     for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < P; i += blockDim.x * gridDim.x) {
@@ -337,6 +344,14 @@ hipError_t launch_rk4_combine(int P, double* x, const double* k1, const double* 
 {
     const int n = 2 * P + 2;
     hipLaunchKernelGGL(rk4_combine_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, x, k1, k2, k3, k4, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_pc_combine(int P, const double* x, const double* k1, const double* k2, double dt, double* y,
+                             hipStream_t st)
+{
+    const int n = 2 * P + 2;
+    hipLaunchKernelGGL(pc_combine_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, x, k1, k2, dt / 2.0, y);
     return hipGetLastError();
 }
 

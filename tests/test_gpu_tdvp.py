@@ -1,10 +1,14 @@
-"""GPU TDVP step (Euler, RK4) against the oracle, and hipGraph replay.
+"""GPU TDVP step (Euler, predictor-corrector, RK4, and the variants that reuse
+the samples of the first stage) against the oracle, and hipGraph replay.
 
 With no Metropolis moves between stages (MC_NTHERMSTEPS = 0) every RK stage
 measures the same fed walkers, so the step is deterministic: the parameters
 after one step must match the oracle's restatement of
-CalculateNextParametersRK4 / ...Euler (src/TDVMC.cpp:1834-1853, :1969-2035)
-built from the oracle's evaluation, moments and solve, within 1e-9.
+CalculateNextParametersRK4 / ...Euler / ...PC / ...RK4ReuseSamples /
+...PCReuseSamples (src/TDVMC.cpp:1834-2103) built from the oracle's evaluation,
+moments and solve, within 1e-9.  The reuse variants are also checked after
+real moves: their later stages evaluate the samples of the first stage, which
+are the walkers the step leaves behind.
 """
 import numpy as np
 import pytest
@@ -39,20 +43,32 @@ def _f(s, R, x, imag, eps=1e-3):
     return np.concatenate([r["xR"], r["xI"], [r["phiDotR"], r["phiDotI"]]])
 
 
-@pytest.mark.parametrize("imag", [0, 1])
-@pytest.mark.parametrize("ode", [0, 2])
-def test_step_vs_oracle(imag, ode):
-    cfg, s, R, uR, uI = _setup(imag)
-    dt = 0.01
-    x0 = np.concatenate([uR, uI, [0.1, -0.2]])
+def _oracle_step(s, R, x0, imag, ode, dt):
+    """One step of ODE type `ode` on fixed samples R (the oracle's restatement of
+    src/TDVMC.cpp:1834-1853 Euler, :1855-1910 PC, :1912-1967 PC reusing samples
+    (six correctors), :1969-2035 RK4, :2037-2103 RK4 reusing samples)."""
     if ode == 0:
-        x1 = x0 + dt * _f(s, R, x0, imag)
-    else:
+        return x0 + dt * _f(s, R, x0, imag)
+    if ode in (2, 3):
         k1 = _f(s, R, x0, imag)
         k2 = _f(s, R, x0 + dt / 2 * k1, imag)
         k3 = _f(s, R, x0 + dt / 2 * k2, imag)
         k4 = _f(s, R, x0 + dt * k3, imag)
-        x1 = x0 + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        return x0 + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+    k1 = _f(s, R, x0, imag)
+    xp = x0 + dt * k1
+    for _ in range(6 if ode == 5 else 1):
+        xp = x0 + (k1 + _f(s, R, xp, imag)) * (dt / 2)
+    return xp
+
+
+@pytest.mark.parametrize("imag", [0, 1])
+@pytest.mark.parametrize("ode", [0, 1, 2, 3, 5])
+def test_step_vs_oracle(imag, ode):
+    cfg, s, R, uR, uI = _setup(imag)
+    dt = 0.01
+    x0 = np.concatenate([uR, uI, [0.1, -0.2]])
+    x1 = _oracle_step(s, R, x0, imag, ode, dt)
     print("tdvp", imag, ode, flush=True)
     with tdvmc.Context(cfg) as ctx:
         ctx.set_params(uR, uI, 0.1, -0.2)
@@ -179,3 +195,52 @@ def test_parameter_acceptance_check_rolls_back():
         assert np.array_equal(before, after)
         assert ctx.get_info().time == t0
         ctx.tdvp_step(1e-6, ode_type=0, precision=tdvmc.FP64)     # a small step is accepted again
+
+
+@pytest.mark.parametrize("ode", [3, 5])
+@pytest.mark.parametrize("graph", [False, True])
+def test_reuse_after_moves_vs_oracle(ode, graph):
+    # the first stage samples after N moves per walker; the later stages
+    # evaluate those samples, which are the walkers the step leaves behind
+    cfg, s, R, uR, uI = _setup(0)
+    cfg.update(MC_NTHERMSTEPS=9, MC_STEP=0.3)
+    dt = 0.01
+    x0 = np.concatenate([uR, uI, [0.1, -0.2]])
+    print("tdvp reuse after moves", ode, graph, flush=True)
+    with tdvmc.Context(cfg) as ctx:
+        ctx.set_params(uR, uI, 0.1, -0.2)
+        ctx.set_walkers(R)
+        ctx.tdvp_step(dt, ode_type=ode, precision=tdvmc.FP64, use_graph=graph)
+        gR, gI, pR, pI = ctx.get_params()
+        R1 = ctx.get_walkers()
+    assert not np.array_equal(R1, R)
+    x1 = _oracle_step(s, R1, x0, 0, ode, dt)
+    got = np.concatenate([gR, gI, [pR, pI]])
+    assert np.max(np.abs(got - x1)) < 1e-9 * np.max(np.abs(x1 - x0)) + 1e-13
+
+
+@pytest.mark.parametrize("ode", [3, 5])
+def test_reuse_two_samples_per_walker(ode):
+    # two samples per walker without moves: the kept copies hold the same
+    # walkers twice, and the moments of the doubled set equal those of one
+    cfg, s, R, uR, uI = _setup(1)
+    cfg.update(SAMPLES_PER_WALKER=2)
+    dt = 0.01
+    x0 = np.concatenate([uR, uI, [0.1, -0.2]])
+    x1 = _oracle_step(s, R, x0, 1, ode, dt)
+    print("tdvp reuse spw=2", ode, flush=True)
+    with tdvmc.Context(cfg) as ctx:
+        ctx.set_params(uR, uI, 0.1, -0.2)
+        ctx.set_walkers(R)
+        ctx.tdvp_step(dt, ode_type=ode, precision=tdvmc.FP64)
+        gR, gI, pR, pI = ctx.get_params()
+    got = np.concatenate([gR, gI, [pR, pI]])
+    assert np.max(np.abs(got - x1)) < 1e-9 * np.max(np.abs(x1 - x0)) + 1e-13
+
+
+def test_unsupported_ode_type_refused():
+    cfg, s, R, uR, uI = _setup(0)
+    with tdvmc.Context(cfg) as ctx:
+        with pytest.raises(tdvmc.TdvmcError) as ei:
+            ctx.tdvp_step(0.01, ode_type=4, precision=tdvmc.FP64)
+    assert ei.value.code == tdvmc.ERR_BAD_ARG
