@@ -175,6 +175,26 @@ int tdvmc_observables_configs(tdvmc_ctx* ctx, const double* R, int64_t W, int pr
 int tdvmc_additional_properties(tdvmc_ctx* ctx, int64_t n_samples, uint64_t seed, int precision, double* gr,
                                 double* sk);
 
+/* Output files of the reference driver, byte for byte in its format (one
+ * ".dat" file per series in out_dir, which ends with the path separator as
+ * OUT_DIR does; numbers in scientific notation, 15 digits, width 24:
+ * src/Utils.h:25-28, WriteDataToFile / AppendDataToFile, src/Utils.cpp:818-1016).
+ * Host only; no context needed.
+ *   tdvmc_files_init          - ParametersR0 / ParametersI0 with the start
+ *                               parameters and the series files with their headers
+ *                               (src/TDVMC.cpp:3117-3129);
+ *   tdvmc_files_append_step   - one line each for timesSystem, LocalEnergyR,
+ *                               LocalEnergyI, LocalOperators [P], OtherExpectationValues
+ *                               [9], ParametersR [P], ParametersI [P] (src/TDVMC.cpp:3833-3839);
+ *   tdvmc_files_write_grids   - gr_grid / sk_grid (src/TDVMC.cpp:3151-3157);
+ *   tdvmc_files_append_observables - timesAdditional, gr, sk (src/TDVMC.cpp:3475, :3496-3500). */
+int tdvmc_files_init(const char* out_dir, const double* uR, const double* uI, int P);
+int tdvmc_files_append_step(const char* out_dir, double time, const tdvmc_step_data* d, const double* uR,
+                            const double* uI, int P);
+int tdvmc_files_write_grids(const char* out_dir, const double* gr_grid, int n_gr, const double* sk_grid, int n_sk);
+int tdvmc_files_append_observables(const char* out_dir, double time, const double* gr, int n_gr, const double* sk,
+                                   int n_sk);
+
 /* TDVP moments of given samples (the accumulation of src/TDVMC.cpp:1103-1109
  * and the centring of :1506-1537).  mean [P+2] = (<O>, <E_R>, <E_I>),
  * S [P][P], F [2][P] = (cov(O, E_R), cov(O, E_I)). */

@@ -314,4 +314,28 @@ int ref_bosonsbulk_observables(int n, int dim, double L, int nparam, const doubl
     return 0;
 }
 
+// The reference's file writers (src/Utils.cpp:818-1016) with OUT_DIR = dir.
+int ref_write_vector(const char* dir, const char* name, const double* v, int n, const char* header)
+{
+    OUT_DIR = dir;
+    vector<double> d(v, v + n);
+    WriteDataToFile(d, name, header, 1);
+    return 0;
+}
+
+int ref_append_vector(const char* dir, const char* name, const double* v, int n)
+{
+    OUT_DIR = dir;
+    vector<double> d(v, v + n);
+    AppendDataToFile(d, name);
+    return 0;
+}
+
+int ref_append_scalar(const char* dir, const char* name, double x)
+{
+    OUT_DIR = dir;
+    AppendDataToFile(x, name);
+    return 0;
+}
+
 }  // extern "C"

@@ -23,6 +23,7 @@ LIB = os.path.join(HERE, "libtdvmc.so")
 SOURCES = [
     "capi.cpp",
     "config.cpp",
+    "writer.cpp",
     "kernels_eval.hip",
     "kernels_sweep.hip",
     "kernels_moments.hip",
