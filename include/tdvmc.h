@@ -194,6 +194,13 @@ int tdvmc_files_append_step(const char* out_dir, double time, const tdvmc_step_d
 int tdvmc_files_write_grids(const char* out_dir, const double* gr_grid, int n_gr, const double* sk_grid, int n_sk);
 int tdvmc_files_append_observables(const char* out_dir, double time, const double* gr, int n_gr, const double* sk,
                                    int n_sk);
+/* The config for a successive run, as WriteConfig writes AAFinish_tVMC.config
+ * (src/TDVMC.cpp:387-410, :4026-4034): the items of `config` (text or path)
+ * with the final parameters in PARAMS_REAL / PARAMS_IMAGINARY [P] and
+ * PARAM_PHIR / PARAM_PHII, in the reference's order and value format, then
+ * this library's keys (which the reference's reader skips).  Host only. */
+int tdvmc_write_final_config(const char* config, const double* uR, const double* uI, double phiR, double phiI, int P,
+                             const char* path);
 
 /* TDVP moments of given samples (the accumulation of src/TDVMC.cpp:1103-1109
  * and the centring of :1506-1537).  mean [P+2] = (<O>, <E_R>, <E_I>),

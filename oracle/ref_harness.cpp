@@ -14,11 +14,13 @@
 
 #include "PhysicalSystems/BosonsBulk.h"
 #include "CSDataBulkSplines.h"
+#include "ConfigItem.h"
 #include "Utils.h"
 
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <random>
 #include <string>
@@ -336,6 +338,27 @@ int ref_append_scalar(const char* dir, const char* name, double x)
     OUT_DIR = dir;
     AppendDataToFile(x, name);
     return 0;
+}
+
+// ConfigItem::getJsonString (src/ConfigItem.cpp:82-106) of one value of each
+// type: kind 0 string (s), 1 int (i), 2 double (d[0]), 3 int array (ia[0..n)),
+// 4 double array (d[0..n)); the text goes to out (cap bytes).
+int ref_config_json(int kind, const char* s, int i, const double* d, const int* ia, int n, char* out, int cap)
+{
+    string sv = s ? s : "";
+    int iv = i;
+    double dv = d ? d[0] : 0.0;
+    vector<int> va(ia, ia + (ia ? n : 0));
+    vector<double> vd(d, d + (d ? n : 0));
+    string r;
+    if (kind == 0) r = ConfigItem("X", &sv, STRING).getJsonString();
+    else if (kind == 1) r = ConfigItem("X", &iv, INT).getJsonString();
+    else if (kind == 2) r = ConfigItem("X", &dv, DOUBLE).getJsonString();
+    else if (kind == 3) r = ConfigItem("X", va, ARR_INT).getJsonString();
+    else r = ConfigItem("X", vd, ARR_DOUBLE).getJsonString();
+    if ((int)r.size() + 1 > cap) return -1;
+    memcpy(out, r.c_str(), r.size() + 1);
+    return (int)r.size();
 }
 
 }  // extern "C"

@@ -88,6 +88,7 @@ SYMBOLS = {
     "tdvmc_files_append_step": (C.c_int, [C.c_char_p, C.c_double, C.POINTER(StepData), _dp, _dp, C.c_int]),
     "tdvmc_files_write_grids": (C.c_int, [C.c_char_p, _dp, C.c_int, _dp, C.c_int]),
     "tdvmc_files_append_observables": (C.c_int, [C.c_char_p, C.c_double, _dp, C.c_int, _dp, C.c_int]),
+    "tdvmc_write_final_config": (C.c_int, [C.c_char_p, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_char_p]),
     "tdvmc_moments": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int64, C.c_int, C.c_int, _dp, _dp, _dp]),
     "tdvmc_solve": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp,
                               C.POINTER(C.c_int)]),

@@ -1,5 +1,6 @@
 #include "config.hpp"
 
+#include <cstdio>
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
@@ -52,14 +53,11 @@ void apply(const Item& it, const JsonValue* v)
     }
 }
 
-}  // namespace
-
-Config parse_config(const std::string& text)
+// The config items in the reference's order (src/TDVMC.cpp:297-349), then
+// this library's keys
+std::vector<Item> items_of(Config& c)
 {
-    JsonValue root = JsonParser(text).parse();
-    if (root.type != JsonValue::Object) throw std::runtime_error("config is not a JSON object");
-    Config c;
-    const Item items[] = {
+    return {
         {"CONFIG_VERSION", ItemType::String, &c.CONFIG_VERSION},
         {"SYSTEM_TYPE", ItemType::String, &c.SYSTEM_TYPE},
         {"OUTPUT_DIRECTORY", ItemType::String, &c.OUTPUT_DIRECTORY},
@@ -124,6 +122,60 @@ Config parse_config(const std::string& text)
         {"REG_EPS", ItemType::Double, &c.REG_EPS},
         {"MOVE_ORDER", ItemType::String, &c.MOVE_ORDER},
     };
+}
+
+// ConfigItem::getJsonString (src/ConfigItem.cpp:82-106): strings quoted,
+// integers and doubles by std::to_string (doubles therefore with six decimals),
+// arrays on their own lines with JoinVector's comma-joined entries (doubles in
+// scientific notation with 15 digits, src/Utils.h:56-72)
+std::string json_value(const Item& it)
+{
+    char buf[64];
+    std::string s;
+    switch (it.type) {
+    case ItemType::String:
+        return "\"" + *static_cast<const std::string*>(it.ptr) + "\"";
+    case ItemType::Int:
+        return std::to_string(*static_cast<const int*>(it.ptr));
+    case ItemType::Int64:
+        return std::to_string(*static_cast<const int64_t*>(it.ptr));
+    case ItemType::Double:
+        return std::to_string(*static_cast<const double*>(it.ptr));
+    case ItemType::ArrInt:
+        for (int v : *static_cast<const std::vector<int>*>(it.ptr)) s += (s.empty() ? "" : ",") + std::to_string(v);
+        break;
+    case ItemType::ArrDouble:
+        for (double v : *static_cast<const std::vector<double>*>(it.ptr)) {
+            std::snprintf(buf, sizeof(buf), "%.15e", v);
+            s += (s.empty() ? "" : ",") + std::string(buf);
+        }
+        break;
+    }
+    return "\n\t[\n\t\t " + s + "\n\t]";
+}
+
+}  // namespace
+
+std::string config_to_json(const Config& cfg)
+{
+    Config c = cfg;
+    const std::vector<Item> items = items_of(c);
+    // WriteConfig (src/TDVMC.cpp:387-410)
+    std::string out = "{\n";
+    for (size_t i = 0; i < items.size(); i++)
+        out += "\t\"" + std::string(items[i].name) + "\" : " + json_value(items[i]) + (i + 1 < items.size() ? ",\n" : "\n");
+    return out + "}";
+}
+
+namespace {
+}  // namespace
+
+Config parse_config(const std::string& text)
+{
+    JsonValue root = JsonParser(text).parse();
+    if (root.type != JsonValue::Object) throw std::runtime_error("config is not a JSON object");
+    Config c;
+    const std::vector<Item> items = items_of(c);
     for (const auto& it : items) apply(it, root.get(it.name));
     return c;
 }

@@ -53,6 +53,11 @@ struct Config {
 // semantics (src/ConfigItem.cpp:45-80).  Throws std::runtime_error.
 Config parse_config(const std::string& text);
 
+// The config as WriteConfig writes it (src/TDVMC.cpp:387-410): every item
+// in the reference's order, then this library's keys, which the reference's
+// reader ignores.
+std::string config_to_json(const Config& c);
+
 // Reads `arg` as JSON text if it starts with '{', else as a file path.
 std::string read_config_text(const std::string& arg);
 

@@ -9,8 +9,10 @@
 // Host code only; nothing here touches the device.
 
 #include "../../include/tdvmc.h"
+#include "config.hpp"
 
 #include <cstdio>
+#include <exception>
 #include <string>
 
 namespace {
@@ -103,6 +105,30 @@ int tdvmc_files_append_observables(const char* out_dir, double time, const doubl
     if (!rc) rc = append_line(out_dir, "gr", gr, n_gr);
     if (!rc) rc = append_line(out_dir, "sk", sk, n_sk);
     return rc;
+}
+
+int tdvmc_write_final_config(const char* config, const double* uR, const double* uI, double phiR, double phiI, int P,
+                             const char* path)
+{
+    if (!config || !path || P < 0 || (P > 0 && (!uR || !uI))) return TDVMC_ERR_BAD_ARG;
+    std::string text;
+    try {
+        // src/TDVMC.cpp:4026-4034: the final parameters go into PARAMS_REAL,
+        // PARAMS_IMAGINARY, PARAM_PHIR and PARAM_PHII
+        tdvmc::Config c = tdvmc::parse_config(tdvmc::read_config_text(config));
+        c.PARAMS_REAL.assign(uR, uR + P);
+        c.PARAMS_IMAGINARY.assign(uI, uI + P);
+        c.PARAM_PHIR = phiR;
+        c.PARAM_PHII = phiI;
+        text = tdvmc::config_to_json(c);
+    } catch (const std::exception&) {
+        return TDVMC_ERR_BAD_CONFIG;
+    }
+    std::FILE* f = std::fopen(path, "w");
+    if (!f) return TDVMC_ERR_BAD_ARG;
+    int rc = std::fputs(text.c_str(), f) < 0 ? -1 : 0;
+    rc |= std::fclose(f);
+    return rc ? TDVMC_ERR_BAD_ARG : TDVMC_OK;
 }
 
 }  // extern "C"

@@ -20,6 +20,8 @@ Outputs (inputs and expected outputs only; no reference source):
                          its grid, S(k) on the 50 wave-vector shells
   kshells.json        -- the first 50 shells of config/kVectors<D>D.json and
                          config/kNorm<D>D.csv (data) for D = 1, 2, 3
+  config_items.json   -- names and types of the reference's config items in
+                         registration order (src/TDVMC.cpp:297-349), as data
 
     python tests/golden/make_golden.py obs    # only the observables fixtures
 """
@@ -210,6 +212,15 @@ def kshells():
         json.dump(dict(source="config/kVectors<D>D.json, config/kNorm<D>D.csv (first 50 shells)", shells=out), fh)
 
 
+def config_items():
+    import re
+    with open(os.path.join(REF, "src", "TDVMC.cpp")) as fh:
+        text = fh.read()
+    items = re.findall(r'configItems\.push_back\(ConfigItem\("([A-Z0-9_]+)",[^,]+, ConfigItemType::([A-Z_]+)', text)
+    with open(os.path.join(HERE, "config_items.json"), "w") as fh:
+        json.dump(dict(source="src/TDVMC.cpp:297-349 (registration order)", items=items), fh)
+
+
 def configs():
     for name in ["BosonsBulk1D.config", "InhContactBosonsLinearResponseToPulse.config"]:
         src = os.path.join(REF, "config", name)
@@ -227,4 +238,5 @@ if __name__ == "__main__":
         configs()
     obs_cases()
     kshells()
+    config_items()
     print("golden vectors written to", HERE)

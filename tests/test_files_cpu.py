@@ -5,7 +5,9 @@ as src/TDVMC.cpp:3117-3129 (start), :3833-3839 (per step), :3151-3157 (grids)
 and :3475-3500 (observables).  The values include signed zeros, subnormals,
 huge and tiny magnitudes and infinities.  Host code only."""
 import ctypes as C
+import json
 import os
+import re
 
 import numpy as np
 import pytest
@@ -99,3 +101,66 @@ def test_files_refuse_bad_arguments(tmp_path):
     assert L.tdvmc_files_init(None, _dp(x), _dp(x), 3) == tdvmc.ERR_BAD_ARG
     assert L.tdvmc_files_init((str(tmp_path / "missing") + "/").encode(), _dp(x), _dp(x), 3) == tdvmc.ERR_BAD_ARG
     assert L.tdvmc_files_append_observables(str(tmp_path).encode(), 0.0, None, 2, None, 0) == tdvmc.ERR_BAD_ARG
+
+
+KIND = {"STRING": 0, "INT": 1, "DOUBLE": 2, "ARR_INT": 3, "ARR_DOUBLE": 4}
+EXTENSION_KEYS = ["HBAR2_2M", "PAIR_POTENTIAL", "K_VECTORS", "ONE_BODY", "TRAP_OMEGA", "SPLINE_CUTOFF", "N_WALKERS",
+                  "SAMPLES_PER_WALKER", "SEED", "PRECISION", "REG_EPS", "MOVE_ORDER"]
+
+
+def _ref_json(kind, value):
+    r = o.ref()
+    r.ref_config_json.restype = C.c_int
+    r.ref_config_json.argtypes = [C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int,
+                                  C.c_char_p, C.c_int]
+    buf = C.create_string_buffer(1 << 16)
+    if kind == 0:
+        n = r.ref_config_json(0, value.encode(), 0, None, None, 0, buf, len(buf))
+    elif kind == 1:
+        n = r.ref_config_json(1, None, int(value), None, None, 0, buf, len(buf))
+    elif kind == 2:
+        d = np.array([float(value)])
+        n = r.ref_config_json(2, None, 0, _dp(d), None, 1, buf, len(buf))
+    elif kind == 3:
+        a = np.array(value, dtype=np.int32)
+        n = r.ref_config_json(3, None, 0, None, a.ctypes.data_as(C.POINTER(C.c_int)), len(a), buf, len(buf))
+    else:
+        a = np.array(value, dtype=np.float64)
+        n = r.ref_config_json(4, None, 0, _dp(a) if len(a) else None, None, len(a), buf, len(buf))
+    assert n >= 0
+    return buf.value.decode()
+
+
+@pytest.mark.skipif(not o.ref_available(), reason="needs oracle/_ref (built from the reference checkout)")
+def test_final_config_matches_reference_format(tmp_path, golden_dir):
+    # the reference's own config file, its final parameters replaced
+    with open(os.path.join(golden_dir, "config_items.json")) as fh:
+        ref_items = json.load(fh)["items"]
+    src = os.path.join(golden_dir, "config_BosonsBulk1D.json")
+    with open(src) as fh:
+        cfg = json.load(fh)
+    P = int(cfg["N_PARAM"])
+    rng = np.random.default_rng(3)
+    uR, uI = rng.normal(0, 1, P) * 10.0 ** rng.integers(-8, 8, P), rng.normal(0, 1e-3, P)
+    phiR, phiI = -12.3456789123, 4.5e-7
+    out = str(tmp_path / "AAFinish_tVMC.config")
+    L = tdvmc.lib()
+    assert L.tdvmc_write_final_config(src.encode(), _dp(uR), _dp(uI), phiR, phiI, P, out.encode()) == 0
+    text = open(out).read()
+    assert text.startswith("{\n") and text.endswith("\n}")
+    pairs = re.findall(r'\t"([A-Z0-9_]+)" : (.*?)(?:,\n|\n)(?=\t"|\}$)', text[2:], re.S)
+    names = [n for n, _ in pairs]
+    assert names == [n for n, _ in ref_items] + EXTENSION_KEYS
+    final = dict(PARAMS_REAL=list(uR), PARAMS_IMAGINARY=list(uI), PARAM_PHIR=phiR, PARAM_PHII=phiI)
+    for (name, value_text), (rname, rtype) in zip(pairs, ref_items):
+        kind = KIND[rtype]
+        v = final.get(name, cfg.get(name))
+        if v is None or (isinstance(v, int) and not isinstance(v, bool) and v == 0 and name not in final):
+            v = {0: "", 1: 0, 2: 0.0, 3: [], 4: []}[kind]   # missing key or explicit integer 0 (ConfigItem::setValue)
+        assert value_text == _ref_json(kind, v), name
+    # and it reads back (integers exact, doubles to the reference's six decimals, parameters to 15 digits)
+    info = tdvmc.parse_config(out)
+    assert info.N == cfg["N"] and info.DIM == cfg["DIM"] and info.P == P
+    back = json.loads(text)
+    np.testing.assert_allclose(back["PARAMS_REAL"], uR, rtol=1e-14)
+    assert back["PARAM_PHIR"] == float("%f" % phiR)
