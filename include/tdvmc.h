@@ -304,7 +304,9 @@ int tdvmc_batch_autotune(tdvmc_batch* b, int precision, int* waves_out, int* lan
 int tdvmc_batch_sweep(tdvmc_batch* b, int64_t n_moves, uint64_t seed, int precision);
 /* measurement + moments of every system; out [n_systems] (may be NULL) */
 int tdvmc_batch_measure(tdvmc_batch* b, int precision, tdvmc_step_data* out);
-/* one Euler / RK4 TDVP step of every system with its own fresh samples */
+/* one TDVP step of every system with its own samples: ODE types 0, 1, 2, 3
+ * and 5 as in tdvmc_tdvp_step (types 3 and 5 measure the samples of k1
+ * again in their later stages) */
 int tdvmc_batch_tdvp_step(tdvmc_batch* b, double dt, int ode_type, int precision, int use_graph);
 
 #ifdef __cplusplus

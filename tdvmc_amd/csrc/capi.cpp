@@ -1386,14 +1386,18 @@ int batch_prep(tdvmc_batch* b, int prec)
 }
 
 // x (concatenated) -> xdot for stage k: parameters, tables, sweeps, measurement, moments, solves
-int batch_rhs(tdvmc_batch* b, int prec, const double* x, int k)
+// reuse: measure the walkers of the previous stage again instead of sweeping
+// (batches take one sample per walker, so the walkers are the samples)
+int batch_rhs(tdvmc_batch* b, int prec, const double* x, int k, bool reuse = false)
 {
     const int n = (int)b->sys.size();
     BHIPC(launch_unpack_batch<double>(b->dBpD, n, x, b->stream));
     int rc = batch_prep(b, prec);
     if (rc) return rc;
-    rc = batch_sweep_launch(b, prec);
-    if (rc) return rc;
+    if (!reuse) {
+        rc = batch_sweep_launch(b, prec);
+        if (rc) return rc;
+    }
     rc = batch_measure_launch(b, prec);
     if (rc) return rc;
     BHIPC(launch_solve_batch(b->dSolve[k], n, b->P_max, b->stream));
@@ -1409,14 +1413,28 @@ int batch_step_enqueue(tdvmc_batch* b, int prec, int ode, double dt)
     if (ode == 0) {
         if ((rc = batch_rhs(b, prec, b->dX, 0))) return rc;
         BHIPC(launch_axpy_params(P2, b->dX, b->dK[0], dt, b->dX, b->stream));
+    } else if (ode == 1 || ode == 5) {
+        // predictor-corrector (src/TDVMC.cpp:1855-1910), reusing the samples of
+        // k1 with six correctors for type 5 (:1912-1967)
+        const bool reuse = ode == 5;
+        if ((rc = batch_rhs(b, prec, b->dX, 0))) return rc;
+        BHIPC(launch_axpy_params(P2, b->dX, b->dK[0], dt, b->dXtmp, b->stream));
+        for (int c = 0; c < (reuse ? 6 : 1); c++) {
+            if ((rc = batch_rhs(b, prec, b->dXtmp, 1, reuse))) return rc;
+            BHIPC(launch_pc_combine(P2, b->dX, b->dK[0], b->dK[1], dt, b->dXtmp, b->stream));
+        }
+        BHIPC(hipMemcpyAsync(b->dX, b->dXtmp, (size_t)b->xtotal * sizeof(double), hipMemcpyDeviceToDevice,
+                             b->stream));
     } else {
+        // RK4 (type 2) or RK4 reusing the samples of k1 (type 3, src/TDVMC.cpp:2037-2103)
+        const bool reuse = ode == 3;
         if ((rc = batch_rhs(b, prec, b->dX, 0))) return rc;
         BHIPC(launch_axpy_params(P2, b->dX, b->dK[0], dt / 2.0, b->dXtmp, b->stream));
-        if ((rc = batch_rhs(b, prec, b->dXtmp, 1))) return rc;
+        if ((rc = batch_rhs(b, prec, b->dXtmp, 1, reuse))) return rc;
         BHIPC(launch_axpy_params(P2, b->dX, b->dK[1], dt / 2.0, b->dXtmp, b->stream));
-        if ((rc = batch_rhs(b, prec, b->dXtmp, 2))) return rc;
+        if ((rc = batch_rhs(b, prec, b->dXtmp, 2, reuse))) return rc;
         BHIPC(launch_axpy_params(P2, b->dX, b->dK[2], dt, b->dXtmp, b->stream));
-        if ((rc = batch_rhs(b, prec, b->dXtmp, 3))) return rc;
+        if ((rc = batch_rhs(b, prec, b->dXtmp, 3, reuse))) return rc;
         BHIPC(launch_rk4_combine(P2, b->dX, b->dK[0], b->dK[1], b->dK[2], b->dK[3], dt, b->stream));
     }
     BHIPC(launch_gather_batch<double>(b->dBpD, n, b->dX, 0, b->stream));
@@ -2672,8 +2690,9 @@ int tdvmc_batch_tdvp_step(tdvmc_batch* b, double dt, int ode_type, int precision
     int prec, rc = batch_precision(b, precision, &prec);
     if (rc) return rc;
     const int ode = ode_type < 0 ? b->sys[0]->ode_type : ode_type;
-    if (ode != 0 && ode != 2) {
-        b->err = "ODE type must be 0 (Euler) or 2 (RK4)";
+    if (ode != 0 && ode != 1 && ode != 2 && ode != 3 && ode != 5) {
+        b->err = "ODE type must be 0 (Euler), 1 (predictor-corrector), 2 (RK4), 3 (RK4 reusing samples) or 5 "
+                 "(predictor-corrector reusing samples)";
         return TDVMC_ERR_BAD_ARG;
     }
     // parameters set through the member contexts live in their packed vectors,

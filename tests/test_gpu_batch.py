@@ -187,3 +187,52 @@ def test_batch_autotune_keeps_results():
         assert rel(after[i]["S"], m["S"]) < 1e-10
         assert rel(after[i]["FR"], m["FR"]) < 1e-9
         assert rel(mixed[i]["mO"], m["mO"]) < 2e-6
+
+
+def _f(s, R, x, eps=1e-3):
+    P = s.P
+    out = o.eval_configs(s, R, x[:P], x[P:2 * P])
+    m = o.moments(out["O"], out["ER"], out["EI"])
+    r = o.solve_param_dot(m, imag_time=1, precondition=0, eps=eps)
+    return np.concatenate([r["xR"], r["xI"], [r["phiDotR"], r["phiDotI"]]])
+
+
+def _oracle_step(s, R, x0, ode, dt):
+    """Predictor-corrector (src/TDVMC.cpp:1855-1910; six correctors on the same
+    samples for type 5, :1912-1967) or RK4 on the samples of k1 (:2037-2103)."""
+    k1 = _f(s, R, x0)
+    if ode == 3:
+        k2 = _f(s, R, x0 + dt / 2 * k1)
+        k3 = _f(s, R, x0 + dt / 2 * k2)
+        k4 = _f(s, R, x0 + dt * k3)
+        return x0 + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+    xp = x0 + dt * k1
+    for _ in range(6 if ode == 5 else 1):
+        xp = x0 + (k1 + _f(s, R, xp)) * (dt / 2)
+    return xp
+
+
+@pytest.mark.parametrize("ode", [1, 3, 5])
+def test_batch_pc_and_reuse_steps_vs_oracle(ode):
+    # type 1 samples afresh for its corrector, so it runs without moves; types
+    # 3 and 5 run with moves: their later stages measure the samples of k1,
+    # which are the walkers each system's step leaves behind
+    therm = 0 if ode == 1 else 20
+    cfgs = _cfgs(therm=therm, ns=(3, 10, 20), IMAGINARY_TIME=1, REG_EPS=1e-3)
+    dt = 0.01
+    print("batch ode", ode, flush=True)
+    with tdvmc.Batch(cfgs) as b:
+        start = []
+        for i, c in enumerate(b.systems):
+            s, uR, uI = _params(cfgs[i], i)
+            c.set_params(uR, uI, 0.1, -0.2)
+            start.append((s, np.concatenate([uR, uI, [0.1, -0.2]]), c.get_walkers()))
+        b.tdvp_step(dt, ode_type=ode, precision=tdvmc.FP64)
+        for i, c in enumerate(b.systems):
+            s, x0, R0 = start[i]
+            R = c.get_walkers()
+            assert (ode == 1) == np.array_equal(R, R0), f"system {i}"
+            x1 = _oracle_step(s, R, x0, ode, dt)
+            gR, gI, pR, pI = c.get_params()
+            got = np.concatenate([gR, gI, [pR, pI]])
+            assert np.max(np.abs(got - x1)) < 1e-9 * np.max(np.abs(x1 - x0)) + 1e-13, f"system {i}"
