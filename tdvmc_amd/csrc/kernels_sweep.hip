@@ -282,8 +282,7 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
             const T lu = bcast<LPW>(logu, tt);
             const T us = bcast<LPW>(uself, tt);
             T delta = (T)0;
-#pragma unroll 1
-            for (int i = gl; i < N; i += LPW) {
+            auto partner = [&](int i) {
                 T ro2 = (T)0, rn2 = (T)0;
 #pragma unroll
                 for (int c = 0; c < D; c++) {
@@ -295,7 +294,15 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
                 const T uo = pair_u<T, PER>(Prec<T>::sqrt_(ro2), rc, hinv, Ps, spl, u_out);
                 const T un = pair_u<T, PER>(Prec<T>::sqrt_(rn2), rc, hinv, Ps, spl, u_out);
                 delta += un - uo;
-            }
+            };
+            // rounds in which every lane of the group has a partner run under a
+            // bound common to the group (loop control on scalar registers), the
+            // last, partial round under the lane's own test; same partners, same
+            // order as one loop over i = gl, gl + LPW, ... < N
+            const int nfull = N / LPW;
+#pragma unroll 1
+            for (int m = 0; m < nfull; m++) partner(gl + m * LPW);
+            if (gl + nfull * LPW < N) partner(gl + nfull * LPW);
             T oc[D], ncd[D];
 #pragma unroll
             for (int c = 0; c < D; c++) {
