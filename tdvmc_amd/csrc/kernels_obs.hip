@@ -13,7 +13,8 @@
 //   S(k): the shells' wave vectors are integer n with non-negative
 //   components, k = 2 pi n / L, so exp(i k.r) = prod_a e_a^{n_a} with
 //   e_a = exp(2 pi i x_a / L).  Per chunk of particles the powers e_a^m,
-//   m = 0..nmax, are tabulated in LDS; a thread then forms the products for
+//   m = 0..nmax, are tabulated in LDS (particle index fastest, so a thread
+//   walks three fixed rows with immediate offsets); a thread then forms the products for
 //   its wave vectors (D-1 complex multiplies per particle and vector) and
 //   adds them up (the vectors past the last multiple of 256 are split over
 //   the threads by particle residue).  One thread per shell sums its
@@ -67,9 +68,16 @@ __global__ __launch_bounds__(kObsThreads) void obs_kernel(ObsArgs a)
     const int nm = a.nmax + 1;
     double* pos = reinterpret_cast<double*>(smem);                           // [N][4] (x, y, z, 0)
     uint4* fix = reinterpret_cast<uint4*>(pos + 4 * (size_t)N);               // [N] box fractions (mixed form)
-    double* part = reinterpret_cast<double*>(fix + (sizeof(T) == 4 ? N : 0));   // [V][2] vector totals
-    Cx<T>* tab = reinterpret_cast<Cx<T>*>(part + 2 * (size_t)a.V);            // [chunk][D][nm]
-    unsigned* cnt = reinterpret_cast<unsigned*>(tab + (size_t)a.chunk * D * nm);   // [count]
+    // S(k) work split (below): R rounds of 256 vectors, Vr = V mod 256 vectors
+    // left over, split into G items each
+    const int R = a.V >> 8, Vr = a.V & 255;
+    const int G = Vr ? kObsThreads / Vr : 1;
+    double* part = reinterpret_cast<double*>(fix + (sizeof(T) == 4 ? N : 0));   // [256 R + Vr G][2] totals
+    // phase tables [D][nm][chunk + 1]: the particle index runs fastest, and the
+    // padded row keeps the rows of different powers in different banks
+    const int CS = a.chunk + 1;
+    Cx<T>* tab = reinterpret_cast<Cx<T>*>(part + 2 * (size_t)(256 * R + Vr * G));
+    unsigned* cnt = reinterpret_cast<unsigned*>(tab + (size_t)D * nm * CS);   // [count]
 
     const double* Rw = a.R + w * (int64_t)N * D;
     for (int e = tid; e < N * 4; e += kObsThreads) {
@@ -166,8 +174,6 @@ __global__ __launch_bounds__(kObsThreads) void obs_kernel(ObsArgs a)
     // i = g mod G, so every thread has at most one such item.  Per-vector and
     // per-item totals meet in LDS and are added per shell in a fixed order.
     constexpr int kRounds = 4;
-    const int R = a.V >> 8, Vr = a.V & 255;
-    const int G = Vr ? kObsThreads / Vr : 1;
     const bool has_item = tid < Vr * G;
     const int ir = has_item ? tid % Vr : 0, ig = has_item ? tid / Vr : 0;
     int nv[kRounds + 1][3];
@@ -180,10 +186,15 @@ __global__ __launch_bounds__(kObsThreads) void obs_kernel(ObsArgs a)
         for (int c = 0; c < 3; c++) nv[q][c] = (ok && c < D) ? a.kvec[v * 4 + c] : 0;
         accR[q] = accI[q] = 0.0;
     }
-    auto term = [&](const Cx<T>* row, const int* n) {
-        Cx<T> z = row[n[0]];
-        if constexpr (D > 1) z = cmul(z, row[nm + n[1]]);
-        if constexpr (D > 2) z = cmul(z, row[2 * nm + n[2]]);
+    // this thread's table rows: power n[c] of axis c, all particles of the chunk
+    auto rows = [&](const int* n, const Cx<T>** r) {
+#pragma unroll
+        for (int c = 0; c < D; c++) r[c] = tab + ((size_t)c * nm + n[c]) * CS;
+    };
+    auto term = [&](const Cx<T>* const* r, int il) {
+        Cx<T> z = r[0][il];
+        if constexpr (D > 1) z = cmul(z, r[1][il]);
+        if constexpr (D > 2) z = cmul(z, r[2][il]);
         return z;
     };
     for (int c0 = 0; c0 < N; c0 += a.chunk) {
@@ -193,14 +204,14 @@ __global__ __launch_bounds__(kObsThreads) void obs_kernel(ObsArgs a)
             const int il = e / D, c = e - il * D;
             double t = pos[(c0 + il) * 4 + c] * a.Lr;
             t -= __builtin_floor(t);
-            Cx<T>* row = tab + ((size_t)il * D + c) * nm;
+            Cx<T>* col = tab + (size_t)c * nm * CS + il;   // power m at col[m * CS]
             if constexpr (sizeof(T) == 8) {
                 double sn, cs;
                 sincospi(2.0 * t, &sn, &cs);
                 const Cx<T> e1{cs, sn};
                 Cx<T> pw{1.0, 0.0};
                 for (int m = 0; m < nm; m++) {
-                    row[m] = pw;
+                    col[(size_t)m * CS] = pw;
                     pw = cmul(pw, e1);
                 }
             } else {
@@ -209,7 +220,7 @@ __global__ __launch_bounds__(kObsThreads) void obs_kernel(ObsArgs a)
                     u -= __builtin_floor(u);
                     float sn, cs;
                     Prec<float>::sincos_small((float)u, &sn, &cs);
-                    row[m] = Cx<T>{cs, sn};
+                    col[(size_t)m * CS] = Cx<T>{cs, sn};
                 }
             }
         }
@@ -217,9 +228,12 @@ __global__ __launch_bounds__(kObsThreads) void obs_kernel(ObsArgs a)
 #pragma unroll
         for (int q = 0; q < kRounds; q++) {
             if (q >= R) break;
+            const Cx<T>* r[D];
+            rows(nv[q], r);
             T sr = (T)0, si = (T)0;
+#pragma unroll 4
             for (int il = 0; il < nc; il++) {
-                const Cx<T> z = term(tab + (size_t)il * D * nm, nv[q]);
+                const Cx<T> z = term(r, il);
                 sr += z.re;
                 si += z.im;
             }
@@ -227,9 +241,11 @@ __global__ __launch_bounds__(kObsThreads) void obs_kernel(ObsArgs a)
             accI[q] += (double)si;
         }
         if (has_item) {
+            const Cx<T>* r[D];
+            rows(nv[kRounds], r);
             T sr = (T)0, si = (T)0;
             for (int il = (ig - c0 % G + G) % G; il < nc; il += G) {
-                const Cx<T> z = term(tab + (size_t)il * D * nm, nv[kRounds]);
+                const Cx<T> z = term(r, il);
                 sr += z.re;
                 si += z.im;
             }
@@ -303,7 +319,7 @@ size_t obs_lds_bytes(int N, int D, int V, int chunk, int nmax, int count, bool f
     const size_t cx = fp32 ? 2 * sizeof(float) : 2 * sizeof(double);
     const int R = V >> 8, Vr = V & 255, G = Vr ? 256 / Vr : 0;
     return (size_t)N * 4 * sizeof(double) + (fp32 ? (size_t)N * 16 : 0) + 2 * (size_t)(256 * R + Vr * G) * sizeof(double) +
-           (size_t)chunk * D * (nmax + 1) * cx + (size_t)count * sizeof(unsigned);
+           (size_t)(chunk + 1) * D * (nmax + 1) * cx + (size_t)count * sizeof(unsigned);
 }
 
 template <typename T>

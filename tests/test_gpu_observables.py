@@ -45,6 +45,10 @@ def test_fp64_vs_reference_golden(path):
         np.testing.assert_array_equal(gg, z["gr_grid"])
         np.testing.assert_allclose(sg, z["sk_grid"], rtol=3e-15, atol=0)
         gr, sk = ctx.observables_configs(z["R"], precision=tdvmc.FP64)
+        # the same configurations again: bit for bit (fixed-order sums, integer counts)
+        gr2, sk2 = ctx.observables_configs(z["R"], precision=tdvmc.FP64)
+        np.testing.assert_array_equal(gr2, gr)
+        np.testing.assert_array_equal(sk2, sk)
     np.testing.assert_allclose(gr, z["gr"], rtol=1e-12, atol=0)
     err = np.max(np.abs(sk - z["sk"])) / np.max(np.abs(z["sk"]))
     print(f"fp64 S(k) max rel err {err:.2e}")
@@ -61,6 +65,10 @@ def test_mixed_vs_oracle(D, N, L, P, bins):
     with tdvmc.Context(make_cfg(N, D, L, P, GR_BIN_COUNT=bins)) as ctx:
         gr, sk = ctx.observables_configs(R, precision=tdvmc.MIXED)
         gr64, sk64 = ctx.observables_configs(R, precision=tdvmc.FP64)
+        for prec, (g0, s0) in ((tdvmc.MIXED, (gr, sk)), (tdvmc.FP64, (gr64, sk64))):
+            g1, s1 = ctx.observables_configs(R, precision=prec)
+            np.testing.assert_array_equal(g1, g0)
+            np.testing.assert_array_equal(s1, s0)
     ogr, osk = o.observables(R, L, P, L / 2, bins)
     np.testing.assert_allclose(gr64, ogr, rtol=1e-12, atol=0)
     c, co = _counts(gr, D, N, P, L, bins), _counts(ogr, D, N, P, L, bins)
