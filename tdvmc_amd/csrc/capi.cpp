@@ -162,7 +162,7 @@ template <typename T>
 hipError_t dalloc(T** p, size_t n)
 {
     if (*p) {
-        hipFree(*p);
+        (void)hipFree(*p);
         *p = nullptr;
     }
     if (n == 0) return hipSuccess;
@@ -172,7 +172,7 @@ hipError_t dalloc(T** p, size_t n)
 template <typename T>
 void dfree(T*& p)
 {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
 }
 
@@ -476,7 +476,7 @@ int seg_cut(tdvmc_ctx* ctx)
                                       ctx->seg_last ? 1 : 0, g);
         if (e == hipSuccess) ctx->seg_last = node;
     }
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     HIPC(e);
     return TDVMC_OK;
 }
@@ -500,7 +500,7 @@ int prof_begin(tdvmc_ctx* ctx, const char* name, hipEvent_t* a)
     if (ctx->capturing) {
         int rc = seg_event(ctx, *a);
         if (rc) {
-            hipEventDestroy(*a);
+            (void)hipEventDestroy(*a);
             return rc;
         }
         return TDVMC_OK;
@@ -535,8 +535,8 @@ int prof_flush(tdvmc_ctx* ctx)
         acc.ms += ms;
         acc.launches++;
         if (p.owned) {
-            hipEventDestroy(p.a);
-            hipEventDestroy(p.b);
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
         }
     }
     ctx->pending.clear();
@@ -545,13 +545,13 @@ int prof_flush(tdvmc_ctx* ctx)
 
 void drop_graph(tdvmc_ctx* ctx)
 {
-    if (ctx->graph) hipGraphExecDestroy(ctx->graph);
-    if (ctx->graph_src) hipGraphDestroy(ctx->graph_src);
+    if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
+    if (ctx->graph_src) (void)hipGraphDestroy(ctx->graph_src);
     ctx->graph = nullptr;
     ctx->graph_src = nullptr;
     for (auto& p : ctx->graph_ev) {
-        hipEventDestroy(p.a);
-        hipEventDestroy(p.b);
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
     }
     ctx->graph_ev.clear();
     ctx->graph_key.clear();
@@ -580,7 +580,7 @@ int ensure_moments(tdvmc_ctx* ctx, int64_t n)
     moments_work_bytes(n, P, 1, &x1, &p1, &c1);
     size_t xb = std::max(x0, x1), pb = std::max(p0, p1), cb = std::max(c0, c1);
     void* xt = ctx->mw.xt;
-    if (xt) hipFree(xt);
+    if (xt) (void)hipFree(xt);
     ctx->mw.xt = nullptr;
     HIPC(hipMalloc(&ctx->mw.xt, xb));
     HIPC(dalloc(&ctx->mw.partial, (pb + 7) / 8));
@@ -1072,7 +1072,7 @@ int alloc_ctx(tdvmc_ctx* ctx)
 
 void release(tdvmc_ctx* ctx)
 {
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     dfree(ctx->obs.dVec);
     dfree(ctx->obs.dOff);
     dfree(ctx->obs.dScale);
@@ -1081,8 +1081,8 @@ void release(tdvmc_ctx* ctx)
     dfree(ctx->obs.dSkW);
     for (auto& p : ctx->pending) {
         if (p.owned) {
-            hipEventDestroy(p.a);
-            hipEventDestroy(p.b);
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
         }
     }
     ctx->pending.clear();
@@ -1118,12 +1118,12 @@ void release(tdvmc_ctx* ctx)
     dfree(ctx->dMean);
     dfree(ctx->dS);
     dfree(ctx->dF);
-    if (ctx->mw.xt) hipFree(ctx->mw.xt);
+    if (ctx->mw.xt) (void)hipFree(ctx->mw.xt);
     dfree(ctx->mw.partial);
     dfree(ctx->mw.colsum);
     dfree(ctx->dWork);
     dfree(ctx->dStatus);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 }
 
 }  // namespace
@@ -1444,8 +1444,8 @@ int batch_step_enqueue(tdvmc_batch* b, int prec, int ode, double dt)
 
 void batch_drop_graph(tdvmc_batch* b)
 {
-    if (b->graph) hipGraphExecDestroy(b->graph);
-    if (b->graph_src) hipGraphDestroy(b->graph_src);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->graph_src) (void)hipGraphDestroy(b->graph_src);
     b->graph = nullptr;
     b->graph_src = nullptr;
     b->graph_key.clear();
@@ -1453,7 +1453,7 @@ void batch_drop_graph(tdvmc_batch* b)
 
 void batch_release(tdvmc_batch* b)
 {
-    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     batch_drop_graph(b);
     dfree(b->dSys);
     dfree(b->dTabF);
@@ -1476,7 +1476,7 @@ void batch_release(tdvmc_batch* b)
     dfree(b->dEpoch);
     for (auto* c : b->sys) tdvmc_destroy(c);
     b->sys.clear();
-    if (b->stream) hipStreamDestroy(b->stream);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     b->stream = nullptr;
 }
 
@@ -1534,9 +1534,9 @@ int batch_time_sweep(tdvmc_batch* b, int s, int lanes, const KTables<T>* tabs, d
         }
         *ms = tot / kReps;
     }
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    hipStreamSynchronize(b->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(b->stream);
     dfree(dA);
     dfree(dStart);
     return rc;
@@ -1591,9 +1591,9 @@ int batch_time_eval(tdvmc_batch* b, int s, int wpw, const EvalBatchItem<T>* item
             }
         }
     }
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    hipStreamSynchronize(b->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(b->stream);
     dfree(dIt);
     dfree(dStart);
     return rc;
@@ -2106,7 +2106,7 @@ int tdvmc_moments(tdvmc_ctx* ctx, const double* O, const double* ER, const doubl
     if (F) step(hipMemcpyAsync(F, dF, 2 * (size_t)P * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     step(hipStreamSynchronize(ctx->stream));
     dfree(dO); dfree(dE); dfree(dM); dfree(dS); dfree(dF); dfree(dPart); dfree(dCol);
-    if (dXt) hipFree(dXt);
+    if (dXt) (void)hipFree(dXt);
     if (e != hipSuccess) {
         ctx->err = std::string("moments: ") + hipGetErrorString(e);
         return TDVMC_ERR_HIP;
@@ -2202,15 +2202,15 @@ int tdvmc_tdvp_step(tdvmc_ctx* ctx, double dt, int ode_type, int precision, int 
                 ctx->capturing = false;
                 if (ctx->cap_active) {
                     hipGraph_t g = nullptr;
-                    hipStreamEndCapture(ctx->stream, &g);
-                    if (g) hipGraphDestroy(g);
+                    (void)hipStreamEndCapture(ctx->stream, &g);
+                    if (g) (void)hipGraphDestroy(g);
                     ctx->cap_active = false;
                 }
                 hipGraph_t top = ctx->seg_top;
                 ctx->seg_top = nullptr;
                 ctx->seg_last = nullptr;
                 if (rc) {
-                    hipGraphDestroy(top);
+                    (void)hipGraphDestroy(top);
                     drop_graph(ctx);
                     return rc;
                 }
@@ -2221,7 +2221,7 @@ int tdvmc_tdvp_step(tdvmc_ctx* ctx, double dt, int ode_type, int precision, int 
                 hipGraph_t g = nullptr;
                 hipError_t e = hipStreamEndCapture(ctx->stream, &g);
                 if (rc) {
-                    if (g) hipGraphDestroy(g);
+                    if (g) (void)hipGraphDestroy(g);
                     drop_graph(ctx);
                     return rc;
                 }
@@ -2430,8 +2430,8 @@ int tdvmc_autotune(tdvmc_ctx* ctx, int precision, int* lanes_out)
             best_ms = ms;
         }
     }
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     dfree(scratch);
     if (e != hipSuccess) {
         ctx->err = std::string("autotune: ") + hipGetErrorString(e);
@@ -2709,7 +2709,7 @@ int tdvmc_batch_tdvp_step(tdvmc_batch* b, double dt, int ode_type, int precision
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(b->stream, &g);
             if (rc) {
-                if (g) hipGraphDestroy(g);
+                if (g) (void)hipGraphDestroy(g);
                 return rc;
             }
             BHIPC(e);
