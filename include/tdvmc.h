@@ -234,8 +234,9 @@ int tdvmc_solve(tdvmc_ctx* ctx, const double* mean, const double* S, const doubl
  * ode_type < 0 uses the configured ODE_SOLVER_TYPE.  With use_graph != 0 the
  * step is captured once into a hipGraph and replayed.  After the step
  * USE_NORMALIZE_WF sets phi R = -exponent of walker 0's last sample and
- * USE_ADJUST_PARAMETERS subtracts alpha[P-1] from every alpha
- * (src/TDVMC.cpp:3757-3775, :2525-2567). */
+ * USE_ADJUST_PARAMETERS subtracts the last pair-spline parameter from the
+ * N_PARAM pair-spline parameters, real and imaginary parts alike (one-body and
+ * k-space parameters are left as they are; src/TDVMC.cpp:3757-3775, :2525-2567). */
 int tdvmc_tdvp_step(tdvmc_ctx* ctx, double dt, int ode_type, int precision, int use_graph);
 
 /* Non-positive Cholesky pivots met by the solves of the last TDVP step (the

@@ -109,6 +109,7 @@ struct tdvmc_ctx {
     bool prof = false;
     bool capturing = false;     // building a profiled graph from captured segments
     bool cap_active = false;    // a stream capture is open
+    bool in_capture = false;    // enqueueing a TDVP step into a graph (its replays count the trials)
     hipGraph_t seg_top = nullptr;
     hipGraphNode_t seg_last = nullptr;
     struct Pending {
@@ -120,6 +121,9 @@ struct tdvmc_ctx {
     std::vector<Pending> graph_ev;   // event pairs recorded by nodes of the captured graph
     std::map<std::string, Prof> prof_acc;
     int64_t trials_total = 0;
+    // bumped whenever a value that a batch's device descriptors copy changes
+    // (the walker count and the buffers sized by it, the pair potential in force)
+    uint64_t gen = 0;
 
     // pair distribution and S(k) (BosonsBulk's additional observables), set up on first use
     struct Obs {
@@ -222,9 +226,15 @@ void resolve_potential(tdvmc_ctx* ctx)
         a = p[3];
         b = p[4];
     }
+    if (a != ctx->sy.pot_a || b != ctx->sy.pot_b) ctx->gen++;
     ctx->sy.pot_a = a;
     ctx->sy.pot_b = b;
 }
+
+// Fresh sampling stages of one TDVP step of ODE type `ode` (the stages that
+// sweep): Euler 1, predictor-corrector 2 (k1 and the corrector), RK4 4; the
+// variants reusing the samples of k1 (types 3 and 5) sample once.
+int fresh_stages(int ode) { return ode == 2 ? 4 : (ode == 1 ? 2 : 1); }
 
 int build_system(tdvmc_ctx* ctx)
 {
@@ -405,8 +415,8 @@ int upload_params(tdvmc_ctx* ctx)
     packed[2 * P] = ctx->phiR;
     packed[2 * P + 1] = ctx->phiI;
     HIPC(hipMemcpyAsync(ctx->dX, packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    HIPC(hipMemcpyAsync(ctx->dUR, ctx->uR.data(), P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    HIPC(hipMemcpyAsync(ctx->dUI, ctx->uI.data(), P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    // device copies [P + 1]: the parameters, then phi R / phi I
+    HIPC(launch_unpack_params(P, ctx->dX, ctx->dUR, ctx->dUI, ctx->stream));
     ctx->tf.dirty = ctx->td.dirty = true;
     ctx->host_params_stale = false;
     return TDVMC_OK;
@@ -626,7 +636,7 @@ int enqueue_sweep(tdvmc_ctx* ctx, int prec, int64_t n_moves, uint64_t seed, bool
     rc = prof_end(ctx, "sweep", ev);
     if (rc) return rc;
     HIPC(launch_add_u32(ctx->dEpoch, 1u, ctx->stream));
-    ctx->trials_total += n_moves * ctx->W;
+    if (!ctx->in_capture) ctx->trials_total += n_moves * ctx->W;
     return TDVMC_OK;
 }
 
@@ -643,12 +653,12 @@ int enqueue_eval(tdvmc_ctx* ctx, int prec, int64_t row0, bool use_rho = false, c
     if (prec == TDVMC_MIXED) {
         HIPC((launch_eval<float, float>(ctx->sy, tables_of(ctx, ctx->tf), R, ctx->W, ctx->dExpo + row0,
                                         ctx->dER + row0, ctx->dEI + row0, ctx->dO32 + row0 * P,
-                                        ctx->dOther + row0 * 9, nullptr, nullptr, nullptr, ctx->phiR,
+                                        ctx->dOther + row0 * 9, nullptr, nullptr, nullptr,
                                         rho ? reinterpret_cast<const float*>(ctx->dRho) : nullptr, ctx->stream)));
     } else {
         HIPC((launch_eval<double, double>(ctx->sy, tables_of(ctx, ctx->td), R, ctx->W, ctx->dExpo + row0,
                                           ctx->dER + row0, ctx->dEI + row0, ctx->dO64 + row0 * P,
-                                          ctx->dOther + row0 * 9, nullptr, nullptr, nullptr, ctx->phiR,
+                                          ctx->dOther + row0 * 9, nullptr, nullptr, nullptr,
                                           rho ? ctx->dRho : nullptr, ctx->stream)));
     }
     return prof_end(ctx, "eval", ev);
@@ -965,7 +975,7 @@ int enqueue_tdvp_step(tdvmc_ctx* ctx, int prec, int ode, double dt)
     // here walker 0 at the last sample), then the gauge shift
     if (ctx->normalize_wf || ctx->adjust_params)
         HIPC(launch_post_step(P, ctx->dX, ctx->dExpo + (int64_t)(ctx->spw - 1) * ctx->W, ctx->normalize_wf,
-                              ctx->adjust_params, ctx->stream));
+                              ctx->adjust_params, ctx->sy.off_pair, ctx->sy.P_s, ctx->stream));
     HIPC(launch_unpack_params(P, ctx->dX, ctx->dUR, ctx->dUI, ctx->stream));
     return TDVMC_OK;
 }
@@ -1033,8 +1043,8 @@ int alloc_ctx(tdvmc_ctx* ctx)
     const int P = P_of(ctx);
     HIPC(hipGetDevice(&ctx->device));
     HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    HIPC(dalloc(&ctx->dUR, P));
-    HIPC(dalloc(&ctx->dUI, P));
+    HIPC(dalloc(&ctx->dUR, P + 1));
+    HIPC(dalloc(&ctx->dUI, P + 1));
     HIPC(dalloc(&ctx->dX, 2 * P + 2));
     HIPC(dalloc(&ctx->dXtmp, 2 * P + 2));
     HIPC(dalloc(&ctx->dXprev, 2 * P + 2));
@@ -1160,6 +1170,8 @@ struct tdvmc_batch {
     EvalBatchItem<double>* dEvalD = nullptr;
     MomentsItem* dMom = nullptr;
     double* dMomPart = nullptr;       // partial sums of the batched moments
+    int64_t mom_part_cap = 0;         // doubles allocated at dMomPart
+    std::vector<int64_t> desc_W;      // walker count of each system the descriptors and the graph were built for
     int mom_nchunk = 0;
     size_t mom_lds = 0;
     SolveArgs* dSolve[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -1232,15 +1244,26 @@ int batch_eval_wpw(const tdvmc_batch* b, int s)
     return wpw;
 }
 
+void batch_drop_graph(tdvmc_batch* b);
+
 // Descriptor arrays of a batched sweep with n_moves per system (<= 0: each
 // system's MC_NTHERMSTEPS) and of the measurement, moments and solves.
 int batch_upload(tdvmc_batch* b, int64_t n_moves, uint64_t seed, bool want_rho)
 {
-    char key[128];
-    std::snprintf(key, sizeof(key), "%lld/%llu/%d/%d/%d", (long long)n_moves, (unsigned long long)seed,
-                  want_rho ? 1 : 0, b->lanes, b->tune_gen);
+    // the members' generations cover what the descriptors copy from them: the
+    // walker count and the buffers sized by it, and the pair potential in force
+    uint64_t h = 1469598103934665603ull;
+    for (const auto* c : b->sys) h = (h ^ c->gen) * 1099511628211ull;
+    char key[160];
+    std::snprintf(key, sizeof(key), "%lld/%llu/%d/%d/%d/%llx", (long long)n_moves, (unsigned long long)seed,
+                  want_rho ? 1 : 0, b->lanes, b->tune_gen, (unsigned long long)h);
     if (b->desc_key == key) return TDVMC_OK;
     const int n = (int)b->sys.size();
+    // a changed walker count changes the launch shapes baked into a captured
+    // step and the size of the moments' partial sums
+    std::vector<int64_t> Ws(n);
+    for (int s = 0; s < n; s++) Ws[s] = b->sys[s]->W;
+    if (Ws != b->desc_W) batch_drop_graph(b);
     std::vector<KSys> sy(n);
     std::vector<KTables<float>> tf(n);
     std::vector<KTables<double>> td(n);
@@ -1287,9 +1310,9 @@ int batch_upload(tdvmc_batch* b, int64_t n_moves, uint64_t seed, bool want_rho)
         a.lanes_per_walker = lanes;
         a.rho_out = (want_rho && c->sy.Kh > 0) ? static_cast<void*>(c->dRho) : nullptr;
         const bool rho = want_rho && c->sy.Kh > 0;
-        ef[s] = EvalBatchItem<float>{c->dR, c->W, c->dExpo, c->dER, c->dEI, c->dO64, c->dOther, nullptr, c->phiR,
+        ef[s] = EvalBatchItem<float>{c->dR, c->W, c->dExpo, c->dER, c->dEI, c->dO64, c->dOther, nullptr,
                                      rho ? reinterpret_cast<const float*>(c->dRho) : nullptr, wpw, (int64_t)slot_f};
-        ed[s] = EvalBatchItem<double>{c->dR, c->W, c->dExpo, c->dER, c->dEI, c->dO64, c->dOther, nullptr, c->phiR,
+        ed[s] = EvalBatchItem<double>{c->dR, c->W, c->dExpo, c->dER, c->dEI, c->dO64, c->dOther, nullptr,
                                       rho ? c->dRho : nullptr, wpw, (int64_t)slot_d};
         {
             const int cw = moments_chunk_walkers(c->sy.P);
@@ -1314,8 +1337,12 @@ int batch_upload(tdvmc_batch* b, int64_t n_moves, uint64_t seed, bool want_rho)
             sv[k].push_back(v);
         }
     }
-    // partial-sum scratch of the moments: sized by the systems, allocated once
-    if (!b->dMomPart) BHIPC(dalloc(&b->dMomPart, (size_t)part_len));
+    // partial-sum scratch of the moments: sized by the systems' walker counts
+    // (no captured graph refers to it when it grows: the graph was dropped above)
+    if (part_len > b->mom_part_cap) {
+        BHIPC(dalloc(&b->dMomPart, (size_t)part_len));
+        b->mom_part_cap = part_len;
+    }
     b->mom_nchunk = 0;
     b->mom_lds = 0;
     for (auto& m : mi) {
@@ -1340,6 +1367,7 @@ int batch_upload(tdvmc_batch* b, int64_t n_moves, uint64_t seed, bool want_rho)
         if ((rc = upload_array(b, &b->dSolve[k], sv[k]))) return rc;
     BHIPC(hipStreamSynchronize(b->stream));
     b->desc_key = key;
+    b->desc_W = Ws;
     return TDVMC_OK;
 }
 
@@ -1710,6 +1738,7 @@ int tdvmc_set_walker_count(tdvmc_ctx* ctx, int64_t W)
     HIPC(dalloc(&ctx->dRho, (size_t)W * 2 * std::max(ctx->sy.Kh, 1)));
     HIPC(hipMemsetAsync(ctx->dAcc, 0, (size_t)W * sizeof(unsigned long long), ctx->stream));
     ctx->W = W;
+    ctx->gen++;
     ctx->trials_total = 0;
     int rc = ensure_samples(ctx, W * ctx->spw);
     if (rc) return rc;
@@ -1903,10 +1932,10 @@ int tdvmc_eval_configs_dev(tdvmc_ctx* ctx, const double* R_dev, int64_t W, int p
     if (rc) return rc;
     if (prec == TDVMC_MIXED) {
         HIPC((launch_eval<float, double>(ctx->sy, tables_of(ctx, ctx->tf), R_dev, W, expo_dev, ER_dev, EI_dev, O_dev,
-                                         nullptr, grad_dev, lap_dev, nullptr, ctx->phiR, nullptr, st)));
+                                         nullptr, grad_dev, lap_dev, nullptr, nullptr, st)));
     } else {
         HIPC((launch_eval<double, double>(ctx->sy, tables_of(ctx, ctx->td), R_dev, W, expo_dev, ER_dev, EI_dev, O_dev,
-                                          nullptr, grad_dev, lap_dev, nullptr, ctx->phiR, nullptr, st)));
+                                          nullptr, grad_dev, lap_dev, nullptr, nullptr, st)));
     }
     return TDVMC_OK;
 }
@@ -1947,10 +1976,10 @@ int tdvmc_eval_configs(tdvmc_ctx* ctx, const double* R, int64_t W, int precision
         prof_begin(ctx, "eval", &ev);
         if (prec == TDVMC_MIXED)
             ok = hipc((launch_eval<float, double>(ctx->sy, tables_of(ctx, ctx->tf), dR, W, dEx, dER, dEI, dO, dOt, dG,
-                                                  dL, nullptr, ctx->phiR, nullptr, ctx->stream)), "eval kernel");
+                                                  dL, nullptr, nullptr, ctx->stream)), "eval kernel");
         else
             ok = hipc((launch_eval<double, double>(ctx->sy, tables_of(ctx, ctx->td), dR, W, dEx, dER, dEI, dO, dOt,
-                                                   dG, dL, nullptr, ctx->phiR, nullptr, ctx->stream)), "eval kernel");
+                                                   dG, dL, nullptr, nullptr, ctx->stream)), "eval kernel");
         prof_end(ctx, "eval", ev);
     }
     auto down = [&](double* h, double* d, size_t n) {
@@ -2196,10 +2225,12 @@ int tdvmc_tdvp_step(tdvmc_ctx* ctx, double dt, int ode_type, int precision, int 
                 HIPC(hipGraphCreate(&ctx->seg_top, 0));
                 ctx->seg_last = nullptr;
                 ctx->capturing = true;
+                ctx->in_capture = true;
                 rc = seg_begin(ctx);
                 if (!rc) rc = enqueue_tdvp_step(ctx, prec, ode, dt);
                 if (!rc) rc = seg_cut(ctx);
                 ctx->capturing = false;
+                ctx->in_capture = false;
                 if (ctx->cap_active) {
                     hipGraph_t g = nullptr;
                     (void)hipStreamEndCapture(ctx->stream, &g);
@@ -2217,7 +2248,9 @@ int tdvmc_tdvp_step(tdvmc_ctx* ctx, double dt, int ode_type, int precision, int 
                 ctx->graph_src = top;
             } else {
                 HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                ctx->in_capture = true;
                 rc = enqueue_tdvp_step(ctx, prec, ode, dt);
+                ctx->in_capture = false;
                 hipGraph_t g = nullptr;
                 hipError_t e = hipStreamEndCapture(ctx->stream, &g);
                 if (rc) {
@@ -2237,7 +2270,7 @@ int tdvmc_tdvp_step(tdvmc_ctx* ctx, double dt, int ode_type, int precision, int 
         HIPC(hipGraphLaunch(ctx->graph, ctx->stream));
         // the graph's event nodes were re-recorded by this replay
         for (const auto& p : ctx->graph_ev) ctx->pending.push_back(p);
-        ctx->trials_total += (ode == 2 ? 4 : 1) * (ctx->init_moves + ctx->spw * ctx->therm_moves) * ctx->W;
+        ctx->trials_total += fresh_stages(ode) * (ctx->init_moves + ctx->spw * ctx->therm_moves) * ctx->W;
     } else {
         if (ctx->accept_check)
             HIPC(hipMemcpyAsync(ctx->dXprev, ctx->dX, (2 * (size_t)P_of(ctx) + 2) * sizeof(double),
@@ -2315,10 +2348,10 @@ int tdvmc_adjoint(tdvmc_ctx* ctx, const double* R, int64_t W, const double* gbar
         if (rc == TDVMC_OK) {
             if (prec == TDVMC_MIXED)
                 e = launch_eval<float, double>(ctx->sy, tables_of(ctx, ctx->tf), dRin, W, dEx, dER, dEI, dO, nullptr,
-                                               dG, dL, dOuter, ctx->phiR, nullptr, ctx->stream);
+                                               dG, dL, dOuter, nullptr, ctx->stream);
             else
                 e = launch_eval<double, double>(ctx->sy, tables_of(ctx, ctx->td), dRin, W, dEx, dER, dEI, dO, nullptr,
-                                                dG, dL, dOuter, ctx->phiR, nullptr, ctx->stream);
+                                                dG, dL, dOuter, nullptr, ctx->stream);
         }
     }
     if (e == hipSuccess && rc == TDVMC_OK) {
@@ -2698,12 +2731,15 @@ int tdvmc_batch_tdvp_step(tdvmc_batch* b, double dt, int ode_type, int precision
     // parameters set through the member contexts live in their packed vectors,
     // which the step gathers first
     if ((rc = batch_sync_members(b))) return rc;
+    // descriptors current with the members (a direct sweep or measurement, a
+    // new walker count or a quench since the last step); drops a captured
+    // step whose launch shapes no longer fit
+    if ((rc = batch_upload(b, 0, b->seed, true))) return rc;
     if (use_graph) {
         char key[160];
         std::snprintf(key, sizeof(key), "%d/%d/%.17g/%d", prec, ode, dt, b->lanes);
         if (!b->graph || b->graph_key != key) {
             batch_drop_graph(b);
-            if ((rc = batch_upload(b, 0, b->seed, true))) return rc;
             BHIPC(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             rc = batch_step_enqueue(b, prec, ode, dt);
             hipGraph_t g = nullptr;
@@ -2717,11 +2753,8 @@ int tdvmc_batch_tdvp_step(tdvmc_batch* b, double dt, int ode_type, int precision
             BHIPC(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
             b->graph_key = key;
         }
-        // a direct sweep / measurement may have rewritten the descriptors since the capture
-        if ((rc = batch_upload(b, 0, b->seed, true))) return rc;
         BHIPC(hipGraphLaunch(b->graph, b->stream));
     } else {
-        if ((rc = batch_upload(b, 0, b->seed, true))) return rc;
         if ((rc = batch_step_enqueue(b, prec, ode, dt))) return rc;
     }
     BHIPC(hipStreamSynchronize(b->stream));
@@ -2730,7 +2763,7 @@ int tdvmc_batch_tdvp_step(tdvmc_batch* b, double dt, int ode_type, int precision
         c->host_params_stale = true;
         c->time += dt;
         c->step += 1;
-        c->trials_total += (ode == 2 ? 4 : 1) * c->therm_moves * c->W;
+        c->trials_total += fresh_stages(ode) * c->therm_moves * c->W;
         resolve_potential(c);
     }
     return TDVMC_OK;

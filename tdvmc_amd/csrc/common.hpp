@@ -43,8 +43,8 @@ struct KTables {
     const T* alpha;   // [Kh][2]: (Re, Im) of alpha_k (+ alpha_{-k} when paired)
     const T* kag;     // [Kh][8]: Re alpha_k times (kv, |k|^2), then Im alpha_k times the same
     const int* kmap;  // [Kh][2]: parameter slots (relative to off_k) of k and of -k (-1: unpaired)
-    const double* uR; // [P] parameters (FP64 master copy)
-    const double* uI;
+    const double* uR; // [P + 1] parameters (FP64 master copy), then phi R
+    const double* uI; // [P + 1] parameters, then phi I
 };
 
 constexpr int kMaxD = 3;

@@ -21,7 +21,7 @@ size_t eval_lds_bytes(const KSys& sy, bool per_particle, int wpw = 4);
 // rho_k pass), or null to compute it from the coordinates
 template <typename T, typename TO>
 hipError_t launch_eval(const KSys& sy, const KTables<T>& tb, const double* R, int64_t W, double* expo, double* ER,
-                       double* EI, TO* O, double* other, double* grad, double* lap, double* outer, double phiR,
+                       double* EI, TO* O, double* other, double* grad, double* lap, double* outer,
                        const T* rho_in, hipStream_t st);
 
 // ---- Metropolis sweeps (kernels_sweep.hip)
@@ -68,7 +68,6 @@ struct EvalBatchItem {
     double* O;                 // [W][P]
     double* other;             // [W][9] or null
     double* outer;             // [W] or null
-    double phiR;
     const T* rho;              // rho_k handed over by the sweep, or null
     int wpw;                   // wavefronts per walker: 1, 2 or 4 (4 / wpw walkers per workgroup)
     int64_t slot_bytes;        // LDS per walker (eval_lds_bytes)
@@ -130,7 +129,7 @@ hipError_t launch_solve_batch(const SolveArgs* args_dev, int n, int P_max, hipSt
 template <typename T>
 struct BatchParams {
     KSys sy;
-    double* uR;                // [P] this system's parameters (device)
+    double* uR;                // [P + 1] this system's parameters, then phi R (device)
     double* uI;
     const int* kmap;
     const T* kv;
@@ -171,8 +170,10 @@ __host__ __device__ inline int64_t moments_part_len(int P)
 hipError_t launch_moments_batch(const MomentsItem* items_dev, int n, int nchunk_max, size_t lds, int P_max,
                                 hipStream_t st);
 
-// after a step: alpha -= alpha[P-1] (adjust), phi R = -exponent (normalize)
-hipError_t launch_post_step(int P, double* x, const double* expo, int normalize, int adjust, hipStream_t st);
+// after a step: alpha[off .. off+n) -= alpha[off+n-1] in both parts (adjust: the
+// pair-spline block), phi R = -exponent (normalize)
+hipError_t launch_post_step(int P, double* x, const double* expo, int normalize, int adjust, int off, int n,
+                            hipStream_t st);
 // y = x0 + c * xdot  over the 2P+2 entries (alpha R, alpha I, phi R, phi I)
 hipError_t launch_axpy_params(int P, const double* x0, const double* xdot, double c, double* y, hipStream_t st);
 // x += dt/6 (k1 + 2 k2 + 2 k3 + k4)

@@ -52,7 +52,6 @@ struct EvalOut {
     double* grad;     // [W][N][D][2] or null (PP)
     double* lap;      // [W][N][2] or null (PP)
     double* outer;    // [W] count of pairs beyond the cutoff, or null
-    double phiR;
     __device__ __forceinline__ void O_store(int64_t w, int P, int k, double v) const { O[w * P + k] = (TO)v; }
 };
 
@@ -692,7 +691,7 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
             double* o = out.other + w * 9;
             o[0] = kinR;
             o[1] = vpot;
-            o[2] = exp(ex + out.phiR);
+            o[2] = exp(ex + tb.uR[sy.P]);   // phi R of the parameters in force (device copy)
             o[3] = ex;
             o[4] = sR1;
             o[5] = sI1;
@@ -725,7 +724,7 @@ __global__ __launch_bounds__(kThreads, 4) void eval_batch_kernel(EvalBatch<T> bt
             hi = mid;
     }
     const EvalBatchItem<T>& it = bt.item[lo];
-    EvalOut<double> o{it.expo, it.ER, it.EI, it.O, it.other, nullptr, nullptr, it.outer, it.phiR};
+    EvalOut<double> o{it.expo, it.ER, it.EI, it.O, it.other, nullptr, nullptr, it.outer};
     const int64_t b = (int64_t)blockIdx.x - bt.block_start[lo];
     const size_t sb = (size_t)it.slot_bytes;
     switch (it.wpw) {
@@ -793,11 +792,11 @@ static hipError_t launch_eval_d(const KSys& sy, const KTables<T>& tb, const doub
 
 template <typename T, typename TO>
 hipError_t launch_eval(const KSys& sy, const KTables<T>& tb, const double* R, int64_t W, double* expo, double* ER,
-                       double* EI, TO* O, double* other, double* grad, double* lap, double* outer, double phiR,
+                       double* EI, TO* O, double* other, double* grad, double* lap, double* outer,
                        const T* rho_in, hipStream_t st)
 {
     if (W <= 0) return hipSuccess;
-    EvalOut<TO> o{expo, ER, EI, O, other, grad, lap, outer, phiR};
+    EvalOut<TO> o{expo, ER, EI, O, other, grad, lap, outer};
     const bool pp = grad != nullptr;
     switch (sy.D) {
     case 1: return launch_eval_d<T, 1, TO>(sy, tb, R, W, o, rho_in, pp, st);
@@ -839,13 +838,13 @@ template hipError_t launch_eval_batch<double>(const EvalBatch<double>&, int, boo
 template size_t eval_lds_bytes<float>(const KSys&, bool, int);
 template size_t eval_lds_bytes<double>(const KSys&, bool, int);
 template hipError_t launch_eval<double, double>(const KSys&, const KTables<double>&, const double*, int64_t, double*,
-                                                double*, double*, double*, double*, double*, double*, double*, double,
+                                                double*, double*, double*, double*, double*, double*, double*,
                                                 const double*, hipStream_t);
 template hipError_t launch_eval<float, float>(const KSys&, const KTables<float>&, const double*, int64_t, double*,
-                                              double*, double*, float*, double*, double*, double*, double*, double,
+                                              double*, double*, float*, double*, double*, double*, double*,
                                               const float*, hipStream_t);
 template hipError_t launch_eval<float, double>(const KSys&, const KTables<float>&, const double*, int64_t, double*,
-                                               double*, double*, double*, double*, double*, double*, double*, double,
+                                               double*, double*, double*, double*, double*, double*, double*,
                                                const float*, hipStream_t);
 
 }  // namespace tdvmc

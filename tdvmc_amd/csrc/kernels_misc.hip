@@ -100,9 +100,10 @@ __global__ void unpack_batch_kernel(const BatchParams<T>* __restrict__ bp, const
 {
     const BatchParams<T> b = bp[blockIdx.x];
     const double* xs = x + b.xoff;
-    for (int i = threadIdx.x; i < b.sy.P; i += blockDim.x) {
-        b.uR[i] = xs[i];
-        b.uI[i] = xs[b.sy.P + i];
+    for (int i = threadIdx.x; i <= b.sy.P; i += blockDim.x) {
+        // slot P: phi R, phi I (packed at 2P, 2P + 1)
+        b.uR[i] = i < b.sy.P ? xs[i] : xs[2 * b.sy.P];
+        b.uI[i] = i < b.sy.P ? xs[b.sy.P + i] : xs[2 * b.sy.P + 1];
     }
 }
 
@@ -236,23 +237,28 @@ __global__ void pc_combine_kernel(int n, const double* x, const double* k1, cons
 
 __global__ void unpack_params_kernel(int P, const double* packed, double* uR, double* uI)
 {
-    for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < P; i += blockDim.x * gridDim.x) {
-        uR[i] = packed[i];
-        uI[i] = packed[P + i];
+    for (int i = threadIdx.x + blockIdx.x * blockDim.x; i <= P; i += blockDim.x * gridDim.x) {
+        // slot P: phi R, phi I (packed at 2P, 2P + 1)
+        uR[i] = i < P ? packed[i] : packed[2 * P];
+        uI[i] = i < P ? packed[P + i] : packed[2 * P + 1];
     }
 }
 
-// AdjustParameters (src/TDVMC.cpp:2531-2567: u_i -= u_{P-1}, both parts) and
-// NormalizeWavefunction (:2525-2529 with the exponent of :3763: phi R = -exponent)
-__global__ void post_step_kernel(int P, double* x, const double* expo, int normalize, int adjust)
+// After a step (src/TDVMC.cpp:3757-3775): phi R := -exponent (NormalizeWavefunction,
+// :2525-2529) and the gauge shift of AdjustParameters (:2531-2567), which
+// subtracts the last pair-spline parameter from the pair-spline block
+// [off, off + n) only.  With the partition of unity of the B-spline basis (and
+// the outer-pair term of that same last parameter) this adds a constant to
+// ln psi; the one-body and k-space parameters are left as they are.
+__global__ void post_step_kernel(int P, double* x, const double* expo, int normalize, int adjust, int off, int n)
 {
-    const double lastR = x[P - 1], lastI = x[2 * P - 1];
+    const double lastR = x[off + n - 1], lastI = x[P + off + n - 1];
     const double e = normalize ? expo[0] : 0.0;
     __syncthreads();
     if (adjust) {
-        for (int i = threadIdx.x; i < P; i += blockDim.x) {
-            x[i] -= lastR;
-            x[P + i] -= lastI;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            x[off + i] -= lastR;
+            x[P + off + i] -= lastI;
         }
     }
     if (normalize && threadIdx.x == 0) x[2 * P] = -e;
@@ -357,7 +363,7 @@ hipError_t launch_pc_combine(int P, const double* x, const double* k1, const dou
 
 hipError_t launch_unpack_params(int P, const double* packed, double* uR, double* uI, hipStream_t st)
 {
-    hipLaunchKernelGGL(unpack_params_kernel, dim3((P + 255) / 256), dim3(256), 0, st, P, packed, uR, uI);
+    hipLaunchKernelGGL(unpack_params_kernel, dim3((P + 256) / 256), dim3(256), 0, st, P, packed, uR, uI);
     return hipGetLastError();
 }
 
@@ -411,9 +417,10 @@ hipError_t launch_moments_batch(const MomentsItem* items_dev, int n, int nchunk_
     return hipGetLastError();
 }
 
-hipError_t launch_post_step(int P, double* x, const double* expo, int normalize, int adjust, hipStream_t st)
+hipError_t launch_post_step(int P, double* x, const double* expo, int normalize, int adjust, int off, int n,
+                            hipStream_t st)
 {
-    hipLaunchKernelGGL(post_step_kernel, dim3(1), dim3(256), 0, st, P, x, expo, normalize, adjust);
+    hipLaunchKernelGGL(post_step_kernel, dim3(1), dim3(256), 0, st, P, x, expo, normalize, adjust, off, n);
     return hipGetLastError();
 }
 

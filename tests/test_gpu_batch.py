@@ -236,3 +236,89 @@ def test_batch_pc_and_reuse_steps_vs_oracle(ode):
             gR, gI, pR, pI = c.get_params()
             got = np.concatenate([gR, gI, [pR, pI]])
             assert np.max(np.abs(got - x1)) < 1e-9 * np.max(np.abs(x1 - x0)) + 1e-13, f"system {i}"
+
+
+def test_batch_quench_crosses_switch_time_like_single_contexts():
+    """Gaussian pair potentials quenched at t_q = SYSTEM_PARAMS[2]
+    (src/PhysicalSystems/BosonsBulk.cpp:236-243): the batched Euler steps
+    before and after t_q use the potential in force, as single contexts do.
+    The batch's device descriptors copy each system's potential, so they must
+    follow the switch; the captured step replays with the new values."""
+    sp = (0.5, 2.0, 0.015, 0.5, 3.5)
+    cfgs = _cfgs(therm=0, ns=(3, 10, 20), IMAGINARY_TIME=0, REG_EPS=1e-3, W=256)
+    for c in cfgs:
+        c["SYSTEM_PARAMS"] = list(sp)
+    dt = 0.01
+    print("batch quench", flush=True)
+    for graph in (False, True):
+        with tdvmc.Batch(cfgs) as b:
+            start = []
+            for i, c in enumerate(b.systems):
+                s, uR, uI = _params(cfgs[i], i)
+                c.set_params(uR, uI)
+                start.append((uR, uI, c.get_walkers()))
+            energies = []
+            for _ in range(3):            # t = 0, 0.01 before t_q; t = 0.02 after
+                m = b.measure(precision=tdvmc.FP64)
+                energies.append([r["mE"][0] for r in m])
+                b.tdvp_step(dt, ode_type=0, precision=tdvmc.FP64, use_graph=graph)
+            got = [np.concatenate(c.get_params()[:2]) for c in b.systems]
+        for i, cfg in enumerate(cfgs):
+            uR, uI, R = start[i]
+            with tdvmc.Context(cfg) as ctx:
+                ctx.set_params(uR, uI)
+                ctx.set_walkers(R)
+                for k in range(3):
+                    e = ctx.measure(precision=tdvmc.FP64)["localEnergyR"]
+                    assert abs(e - energies[k][i]) < 1e-10 * max(1.0, abs(e)), (graph, i, k)
+                    ctx.tdvp_step(dt, ode_type=0, precision=tdvmc.FP64)
+                ref = np.concatenate(ctx.get_params()[:2])
+            assert np.allclose(got[i], ref, rtol=1e-10, atol=1e-13), (graph, i)
+    # the switch changes the energy: the third measurement sees the new potential
+    assert abs(energies[2][0] - energies[1][0]) > 1e-3
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batch_member_walker_count_change(prec):
+    """A member context given a new walker count (new device buffers) is swept,
+    measured and stepped by the batch with its new walkers: batched sweep equal
+    to a single context, batched measurement equal to the oracle, a captured
+    step rebuilt for the new launch shapes."""
+    cfgs = _cfgs(W=200)
+    seed = 123
+    print("batch walker count", prec, flush=True)
+    with tdvmc.Batch(cfgs) as b:
+        params = []
+        for i, c in enumerate(b.systems):
+            s, uR, uI = _params(cfgs[i], i)
+            c.set_params(uR, uI)
+            params.append((s, uR, uI))
+        b.set_sweep_lanes(32)
+        b.sweep(10, seed=seed, precision=prec)          # descriptors built for W = 200
+        b.tdvp_step(1e-4, ode_type=0, precision=prec, use_graph=True)
+        sysd = b.systems[1]
+        rng = np.random.default_rng(5)
+        R_new = rng.normal(0.0, 1.0, (640, cfgs[1]["N"], 2))
+        sysd.set_walkers(R_new)                         # W 200 -> 640: buffers reallocated
+        for i, c in enumerate(b.systems):
+            c.set_params(params[i][1], params[i][2])
+        starts = [c.get_walkers() for c in b.systems]
+        b.sweep(30, seed=seed + 1, precision=prec)
+        got = [c.get_walkers() for c in b.systems]
+        res = b.measure(precision=tdvmc.FP64)
+        b.tdvp_step(1e-4, ode_type=0, precision=prec, use_graph=True)
+        after = [np.concatenate(c.get_params()[:2]) for c in b.systems]
+    assert got[1].shape == (640, cfgs[1]["N"], 2)
+    for i, cfg in enumerate(cfgs):
+        s, uR, uI = params[i]
+        with tdvmc.Context(cfg) as ctx:
+            ctx.set_params(uR, uI)
+            ctx.set_walkers(starts[i])
+            ctx.set_sweep_lanes(32)
+            ctx.sweep(30, seed=tdvmc.batch_system_seed(seed + 1, i), precision=prec)
+            assert np.array_equal(ctx.get_walkers(), got[i]), f"system {i}"
+        out = o.eval_configs(s, got[i], uR, uI)
+        m = o.moments(out["O"], out["ER"], out["EI"])
+        assert rel(res[i]["mO"], m["mO"]) < 1e-10, f"system {i}"
+        assert rel(res[i]["S"], m["S"]) < 1e-10, f"system {i}"
+        assert np.all(np.isfinite(after[i]))

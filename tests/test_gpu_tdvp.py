@@ -145,8 +145,11 @@ def test_profiled_graph_replays():
 @pytest.mark.parametrize("solver", [0, 1])
 def test_euler_step_normalize_adjust_vs_oracle(solver):
     """USE_NORMALIZE_WF (phi R := -exponent of walker 0 at the last sample)
-    and USE_ADJUST_PARAMETERS (alpha -= alpha[P-1]) after an Euler step, with
-    either linear solver (src/TDVMC.cpp:3757-3775)."""
+    and USE_ADJUST_PARAMETERS after an Euler step, with either linear solver
+    (src/TDVMC.cpp:3757-3775).  AdjustParameters (:2531-2567) subtracts the
+    last pair-spline parameter from the N_PARAM pair-spline parameters only;
+    the k-space amplitudes after them must be left as the step made them (a
+    shift there would change the state, not only its normalisation)."""
     imag = 1
     cfg, s, R, uR, uI = _setup(imag)
     cfg.update(USE_NORMALIZE_WF=1, USE_ADJUST_PARAMETERS=1, LINEAR_EQUATION_SOLVER_TYPE=solver)
@@ -157,7 +160,10 @@ def test_euler_step_normalize_adjust_vs_oracle(solver):
     r = o.solve_param_dot(mo, imag_time=imag, precondition=0, eps=1e-3, solver_type=solver)
     xR = uR + dt * r["xR"]
     xI = uI + dt * r["xI"]
-    xR, xI = xR - xR[-1], xI - xI[-1]
+    Ps = cfg["N_PARAM"]
+    assert P > Ps  # the k-space block follows the pair block
+    xR[:Ps] -= xR[Ps - 1]
+    xI[:Ps] -= xI[Ps - 1]
     phiR = -out["expo"][0]
     phiI = -0.2 + dt * r["phiDotI"]
     print("normalize/adjust", solver, flush=True)
@@ -170,7 +176,8 @@ def test_euler_step_normalize_adjust_vs_oracle(solver):
     scale = dt * max(np.max(np.abs(r["xR"])), np.max(np.abs(r["xI"])))
     assert np.max(np.abs(gR - xR)) < 1e-9 * scale + 1e-13
     assert np.max(np.abs(gI - xI)) < 1e-9 * scale + 1e-13
-    assert gR[-1] == 0.0 and gI[-1] == 0.0
+    assert gR[Ps - 1] == 0.0 and gI[Ps - 1] == 0.0
+    assert np.all(gR[Ps:] != 0.0)
     assert abs(pR - phiR) < 1e-10 * max(1.0, abs(phiR))
     assert abs(pI - phiI) < 1e-10
 
@@ -244,3 +251,27 @@ def test_unsupported_ode_type_refused():
         with pytest.raises(tdvmc.TdvmcError) as ei:
             ctx.tdvp_step(0.01, ode_type=4, precision=tdvmc.FP64)
     assert ei.value.code == tdvmc.ERR_BAD_ARG
+
+
+@pytest.mark.parametrize("ode", [0, 1, 2, 3])
+def test_trial_counts_graph_and_eager(ode):
+    """tdvmc_get_acceptance counts every trial move once: steps replayed from a
+    captured graph count what eager steps count, and both equal the fresh
+    sampling stages of the ODE type (Euler 1, predictor-corrector 2, RK4 4,
+    RK4 reusing samples 1) x (MC_NINITIALIZATIONSTEPS + MC_NTHERMSTEPS) x W."""
+    N, D, L, Ps, W = 8, 3, 3.0, 6, 128
+    therm, init = 16, 8
+    cfg = make_cfg(N=N, D=D, L=L, P_s=Ps, pot="GAUSSIAN", sysparams=(0.5, 1.0), hbar2_2m=0.5, W=W,
+                   therm=therm, init=init, IMAGINARY_TIME=1, REG_EPS=1e-3)
+    stages = {0: 1, 1: 2, 2: 4, 3: 1}[ode]
+    counts = []
+    for graph in (False, True):
+        print("trial counts", ode, graph, flush=True)
+        with tdvmc.Context(cfg) as ctx:
+            ctx.set_params(np.full(Ps, 0.01))
+            for _ in range(3):
+                ctx.tdvp_step(1e-4, ode_type=ode, precision=tdvmc.FP64, use_graph=graph)
+            acc, trials = ctx.acceptance()
+            counts.append(trials)
+            assert 0 < acc <= trials
+    assert counts[0] == counts[1] == 3 * stages * (init + therm) * W
