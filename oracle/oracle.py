@@ -121,6 +121,10 @@ def lib():
         L.tdo_solve_param_dot_qr.restype = C.c_int
         L.tdo_solve_param_dot_qr.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp,
                                              _dp, _dp, _dp, _dp]
+        L.tdo_build_rhs.restype = None
+        L.tdo_build_rhs.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.tdo_phi_dot.restype = None
+        L.tdo_phi_dot.argtypes = [C.c_int, C.c_int, _dp, C.c_double, _dp, _dp, _dp, _dp]
         L.tdo_fpqr_solve.restype = C.c_int
         L.tdo_fpqr_solve.argtypes = [C.c_int, _dp, _dp, _dp]
         _ip = C.POINTER(C.c_int)
@@ -259,6 +263,55 @@ def solve_param_dot(m, imag_time=1, precondition=0, eps=1e-3, solver_type=0):
     else:
         bad = lib().tdo_solve_param_dot(P, int(imag_time), int(precondition), float(eps), *args)
     return dict(xR=xR, xI=xI, phiDotR=pR.value, phiDotI=pI.value, bad=bad)
+
+
+def build_rhs(m, imag_time):
+    """Right-hand sides (bR, bI) of src/TDVMC.cpp:1506-1537 from centred moments."""
+    P = m["mO"].size
+    bR, bI = np.zeros(P), np.zeros(P)
+    mO, mE, FR, FI = _f64(m["mO"]), _f64(m["mE"]), _f64(m["FR"]), _f64(m["FI"])
+    lib().tdo_build_rhs(P, int(imag_time), _ptr(mO), _ptr(mE), _ptr(FR), _ptr(FI), _ptr(bR), _ptr(bI))
+    return bR, bI
+
+
+def phi_dot(m, imag_time, xR, xI):
+    """phi-dot (R, I) of src/TDVMC.cpp:1658-1682 on given alpha-dot."""
+    P = m["mO"].size
+    pR, pI = C.c_double(0), C.c_double(0)
+    mO, xR, xI = _f64(m["mO"]), _f64(xR), _f64(xI)
+    lib().tdo_phi_dot(P, int(imag_time), _ptr(mO), float(m["mE"][0]), _ptr(xR), _ptr(xI), C.byref(pR),
+                      C.byref(pI))
+    return pR.value, pI.value
+
+
+def ode_step(f, x0, ode, dt):
+    """One step of ODE type `ode` for x' = f(x) on the packed parameter vector
+    (the stage structure of the reference's integrators):
+      0  CalculateNextParametersEuler (src/TDVMC.cpp:1834-1853): x0 + dt k1;
+      1  CalculateNextParametersPC (:1855-1910): predictor x0 + dt k1, one
+         corrector x0 + dt/2 (k1 + f(xp));
+      5  CalculateNextParametersPCReuseSamples (:1912-1967): six correctors;
+      2, 3  CalculateNextParametersRK4 (:1969-2035) / ...ReuseSamples
+         (:2037-2103): classic RK4, x0 + dt/6 (k1 + 2 k2 + 2 k3 + k4) with
+         stages at x0 + dt/2 k1, x0 + dt/2 k2, x0 + dt k3.
+    Which samples each stage evaluates is the caller's f; the arithmetic of
+    the step is this function's."""
+    x0 = np.asarray(x0, dtype=np.float64)
+    if ode == 0:
+        return x0 + dt * f(x0)
+    if ode in (2, 3):
+        k1 = f(x0)
+        k2 = f(x0 + dt / 2 * k1)
+        k3 = f(x0 + dt / 2 * k2)
+        k4 = f(x0 + dt * k3)
+        return x0 + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+    if ode in (1, 5):
+        k1 = f(x0)
+        xp = x0 + dt * k1
+        for _ in range(6 if ode == 5 else 1):
+            xp = x0 + (k1 + f(xp)) * (dt / 2)
+        return xp
+    raise ValueError(f"ODE type {ode}")
 
 
 def fpqr_solve(A, b):

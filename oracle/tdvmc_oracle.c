@@ -20,7 +20,18 @@
  *   tdo_moments                        -> numpy covariance on the same samples
  *   tdo_cholesky_ref / tdo_chol_solve  -> numpy.linalg.cholesky / solve, and the
  *                                         reference's non-positive-pivot behaviour
- *   tdo_build_system / tdo_phi_dot     -> hand-derived small cases (tests)
+ *   tdo_build_rhs / tdo_phi_dot /      -> golden vectors of the reference driver's own
+ *   tdo_solve_param_dot(_qr) /            BuildSystemOfEquationsForParameters,
+ *   tdo_fpqr_solve                        SolveForParametersDot (Cholesky and Eigen
+ *                                         FullPivHouseholderQR, with and without scaling)
+ *                                         and CalculatePhiDot (oracle/_ref/libref_driver.so,
+ *                                         tests/golden/tdvp_cases.npz); the signs of the
+ *                                         right-hand sides also by the energy identities of
+ *                                         a two-body system integrated by quadrature
+ *                                         (tests/test_oracle_math.py)
+ *   tdo_pair_potential                 -> square well / quench: reference eval goldens;
+ *                                         Lennard-Jones: LJ::GetPotential values
+ *                                         (tests/golden/driver_cases.json)
  *   tdo_metropolis                     -> exact two-particle distribution exp(2u(r))
  *   tdo_gr_grid / tdo_kshells /        -> golden vectors of the compiled reference's
  *   tdo_observables                       CalculateAdditionalSystemProperties, the
@@ -159,7 +170,7 @@ void tdo_spline_ops(int P_s, const double* B, double* O)
 /* square well: src/PhysicalSystems/BosonsBulk.cpp:236-271 (b if r < a);
  * Gaussian: src/PhysicalSystems/BosonsBulkDamped.cpp:237-264 (b exp(-(r/a)^2/2));
  * both switch to (a', b') once time >= t_quench.
- * Lennard-Jones: src/Potentials/LJ.cpp:46-52 with (sigma, eps). */
+ * Lennard-Jones: src/Potentials/LJ.cpp:16-22 with (sigma, eps): 4 eps ((sigma/r)^12 - (sigma/r)^6). */
 double tdo_pair_potential(const tdo_system* s, double r, double time)
 {
     double a = s->pot[0], b = s->pot[1];

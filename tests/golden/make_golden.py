@@ -23,7 +23,15 @@ Outputs (inputs and expected outputs only; no reference source):
   config_items.json   -- names and types of the reference's config items in
                          registration order (src/TDVMC.cpp:297-349), as data
 
+  tdvp_cases.npz      -- moment sets and the reference driver's right-hand
+                         sides, solutions of both linear solvers (with and
+                         without scaling) and phi-dot, real and imaginary time
+  driver_cases.json   -- AdjustParameters and LJ::GetPotential values
+  sample_bulk3d_sw16.json -- <E_R> +- error of 256 reference Metropolis chains
+
     python tests/golden/make_golden.py obs    # only the observables fixtures
+    python tests/golden/make_golden.py drv    # only the driver fixtures (needs `make -C oracle drv`)
+    python tests/golden/make_golden.py sample # only the reference sampling fixture
 """
 import ctypes as C
 import json
@@ -228,9 +236,149 @@ def configs():
             shutil.copyfile(src, os.path.join(HERE, "config_" + name.replace(".config", ".json")))
 
 
+def drv():
+    """The reference driver's own TDVP pieces (oracle/_ref/libref_driver.so,
+    built by `make -C oracle drv` from the whole reference source tree)."""
+    path = os.path.join(ROOT, "oracle", "_ref", "libref_driver.so")
+    if not os.path.exists(path):
+        sys.exit("oracle/_ref/libref_driver.so missing: run `make -C oracle drv`")
+    return C.CDLL(path)
+
+
+def _moments_raw(O, ER, EI):
+    """The driver's accumulated means (src/TDVMC.cpp:1103-1109): <O>, (<E_R>,
+    <E_I>), <O O^T>, <O E_R>, <O E_I>."""
+    W = O.shape[0]
+    return (O.mean(0), np.array([ER.mean(), EI.mean()]), (O.T @ O) / W, (O * ER[:, None]).mean(0),
+            (O * EI[:, None]).mean(0))
+
+
+def _tdvp_inputs(name):
+    if name == "p12":
+        # correlated synthetic samples: generic, well-conditioned S
+        rng = np.random.default_rng(31)
+        W, P = 600, 12
+        A = rng.normal(0, 1, (P, P)) / np.sqrt(P) + np.eye(P)
+        O = rng.normal(0, 1, (W, P)) @ A + rng.normal(0, 0.3, P)
+        ER = 2.0 + O @ rng.normal(0, 0.3, P) + rng.normal(0, 0.5, W)
+        EI = O @ rng.normal(0, 0.1, P) + rng.normal(0, 0.2, W)
+        return O, ER, EI
+    # p296_twins: O, E_L of a 3-D bulk system with a 40-parameter pair spline
+    # and 256 k-space terms, evaluated by the oracle on random configurations;
+    # O_k = O_-k for every +-k pair, so S is exactly singular
+    N, D, L, Ps, K, W = 27, 3, 3.0, 40, 256, 400
+    s = o.System(N=N, D=D, L=L, P_s=Ps, kvec=o.kvectors(D, K, L), pot_type=o.POT_GAUSS, pot=(0.5, 2.0),
+                 hbar2_2m=0.5)
+    rng = np.random.default_rng(32)
+    R = rng.uniform(-L / 2, L / 2, (W, N, D))
+    uR = rng.normal(0, 0.2, s.P)
+    uR[Ps:] *= 0.01
+    uI = rng.normal(0, 0.05, s.P)
+    uI[Ps:] *= 0.01
+    out = o.eval_configs(s, R, uR, uI)
+    return out["O"], out["ER"], out["EI"]
+
+
+def tdvp_cases():
+    """Right-hand sides (BuildSystemOfEquationsForParameters), both linear
+    solvers with and without the diagonal scaling (SolveForParametersDot,
+    CalculatePhiDot), in real and imaginary time, on two moment sets; plus
+    CalculatePhiDot alone and AdjustParameters."""
+    L = drv()
+    out = {}
+    for name in ("p12", "p296_twins"):
+        O, ER, EI = _tdvp_inputs(name)
+        mO, mE, OO, OER, OEI = (np.ascontiguousarray(a) for a in _moments_raw(O, ER, EI))
+        P = mO.size
+        rec = dict(mO=mO, mE=mE, OO=OO, OER=OER, OEI=OEI)
+        for imag in (0, 1):
+            S, bR, bI = np.zeros((P, P)), np.zeros(P), np.zeros(P)
+            L.refd_build_rhs(P, imag, f(mO), f(mE), f(OO), f(OER), f(OEI), f(S), f(bR), f(bI))
+            rec[f"bR_i{imag}"], rec[f"bI_i{imag}"] = bR, bI
+            if P <= 64:
+                rec[f"S_i{imag}"] = S
+            for solver in (0, 1):
+                for pre in (0, 1):
+                    xR, xI, pd = np.zeros(P), np.zeros(P), np.zeros(2)
+                    bad = L.refd_solve(P, imag, solver, pre, f(mO), f(mE), f(OO), f(OER), f(OEI), f(xR), f(xI),
+                                       f(pd))
+                    key = f"i{imag}_s{solver}_p{pre}"
+                    rec["xR_" + key], rec["xI_" + key], rec["phidot_" + key] = xR, xI, pd
+                    rec["bad_" + key] = np.array(bad)
+            # phi-dot alone on a given alpha-dot
+            rng = np.random.default_rng(40 + imag)
+            xR, xI, pd = rng.normal(0, 1, P), rng.normal(0, 1, P), np.zeros(2)
+            L.refd_phi_dot(P, imag, f(mO), f(mE), f(xR), f(xI), f(pd))
+            rec[f"phi_xR_i{imag}"], rec[f"phi_xI_i{imag}"], rec[f"phi_out_i{imag}"] = xR, xI, pd
+        out[name] = rec
+    np.savez_compressed(os.path.join(HERE, "tdvp_cases.npz"),
+                        **{f"{n}__{k}": v for n, rec in out.items() for k, v in rec.items()})
+    # AdjustParameters over the first N_PARAM entries of a longer vector
+    rng = np.random.default_rng(43)
+    n_param, P = 10, 16
+    uR, uI = rng.normal(0, 1, P), rng.normal(0, 1, P)
+    aR, aI = uR.copy(), uI.copy()
+    L.refd_adjust(n_param, f(aR), f(aI), P)
+    # Lennard-Jones (LJ::GetPotential) with unit and He-He constants
+    r_unit = np.linspace(0.85, 3.0, 40)
+    r_he = np.linspace(2.2, 12.5, 40)
+    v_unit, v_he = np.zeros(40), np.zeros(40)
+    L.refd_lj(0, f(r_unit), 40, f(v_unit))
+    L.refd_lj(1, f(r_he), 40, f(v_he))
+    with open(os.path.join(HERE, "driver_cases.json"), "w") as fh:
+        json.dump(dict(source="src/TDVMC.cpp:2531-2567 (AdjustParameters), src/Potentials/LJ.cpp:16-22 with "
+                              "LJ_He_He.cpp:13-14 (GetPotential)",
+                       adjust=dict(n_param=n_param, uR=uR.tolist(), uI=uI.tolist(), outR=aR.tolist(),
+                                   outI=aI.tolist()),
+                       lj=dict(r_unit=r_unit.tolist(), v_unit=v_unit.tolist(), r_he=r_he.tolist(),
+                               v_he=v_he.tolist(), sigma_he=2.556, eps_he=10.22)), fh)
+
+
+# Reference Metropolis chains of a 3-D square-well system (BosonsBulk with its
+# own incremental move ratio, src/PhysicalSystems/BosonsBulk.cpp:553-665, and
+# the move loop of DoMetropolisStep, src/TDVMC.cpp:858-916): independent
+# chains from jittered lattices, each thermalised, then n_samples sweeps of N
+# moves followed by one measurement of E_R.
+SAMPLE_CASE = dict(D=3, N=16, L=3.2, P=8, sysparams=[0.6, 2.0], step=0.45, chains=256, burn=400, samples=3000,
+                   seed=51)
+
+
+def sample_case():
+    r = o.ref()
+    c = SAMPLE_CASE
+    D, N, L, P = c["D"], c["N"], c["L"], c["P"]
+    rng = np.random.default_rng(c["seed"])
+    uR = np.array([-0.9, -0.6, -0.35, -0.2, -0.1, -0.05, -0.02, 0.0]) + rng.normal(0, 0.02, P)
+    spa = np.array(c["sysparams"])
+    n = 2
+    base = (np.stack(np.meshgrid(*[np.arange(n)] * D, indexing="ij"), -1).reshape(-1, D) + 0.5) * (L / n)
+    means = []
+    for ch in range(c["chains"]):
+        R = np.ascontiguousarray(base.repeat(N // len(base), 0) + rng.uniform(-0.3, 0.3, (N, D)) - L / 2)
+        e = C.c_double()
+        r.ref_bosonsbulk_sample(N, D, L, P, f(spa), 2, f(R), f(uR), c["step"], 1000 + ch, c["burn"], N,
+                                C.byref(e))
+        r.ref_bosonsbulk_sample(N, D, L, P, f(spa), 2, f(R), f(uR), c["step"], 5000 + ch, c["samples"], N,
+                                C.byref(e))
+        means.append(e.value)
+    means = np.array(means)
+    res = dict(c, uR=uR.tolist(), chain_means=means.tolist(), mean=float(means.mean()),
+               sigma=float(means.std(ddof=1) / np.sqrt(len(means))),
+               source="ref_bosonsbulk_sample over the reference BosonsBulk plug-in (oracle/ref_harness.cpp)")
+    with open(os.path.join(HERE, "sample_bulk3d_sw16.json"), "w") as fh:
+        json.dump(res, fh, indent=1)
+    print("reference <E_R> =", res["mean"], "+-", res["sigma"])
+
+
 if __name__ == "__main__":
     if not o.ref_available():
         sys.exit("oracle/_ref/libref.so missing: run `make -C oracle ref` in a container with the reference")
+    if sys.argv[1:] == ["drv"]:
+        tdvp_cases()
+        sys.exit(0)
+    if sys.argv[1:] == ["sample"]:
+        sample_case()
+        sys.exit(0)
     if sys.argv[1:] != ["obs"]:
         nic()
         eval_cases()

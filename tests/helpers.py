@@ -42,3 +42,14 @@ def energy_scale(out):
     oth = out["other"]
     return np.abs(oth[:, 4]) + np.abs(oth[:, 5]) + np.abs(oth[:, 6]) + np.abs(oth[:, 7]) + np.abs(oth[:, 8]) + \
         np.abs(oth[:, 1]) + 1e-300
+
+
+def adjust_pair_block(uR, uI, off, n):
+    """AdjustParameters (src/TDVMC.cpp:2531-2567) on the pair-spline block
+    [off, off + n): every entry minus the block's last entry (taken before the
+    loop; the last one becomes 0), other entries unchanged."""
+    aR, aI = np.array(uR, dtype=np.float64), np.array(uI, dtype=np.float64)
+    lr, li = aR[off + n - 1], aI[off + n - 1]
+    aR[off:off + n] -= lr
+    aI[off:off + n] -= li
+    return aR, aI

@@ -189,3 +189,35 @@ def test_solve_type1_singular_pm_k(P, ndup):
     assert fro(r["xR"], ref["xR"]) < 1e-8
     assert fro(r["xI"], ref["xI"]) < 1e-8
     assert abs(r["phiDotR"] - ref["phiDotR"]) < 1e-8 * max(1, abs(ref["phiDotR"]))
+
+
+@pytest.mark.parametrize("solver,pre", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("imag", [0, 1])
+@pytest.mark.parametrize("name", ["p12", "p296_twins"])
+def test_solve_vs_reference_driver(golden_dir, name, imag, solver, pre):
+    """The GPU solve against the reference driver's own SolveForParametersDot
+    + CalculatePhiDot (compiled reference, tests/golden/tdvp_cases.npz), on
+    the driver's moment sums: a generic P = 12 set, and a P = 296 set of a
+    bulk system whose +-k twins make S exactly singular (solver type 1 then
+    runs at rank 168 with solutions up to 1.7e10 without the scaling)."""
+    import os
+
+    z = np.load(os.path.join(golden_dir, "tdvp_cases.npz"), allow_pickle=False)
+    g = {k.split("__")[1]: z[k] for k in z.files if k.startswith(name + "__")}
+    mO, mE = g["mO"], g["mE"]
+    S = g["OO"] - np.outer(mO, mO)
+    FR, FI = g["OER"] - mE[0] * mO, g["OEI"] - mE[1] * mO
+    key = f"i{imag}_s{solver}_p{pre}"
+    print("solve vs reference driver", name, key, flush=True)
+    with _ctx() as ctx:
+        r = ctx.solve(mO, mE, S, FR, FI, imag_time=imag, precondition=pre, eps=1e-3, solver_type=solver,
+                      check=False)
+    xR, xI, pd = g["xR_" + key], g["xI_" + key], g["phidot_" + key]
+    scale = max(np.max(np.abs(xR)), np.max(np.abs(xI)))
+    tol = 1e-7 if (name == "p296_twins" and solver == 1 and pre == 0) else 1e-10
+    assert np.max(np.abs(r["xR"] - xR)) <= tol * scale
+    assert np.max(np.abs(r["xI"] - xI)) <= tol * scale
+    sR = np.abs(mO) @ np.abs(xR) + abs(mE[0])
+    sI = np.abs(mO) @ np.abs(xI) + abs(mE[0])
+    assert abs(r["phiDotR"] - pd[0]) <= tol * sR
+    assert abs(r["phiDotI"] - pd[1]) <= tol * sI

@@ -17,6 +17,8 @@ a sign, an index shift, a transposed operand) fails it.
   the reference's non-positive-pivot behaviour;
 * Metropolis: two particles on a ring sample |psi|^2 = exp(2 u(r)).
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -354,3 +356,93 @@ def test_full_pivot_qr_duplicate_columns():
     assert rank == n - 1
     assert np.allclose(S @ x, b, rtol=1e-9, atol=1e-9)
     assert x[n - 1] == 0.0 or x[n - 2] == 0.0
+
+
+# ---------------------------------------------------------------- right-hand-side signs
+def _two_body_quadrature(s, uR, uI, M=20000):
+    """Exact expectations of the 1-D two-particle system on a ring: psi
+    depends on the minimum-image distance only, so <A> = int |psi|^2 A dr /
+    int |psi|^2 dr over r in (-L/2, L/2) (the centre of mass integrates out).
+    Midpoint rule on a periodic, C^2 integrand."""
+    L = s.L
+    r = (np.arange(M) + 0.5) * (L / M) - L / 2
+    R = np.zeros((M, 2, 1))
+    R[:, 1, 0] = r
+    out = o.eval_configs(s, R, uR, uI)
+    w = np.exp(2.0 * (out["expo"] - out["expo"].max()))
+    w /= w.sum()
+    return out, w
+
+
+def _energy_and_moments(s, uR, uI):
+    out, w = _two_body_quadrature(s, uR, uI)
+    O, ER, EI = out["O"], out["ER"], out["EI"]
+    mO, mER, mEI = w @ O, w @ ER, w @ EI
+    Oc = O - mO
+    m = dict(mO=mO, mE=np.array([mER, mEI]), S=(Oc * w[:, None]).T @ Oc, FR=w @ (Oc * (ER - mER)[:, None]),
+             FI=w @ (Oc * (EI - mEI)[:, None]))
+    return mER, m, w @ (O * EI[:, None])
+
+
+@pytest.mark.parametrize("imag", [0, 1])
+def test_rhs_signs_by_energy_identities(imag):
+    """Pins the signs and centring of the right-hand sides (src/TDVMC.cpp:
+    1506-1537) by physics, independently of their formulas: for a hermitian
+    H, d<E>/d uR_k = 2 cov(O_k, E_R) and d<E>/d uI_k = 2 <O_k E_I> (exact
+    expectations; here a two-body system integrated by quadrature).  The
+    TDVP direction then must lower <E> in imaginary time, separately along its
+    real and its imaginary part, and keep <E> constant to first order in real
+    time.  A flipped sign or a dropped term in b_R or b_I breaks one of these."""
+    s = o.System(N=2, D=1, L=6.0, P_s=6, pot_type=o.POT_GAUSS, pot=(0.5, 1.0), hbar2_2m=0.5)
+    rng = np.random.default_rng(17)
+    uR = np.array([-0.8, -0.5, -0.25, -0.1, -0.03, 0.0]) + rng.normal(0, 0.05, 6)
+    uI = rng.normal(0, 0.15, 6)
+    E0, m, g = _energy_and_moments(s, uR, uI)
+    assert abs(m["mE"][1]) < 1e-7 * abs(E0)     # <E_I> = 0 for hermitian H (quadrature error ~1e-9)
+    r = o.solve_param_dot(m, imag_time=imag, precondition=0, eps=1e-3)
+    xR, xI = r["xR"], r["xI"]
+
+    def dE(dR, dI, eps=1e-5):
+        ep = _energy_and_moments(s, uR + eps * dR, uI + eps * dI)[0]
+        em = _energy_and_moments(s, uR - eps * dR, uI - eps * dI)[0]
+        return (ep - em) / (2 * eps)
+
+    gR, gI = dE(xR, 0 * xI), dE(0 * xR, xI)
+    # the gradient formula on both parts (checks the moments' meaning)
+    scale = np.abs(m["FR"]) @ np.abs(xR) + np.abs(g) @ np.abs(xI)
+    assert abs(gR - 2 * m["FR"] @ xR) < 1e-6 * scale
+    assert abs(gI - 2 * g @ xI) < 1e-6 * scale
+    if imag:
+        assert gR < -1e-3 * scale and gI < -1e-6 * scale       # descent on both parts
+    else:
+        assert abs(gR + gI) < 1e-6 * scale                    # energy conserved
+        assert abs(gR) > 1e-3 * scale                          # (each part alone is not zero)
+
+
+def test_ode_steps_on_a_linear_ode():
+    """oracle.ode_step on x' = A x, where each scheme has a closed form:
+    Euler (I + hA), RK4 the degree-4 Taylor polynomial of exp(hA), the
+    predictor-corrector with one corrector Heun's (I + hA + (hA)^2/2), and
+    with six correctors the sixth iterate of y = c + (hA/2) y,
+    c = (I + hA/2) x0, from y0 = (I + hA) x0."""
+    rng = np.random.default_rng(8)
+    n, h = 7, 0.13
+    A = rng.normal(0, 1, (n, n))
+    x0 = rng.normal(0, 1, n)
+    f = lambda x: A @ x  # noqa: E731
+    I = np.eye(n)
+    hA = h * A
+    taylor = lambda k: sum(np.linalg.matrix_power(hA, j) / math.factorial(j) for j in range(k + 1))  # noqa: E731
+    assert np.allclose(o.ode_step(f, x0, 0, h), (I + hA) @ x0, rtol=1e-14, atol=1e-14)
+    for ode in (2, 3):
+        assert np.allclose(o.ode_step(f, x0, ode, h), taylor(4) @ x0, rtol=1e-13, atol=1e-13)
+    assert np.allclose(o.ode_step(f, x0, 1, h), taylor(2) @ x0, rtol=1e-13, atol=1e-13)
+    Mh = hA / 2
+    c = (I + Mh) @ x0
+    y = (I + hA) @ x0
+    for _ in range(6):
+        y = c + Mh @ y
+    assert np.allclose(o.ode_step(f, x0, 5, h), y, rtol=1e-13, atol=1e-13)
+    # and that is the trapezoidal rule up to (hA/2)^6
+    trap = np.linalg.solve(I - Mh, (I + Mh) @ x0)
+    assert np.linalg.norm(y - trap) < 2 * np.linalg.norm(np.linalg.matrix_power(Mh, 6), 2) * np.linalg.norm(x0) * 10
