@@ -158,6 +158,51 @@ def eval_cases():
                             sD=sD, sD2=sD2)
 
 
+# Large cases (N = 343 in 3-D, N = 400 in 2-D): the reference's sD / sD2 are
+# contracted here with the effective spline coefficients c = T^T u (the BC
+# map of src/PhysicalSystems/BosonsBulk.cpp:158-177) into the per-particle
+# gradient and Laplacian of ln psi, which keeps the fixtures small.
+EVAL_LARGE = [
+    ("bulk3d_n343_p40", 3, 343, 9.7949, 40, [0.9, 1.5], 2, 17, 0.0),
+    ("bulk2d_n400_p48", 2, 400, 20.0, 48, [0.25, 3.0], 2, 16, 0.0),
+]
+
+
+def _spline_coeff(u, P):
+    c = np.zeros(P + 2)
+    c[1] += u[0]
+    c[0] += u[1]
+    c[2] += u[1]
+    for k in range(2, P - 2):
+        c[k + 1] += u[k]
+    c[P - 1] += u[P - 2]
+    c[P + 1] += u[P - 2]
+    c[P] += u[P - 1]
+    return c
+
+
+def eval_large():
+    r = o.ref()
+    for (name, D, N, L, P, sp, W, seed, time) in EVAL_LARGE:
+        rng = np.random.default_rng(seed)
+        R = np.ascontiguousarray(rng.uniform(-L / 2, L / 2, (W, N, D)))
+        uR = rng.normal(0, 0.3, P)
+        uI = rng.normal(0, 0.2, P)
+        phiR, phiI = 0.05, -0.02
+        S = P + 2
+        spa = np.array(sp, dtype=np.float64)
+        expo, O, ER, EI = np.zeros(W), np.zeros((W, P)), np.zeros(W), np.zeros(W)
+        oth, sD, sD2 = np.zeros((W, 9)), np.zeros((W, S, N, D)), np.zeros((W, S, N))
+        r.ref_bosonsbulk_eval(N, D, L, P, f(spa), len(sp), time, W, f(R), f(uR), f(uI), phiR, phiI,
+                              f(expo), f(O), f(ER), f(EI), f(oth), f(sD), f(sD2))
+        cR, cI = _spline_coeff(uR, P), _spline_coeff(uI, P)
+        grad = np.stack([np.einsum("s,wsna->wna", cR, sD), np.einsum("s,wsna->wna", cI, sD)], -1)
+        lap = np.stack([np.einsum("s,wsn->wn", cR, sD2), np.einsum("s,wsn->wn", cI, sD2)], -1)
+        np.savez_compressed(os.path.join(HERE, f"eval_{name}.npz"), D=D, N=N, L=L, P=P, sysparams=spa, time=time,
+                            R=R, uR=uR, uI=uI, phiR=phiR, phiI=phiI, expo=expo, O=O, ER=ER, EI=EI, other=oth,
+                            grad=grad, lap=lap)
+
+
 def move_cases():
     r = o.ref()
     rng = np.random.default_rng(21)
@@ -375,6 +420,9 @@ if __name__ == "__main__":
         sys.exit("oracle/_ref/libref.so missing: run `make -C oracle ref` in a container with the reference")
     if sys.argv[1:] == ["drv"]:
         tdvp_cases()
+        sys.exit(0)
+    if sys.argv[1:] == ["eval_large"]:
+        eval_large()
         sys.exit(0)
     if sys.argv[1:] == ["sample"]:
         sample_case()

@@ -169,3 +169,71 @@ def test_real_parameters_vs_oracle(name):
                                                                         max(np.max(np.abs(ref["other"][:, k])), 1e-300)), k
     assert rel(mix["O"], ref["O"]) < 2e-6
     assert np.max(np.abs(mix["ER"] - ref["ER"]) / sc) < 5e-5
+
+
+# The reference's own particle configurations (config/particleconfiguration_*.csv,
+# kept as data under tests/golden/refconf_*.csv: one line of N*D comma-separated
+# coordinates) fed to the benchmark workloads.  particleconfiguration_400 lies
+# in the 20 x 20 box of the 2-D N = 400 workload as it is; the 64- and
+# 343-particle files come from boxes of other sizes (their coordinates span
+# about 4 and 7 x 2.15443), so they are scaled to the workload's box.
+REFCONF = {
+    "bulk2d_gauss400_k512": ("refconf_particleconfiguration_400.csv", 2, None),
+    "bulk3d_lj64": ("refconf_particleconfiguration_64.csv", 3, 4.0),
+    "bulk3d_lj343_k256": ("refconf_particleconfiguration_343_.csv", 3, 7 * 2.15443),
+}
+
+
+def _refconf_data(name, golden_dir):
+    from tdvmc_amd import workloads
+
+    fname, D, file_box = REFCONF[name]
+    cfg = workloads.WORKLOADS[name](W=4)
+    s = oracle_system(cfg)
+    vals = np.array([float(x) for x in open(os.path.join(golden_dir, fname)).read().replace("\n", ",").split(",")
+                     if x.strip()])
+    R0 = vals.reshape(s.N, D)
+    if file_box:
+        R0 = R0 * (s.L / file_box)
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    jitter = 0.02 * s.L / round(s.N ** (1.0 / D))
+    R = np.stack([R0] + [R0 + rng.uniform(-jitter, jitter, R0.shape) for _ in range(3)])
+    uR = np.array(cfg["PARAMS_REAL"], dtype=np.float64) + rng.normal(0, 0.05, s.P)
+    uI = rng.normal(0, 0.05, s.P)
+    uR[s.P_s:] = 1e-3 * rng.normal(0, 1, s.P - s.P_s)
+    uI[s.P_s:] *= 0.01
+    return cfg, s, R, uR, uI
+
+
+@pytest.mark.parametrize("name", list(REFCONF))
+def test_workloads_on_reference_configurations(name, golden_dir):
+    """The 2-D N = 400 quench workload (Gaussian pair potential, V0 = 5 -> 7
+    at t = 0, 48 spline parameters, 512 k-space terms), the 3-D N = 64 and
+    N = 343 Lennard-Jones workloads, each on the reference's own particle
+    configuration (and three jittered copies): FP64 within 1e-10 of the
+    oracle, the mixed form within its stated bounds."""
+    cfg, s, R, uR, uI = _refconf_data(name, golden_dir)
+    t = 0.0                                   # at t_q = 0 the quenched V0 = 7 is in force
+    print("reference configuration", name, flush=True)
+    with _ctx(cfg) as ctx:
+        ctx.set_params(uR, uI, 0.1, 0.0)
+        ctx.set_time(t)
+        f64 = ctx.eval_configs(R, precision=tdvmc.FP64, per_particle=True)
+        mix = ctx.eval_configs(R, precision=tdvmc.MIXED, per_particle=True)
+    ref = o.eval_configs(s, R, uR, uI, phiR=0.1, time=t)
+    sc = energy_scale(ref)
+    assert rel(f64["expo"], ref["expo"]) < 1e-10
+    assert rel(f64["O"], ref["O"]) < 1e-10
+    assert np.max(np.abs(f64["ER"] - ref["ER"]) / sc) < 1e-10
+    assert np.max(np.abs(f64["EI"] - ref["EI"]) / sc) < 1e-10
+    assert rel(f64["grad"], ref["grad"]) < 1e-10
+    assert rel(f64["lap"], ref["lap"]) < 1e-10
+    e_O = rel(mix["O"], ref["O"])
+    e_x = np.max(np.abs(mix["expo"] - ref["expo"]) / (np.abs(ref["O"]) @ np.abs(uR) + 1e-300))
+    e_E = np.max(np.abs(mix["ER"] - ref["ER"]) / sc)
+    e_g = rel(mix["grad"], ref["grad"])
+    print(f"mixed errors {name}: O {e_O:.2e} lnpsi {e_x:.2e} E_L {e_E:.2e} grad {e_g:.2e}", flush=True)
+    assert e_O < 2e-6
+    assert e_x < 2e-6
+    assert e_E < 5e-5
+    assert e_g < 5e-5

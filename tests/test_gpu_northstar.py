@@ -1,0 +1,63 @@
+"""The precision condition of the mixed form on real samples of the N = 343
+3-D bulk workload (Lennard-Jones He-4 at density 0.365 sigma^-3, 40
+McMillan-initialised spline parameters + 256 k-space terms, P = 296).
+
+One mixed-form sampling stage (tdvmc_sample: 343 FP32 trial moves per walker
+with rho_k carried incrementally in FP32 registers and handed to the
+measurement, FP32 pair / spline / phase arithmetic, bf16x3 MFMA moments) on
+16384 walkers; then the same final walkers in the FP64 plain form (itself
+checked against the oracle on a subset) and the oracle's FP64 moments of
+those values.  S, F_R and F_I must agree within 1e-6 (relative Frobenius).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as o
+from tests.helpers import oracle_system, rel
+
+pytestmark = pytest.mark.gpu
+tdvmc = pytest.importorskip("tdvmc_amd")
+
+
+def fro(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("phase", ["real", "complex"])
+def test_c3_mixed_sample_moments_vs_fp64(phase):
+    """'real': the workload's real parameters (imaginary time from a real
+    start: E_I = 0, so F_I vanishes in both forms); 'complex': imaginary parts
+    1e-2 of the real ones, as a real-time run develops them."""
+    from tdvmc_amd import workloads
+
+    W = 16384
+    cfg = workloads.bulk3d_lj343_k256(W=W, precision="MIXED")
+    print("C3 mixed sampling stage", W, phase, flush=True)
+    with tdvmc.Context(cfg) as ctx:
+        if phase == "complex":
+            uR0, _, _, _ = ctx.get_params()
+            ctx.set_params(uR0, 1e-2 * uR0 * np.random.default_rng(1).uniform(0.5, 1.5, uR0.size))
+        ctx.sweep(343 * 8, seed=2, precision=tdvmc.MIXED)            # away from the start lattice
+        step = ctx.sample(seed=9, precision=tdvmc.MIXED)             # 343 moves + measurement + moments
+        R = ctx.get_walkers()
+        uR, uI, _, _ = ctx.get_params()
+        ev = ctx.eval_configs(R, precision=tdvmc.FP64)
+    # the FP64 plain form of these walkers against the oracle (subset)
+    s = oracle_system(cfg)
+    ref = o.eval_configs(s, R[:8], uR, uI)
+    assert rel(ev["O"][:8], ref["O"]) < 1e-10
+    assert rel(ev["ER"][:8], ref["ER"]) < 1e-10
+    m = o.moments(ev["O"], ev["ER"], ev["EI"])
+    eS, eR = fro(step["S"], m["S"]), fro(step["FR"], m["FR"])
+    if phase == "real":
+        assert np.all(m["FI"] == 0.0) and np.all(step["FI"] == 0.0)
+        eI = 0.0
+    else:
+        eI = fro(step["FI"], m["FI"])
+    eO = fro(step["localOperators"], m["mO"])
+    eE = abs(step["localEnergyR"] - m["mE"][0]) / abs(m["mE"][0])
+    print(f"mixed vs FP64 at N=343, K=256, W={W}: S {eS:.2e}  F_R {eR:.2e}  F_I {eI:.2e}  <O> {eO:.2e}  "
+          f"<E_R> {eE:.2e}", flush=True)
+    assert eS <= 1e-6
+    assert eR <= 1e-6
+    assert eI <= 1e-6

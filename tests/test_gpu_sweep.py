@@ -157,3 +157,35 @@ def test_autotune_keeps_walkers_and_stream():
             ctx.sweep(80, seed=3, precision=tdvmc.FP64)
             res.append(ctx.get_walkers())
     assert np.array_equal(res[0], res[1])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_sampled_energy_vs_reference_chains(golden_dir, precision):
+    """<E_R> sampled by the GPU sweep against 256 chains of the compiled
+    reference (BosonsBulk's own move ratio in the DoMetropolisStep loop,
+    tests/golden/sample_bulk3d_sw16.json): a 3-D square-well system of 16
+    bosons.  The two chains use different generators, so they agree within
+    combined error bars, not move for move."""
+    import json
+    import os
+
+    g = json.load(open(os.path.join(golden_dir, "sample_bulk3d_sw16.json")))
+    N, D, L = g["N"], g["D"], g["L"]
+    W, n_meas = 8192, 60
+    cfg = make_cfg(N=N, D=D, L=L, P_s=g["P"], pot="SQUARE_WELL", sysparams=tuple(g["sysparams"]), hbar2_2m=1.0,
+                   W=W, step=g["step"])
+    print("energy vs reference chains", precision, flush=True)
+    with tdvmc.Context(cfg) as ctx:
+        ctx.set_params(np.array(g["uR"]))
+        ctx.sweep(g["burn"] * N, seed=3, precision=precision)
+        E = np.zeros((n_meas, W))
+        for k in range(n_meas):
+            ctx.sweep(N, seed=100 + k, precision=precision)
+            E[k] = ctx.eval_configs(ctx.get_walkers(), precision=tdvmc.FP64)["ER"]
+    per_walker = E.mean(0)                     # walkers are independent chains
+    mean, sigma = per_walker.mean(), per_walker.std(ddof=1) / np.sqrt(W)
+    diff = abs(mean - g["mean"])
+    comb = np.hypot(sigma, g["sigma"])
+    print(f"GPU <E_R> {mean:.4f} +- {sigma:.4f}, reference {g['mean']:.4f} +- {g['sigma']:.4f}: "
+          f"{diff / comb:.2f} sigma", flush=True)
+    assert diff < 4.5 * comb

@@ -72,8 +72,18 @@ def test_eval_vs_reference(path):
         assert rel(out["other"][:, k], z["other"][:, k]) < tol, k
     # other[2] = wf = exp(exponent + phiR): relative error is the absolute
     # error of the exponent
-    assert np.max(np.abs(np.log(out["other"][:, 2]) - np.log(z["other"][:, 2]))) < tol * np.max(np.abs(z["expo"])) + 1e-12
+    # (at large N exp overflows to inf or underflows to 0 in both)
+    wo, wz = out["other"][:, 2], z["other"][:, 2]
+    fin = np.isfinite(wz) & (wz > 0)
+    assert np.array_equal(wo[~fin], wz[~fin])
+    if fin.any():
+        assert np.max(np.abs(np.log(wo[fin]) - np.log(wz[fin]))) < tol * np.max(np.abs(z["expo"])) + 1e-12
 
+    if "grad" in z.files:
+        # large cases: the fixture holds sD / sD2 already contracted with c = T^T u
+        assert rel(out["grad"], z["grad"]) < tol
+        assert rel(out["lap"], z["lap"]) < tol
+        return
     # gradient / Laplacian: contract the reference's sD / sD2 with c = T^T u
     P, S = s.P_s, s.P_s + 2
     cR, cI = np.zeros(S), np.zeros(S)
