@@ -73,6 +73,7 @@ typedef struct {
     double reg_eps;
     int precondition;
     int K_terms;           /* k-space terms the kernels evaluate: one per +-k pair, plus unpaired k */
+    int gr_bin_count;      /* GR_BIN_COUNT: g(r) bins requested */
 } tdvmc_info;
 
 /* Moments of one sampling stage; fields mirror SimulationStepData
@@ -152,6 +153,24 @@ int tdvmc_eval_configs_dev(tdvmc_ctx* ctx, const double* R_dev, int64_t W, int p
                            double* ER_dev, double* EI_dev, double* O_dev, double* grad_dev, double* lap_dev,
                            void* stream);
 
+/* Device-pointer variants of tdvmc_sample / tdvmc_measure and of the walker
+ * exchange, for zero-copy use from device frameworks (torch tensors):
+ *   tdvmc_sample_dev / tdvmc_measure_dev - the same stage as tdvmc_sample /
+ *       tdvmc_measure (UpdateExpectationValues, src/TDVMC.cpp:1038-1220),
+ *       leaving mean_dev [P+2] = (<O>, <E_R>, <E_I>), S_dev [P][P] and
+ *       F_dev [2][P] = (cov(O, E_R), cov(O, E_I)) in device memory (any may
+ *       be NULL);
+ *   tdvmc_set_walkers_dev / tdvmc_get_walkers_dev - R_dev [W][N][DIM] in
+ *       device memory, W equal to the context's walker count (no reallocation).
+ * The work runs on the context stream, ordered after what `stream` (a
+ * hipStream_t; NULL: none) holds at the call, and `stream` then waits for it;
+ * the calls return without synchronising the host. */
+int tdvmc_sample_dev(tdvmc_ctx* ctx, uint64_t seed, int precision, double* mean_dev, double* S_dev, double* F_dev,
+                     void* stream);
+int tdvmc_measure_dev(tdvmc_ctx* ctx, int precision, double* mean_dev, double* S_dev, double* F_dev, void* stream);
+int tdvmc_set_walkers_dev(tdvmc_ctx* ctx, const double* R_dev, int64_t W, void* stream);
+int tdvmc_get_walkers_dev(tdvmc_ctx* ctx, double* R_dev, int64_t W, void* stream);
+
 /* Pair distribution g(r) and structure factor S(k), the additional
  * observables of BosonsBulk (periodic boxes only):
  *   grid     - src/PhysicalSystems/BosonsBulk.cpp:142-151 with Grid::Init and
@@ -194,6 +213,37 @@ int tdvmc_files_append_step(const char* out_dir, double time, const tdvmc_step_d
 int tdvmc_files_write_grids(const char* out_dir, const double* gr_grid, int n_gr, const double* sk_grid, int n_sk);
 int tdvmc_files_append_observables(const char* out_dir, double time, const double* gr, int n_gr, const double* sk,
                                    int n_sk);
+/* Resume files of a successive run.
+ *   tdvmc_files_write_coordinates - the walkers' coordinates as the driver
+ *       leaves them for the next run (src/TDVMC.cpp:4007-4022): moved into
+ *       the first cell when L > 0 (AlignCoordinates, :2569-2582, with
+ *       GetCoordinateNIC), one file coords/particleconfiguration_<N>_<DIM>D_<w>.dat
+ *       per walker w (the reference writes one per MPI rank) holding one line
+ *       of N*DIM values in the .dat number format (WriteParticleInputFile,
+ *       :493-504), and AAFinish_particleconfiguration_<N>_<DIM>D.dat with
+ *       walker 0 (the root rank's copy); out_dir/coords is created;
+ *   tdvmc_files_read_coordinates - LoadLastPositionsFromFile (:640-670): the
+ *       last line of `path`, comma-separated (the config/particleconfiguration_*.csv
+ *       files), N*DIM coordinates into R [N][DIM];
+ *   tdvmc_files_write_rng_state / tdvmc_files_read_rng_state - the random
+ *       stream's state in random/state_philox_<rank>.dat.  The reference
+ *       saves its mt19937_64 and distribution states there
+ *       (WriteRandomGeneratorStatesToFile, src/Utils.cpp:64-78, :3956-3964);
+ *       here the whole stream is fixed by (seed, epoch) of the counter-based
+ *       Philox generator (tdvmc_get_rng_state / tdvmc_set_rng_state), so that
+ *       pair is what the file holds.
+ * Host only; no context needed. */
+int tdvmc_files_write_coordinates(const char* out_dir, const double* R, int64_t W, int N, int D, double L);
+int tdvmc_files_read_coordinates(const char* path, double* R, int N, int D);
+int tdvmc_files_write_rng_state(const char* out_dir, uint64_t seed, uint32_t epoch, int rank);
+int tdvmc_files_read_rng_state(const char* path, uint64_t* seed, uint32_t* epoch);
+
+/* State of the context's random stream: the seed its TDVP steps draw with
+ * (SEED) and the call epoch (advanced by every sweep launch).  Setting both
+ * continues a stream exactly where a saved one stopped. */
+int tdvmc_get_rng_state(tdvmc_ctx* ctx, uint64_t* seed, uint32_t* epoch);
+int tdvmc_set_rng_state(tdvmc_ctx* ctx, uint64_t seed, uint32_t epoch);
+
 /* The config for a successive run, as WriteConfig writes AAFinish_tVMC.config
  * (src/TDVMC.cpp:387-410, :4026-4034): the items of `config` (text or path)
  * with the final parameters in PARAMS_REAL / PARAMS_IMAGINARY [P] and

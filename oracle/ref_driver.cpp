@@ -4,8 +4,9 @@
 // main is renamed to reference_main at compile time, nothing else changes).
 // TEST INFRASTRUCTURE ONLY: used in this container by
 // tests/golden/make_golden.py to write golden vectors for the right-hand
-// sides, both linear solvers, phi-dot, the gauge shift and the Lennard-Jones
-// pair potential.  Never shipped to, or loaded on, a GPU machine.
+// sides, both linear solvers, phi-dot, the gauge shift, the Lennard-Jones
+// pair potential and the successive-run coordinate files.  Never shipped to,
+// or loaded on, a GPU machine.
 //
 // The functions read the driver's globals (src/TDVMC.cpp:84-160): the moment
 // sums localOperators, localEnergyR/I, localOperatorsMatrix,
@@ -41,7 +42,12 @@ extern vector<double> localOperatorlocalEnergyR, localOperatorlocalEnergyI;
 extern bool doNotAcceptStep;
 extern PhysicalSystems::IPhysicalSystem* sys;
 
+extern string OUT_DIR, configDirectory;
+
 // functions defined in src/TDVMC.cpp (no header declares them)
+void WriteParticleInputFile(string fileName, vector<vector<double> >& R);
+bool LoadLastPositionsFromFile(string filename, vector<vector<double> >& R);
+void MoveCoordinatesToFirstCell(vector<vector<double> >& R);
 void BuildSystemOfEquationsForParameters(vector<vector<double> >& matrix, vector<double>& energiesReal,
                                          vector<double>& energiesImag);
 void SolveForParametersDot(vector<double>& uDotR, vector<double>& uDotI, double* phiDotR, double* phiDotI);
@@ -177,6 +183,40 @@ int refd_adjust(int n_param, double* uR, double* uI, int P)
         uI[i] = vI[i];
     }
     return 0;
+}
+
+// The successive-run coordinate file as the driver writes it
+// (src/TDVMC.cpp:4007-4018): MoveCoordinatesToFirstCell (AlignCoordinates for
+// a periodic system, :2569-2582) when align != 0, then WriteParticleInputFile
+// (:493-504) as out_dir + name + ".dat".
+int refd_write_particle_file(const char* out_dir, const char* name, const double* R, int n, int dim, double L,
+                             int align)
+{
+    N = n;
+    DIM = dim;
+    LBOX = L;
+    LBOX_R = 1.0 / L;
+    LBOX_2 = L / 2.0;
+    OUT_DIR = out_dir;
+    vector<vector<double> > Rv(n, vector<double>(dim));
+    for (int i = 0; i < n; i++)
+        for (int a = 0; a < dim; a++) Rv[i][a] = R[i * dim + a];
+    if (align) MoveCoordinatesToFirstCell(Rv);
+    WriteParticleInputFile(name, Rv);
+    return 0;
+}
+
+// LoadLastPositionsFromFile (src/TDVMC.cpp:640-670): config_dir + name + ".csv".
+int refd_load_positions(const char* config_dir, const char* name, double* R, int n, int dim)
+{
+    N = n;
+    DIM = dim;
+    configDirectory = config_dir;
+    vector<vector<double> > Rv(n, vector<double>(dim, 0.0));
+    const bool ok = LoadLastPositionsFromFile(name, Rv);
+    for (int i = 0; i < n; i++)
+        for (int a = 0; a < dim; a++) R[i * dim + a] = Rv[i][a];
+    return ok ? 0 : 1;
 }
 
 // LJ::GetPotential (src/Potentials/LJ.cpp:16-22) with the He-He constants of

@@ -220,6 +220,16 @@ int ref_bosonsbulk_eval(int n, int dim, double L, int nparam, const double* sysp
     return 0;
 }
 
+// A BosonsBulk plug-in with the shape globals and k-vector files set up, as
+// the driver's factory and Init leave it (src/TDVMC.cpp:2850, :567-613), for
+// a caller that drives it through PhysicalSystems::IPhysicalSystem*
+// (tests/cpp/iphys_check.cpp).  Never deleted (see free_system).
+void* ref_new_bosonsbulk(int n, int dim, double L, int nparam, const double* sysparams, int nsys, int gr_bins)
+{
+    Setup s = make_system(n, dim, L, nparam, sysparams, nsys, 0.0, gr_bins);
+    return static_cast<PhysicalSystems::IPhysicalSystem*>(s.sys);
+}
+
 // Incremental move of one particle (CalculateWFQuotient, BosonsBulk.cpp:553-657):
 // R is the configuration before the move; particle p goes to newpos.
 // Returns exponentNew; *quotient receives exp(2 (exponentNew - exponent)).

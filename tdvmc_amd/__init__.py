@@ -47,7 +47,7 @@ class Info(C.Structure):
         ("hbar2_2m", C.c_double), ("imaginary_time", C.c_int), ("ode_solver_type", C.c_int),
         ("precision", C.c_int), ("samples_per_walker", C.c_int), ("timestep", C.c_double),
         ("mc_step", C.c_double), ("therm_moves", C.c_int64), ("init_moves", C.c_int64), ("time", C.c_double),
-        ("reg_eps", C.c_double), ("precondition", C.c_int), ("K_terms", C.c_int),
+        ("reg_eps", C.c_double), ("precondition", C.c_int), ("K_terms", C.c_int), ("gr_bin_count", C.c_int),
     ]
 
 
@@ -81,6 +81,11 @@ SYMBOLS = {
     "tdvmc_eval_configs": (C.c_int, [C.c_void_p, _dp, C.c_int64, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "tdvmc_eval_configs_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tdvmc_sample_dev": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]),
+    "tdvmc_measure_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tdvmc_set_walkers_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "tdvmc_get_walkers_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "tdvmc_observables_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp, _dp]),
     "tdvmc_observables_configs": (C.c_int, [C.c_void_p, _dp, C.c_int64, C.c_int, _dp, _dp]),
     "tdvmc_additional_properties": (C.c_int, [C.c_void_p, C.c_int64, C.c_uint64, C.c_int, _dp, _dp]),
@@ -89,6 +94,12 @@ SYMBOLS = {
     "tdvmc_files_write_grids": (C.c_int, [C.c_char_p, _dp, C.c_int, _dp, C.c_int]),
     "tdvmc_files_append_observables": (C.c_int, [C.c_char_p, C.c_double, _dp, C.c_int, _dp, C.c_int]),
     "tdvmc_write_final_config": (C.c_int, [C.c_char_p, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_char_p]),
+    "tdvmc_files_write_coordinates": (C.c_int, [C.c_char_p, _dp, C.c_int64, C.c_int, C.c_int, C.c_double]),
+    "tdvmc_files_read_coordinates": (C.c_int, [C.c_char_p, _dp, C.c_int, C.c_int]),
+    "tdvmc_files_write_rng_state": (C.c_int, [C.c_char_p, C.c_uint64, C.c_uint32, C.c_int]),
+    "tdvmc_files_read_rng_state": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "tdvmc_get_rng_state": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "tdvmc_set_rng_state": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32]),
     "tdvmc_moments": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int64, C.c_int, C.c_int, _dp, _dp, _dp]),
     "tdvmc_solve": (C.c_int, [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp,
                               C.POINTER(C.c_int)]),
@@ -374,6 +385,15 @@ class Context:
 
     def synchronize(self):
         self._check(lib().tdvmc_synchronize(self._h))
+
+    def get_rng_state(self):
+        """(seed, epoch) of the context's counter-based random stream."""
+        s, e = C.c_uint64(), C.c_uint32()
+        self._check(lib().tdvmc_get_rng_state(self._h, C.byref(s), C.byref(e)))
+        return s.value, e.value
+
+    def set_rng_state(self, seed: int, epoch: int):
+        self._check(lib().tdvmc_set_rng_state(self._h, int(seed), int(epoch)))
 
 
 def batch_system_seed(seed: int, s: int) -> int:

@@ -23,6 +23,7 @@ namespace {
 
 thread_local std::string g_create_error;
 
+
 struct Prof {
     double ms = 0.0;
     int64_t launches = 0;
@@ -60,6 +61,7 @@ struct tdvmc_ctx {
     int lanes_tuned[2] = {0, 0};      // chosen by tdvmc_autotune per precision (0: not tuned)
     std::string err;
     hipStream_t stream = nullptr;
+    hipEvent_t ev_join = nullptr;     // orders a caller's stream against the context stream (*_dev entries)
     int device = 0;
 
     std::vector<double> uR, uI;
@@ -338,7 +340,10 @@ int build_system(tdvmc_ctx* ctx)
     ctx->dt = c.TIMESTEP;
     ctx->therm_moves = c.MC_NTHERMSTEPS > 0 ? c.MC_NTHERMSTEPS : 0;
     ctx->init_moves = c.MC_NINITIALIZATIONSTEPS > 0 ? c.MC_NINITIALIZATIONSTEPS : 0;
-    ctx->spw = c.SAMPLES_PER_WALKER > 0 ? c.SAMPLES_PER_WALKER : 1;
+    // samples per walker and stage: SAMPLES_PER_WALKER, else MC_NSTEPS (the
+    // reference draws MC_NSTEPS samples per rank and stage, src/TDVMC.cpp:1073-1080,
+    // and a walker here is what a rank is there), else 1
+    ctx->spw = c.SAMPLES_PER_WALKER > 0 ? c.SAMPLES_PER_WALKER : (c.MC_NSTEPS > 0 ? c.MC_NSTEPS : 1);
     ctx->seed = c.SEED > 0 ? (uint64_t)c.SEED : 1u;
 
     // parameters: PARAMS_REAL / PARAMS_IMAGINARY of length P, or of the pair block only
@@ -1038,6 +1043,38 @@ int fill_step_data(tdvmc_ctx* ctx, int64_t n, tdvmc_step_data* out)
     return TDVMC_OK;
 }
 
+// Device-pointer entry points run on the context stream, ordered against the
+// caller's stream st: the context stream first waits for what st holds
+// (join_in), and st then waits for the context's work (join_out); no host
+// synchronisation.  A null st (or the context stream itself) needs neither.
+int join_in(tdvmc_ctx* ctx, hipStream_t st)
+{
+    if (!st || st == ctx->stream) return TDVMC_OK;
+    if (!ctx->ev_join) HIPC(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    HIPC(hipEventRecord(ctx->ev_join, st));
+    HIPC(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+    return TDVMC_OK;
+}
+
+int join_out(tdvmc_ctx* ctx, hipStream_t st)
+{
+    if (!st || st == ctx->stream) return TDVMC_OK;
+    HIPC(hipEventRecord(ctx->ev_join, ctx->stream));
+    HIPC(hipStreamWaitEvent(st, ctx->ev_join, 0));
+    return TDVMC_OK;
+}
+
+// mean [P+2], S [P][P], F [2][P] of the last moments into caller device buffers
+int copy_moments_dev(tdvmc_ctx* ctx, double* mean_dev, double* S_dev, double* F_dev)
+{
+    const size_t P = (size_t)P_of(ctx);
+    if (mean_dev)
+        HIPC(hipMemcpyAsync(mean_dev, ctx->dMean, (P + 2) * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    if (S_dev) HIPC(hipMemcpyAsync(S_dev, ctx->dS, P * P * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    if (F_dev) HIPC(hipMemcpyAsync(F_dev, ctx->dF, 2 * P * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    return TDVMC_OK;
+}
+
 int alloc_ctx(tdvmc_ctx* ctx)
 {
     const int P = P_of(ctx);
@@ -1083,6 +1120,7 @@ int alloc_ctx(tdvmc_ctx* ctx)
 void release(tdvmc_ctx* ctx)
 {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     dfree(ctx->obs.dVec);
     dfree(ctx->obs.dOff);
     dfree(ctx->obs.dScale);
@@ -1724,6 +1762,7 @@ int tdvmc_get_info(tdvmc_ctx* ctx, tdvmc_info* info)
     info->reg_eps = ctx->reg_eps;
     info->precondition = ctx->precondition;
     info->K_terms = ctx->sy.Kh;
+    info->gr_bin_count = ctx->cfg.GR_BIN_COUNT;
     return TDVMC_OK;
 }
 
@@ -1912,6 +1951,61 @@ int tdvmc_measure(tdvmc_ctx* ctx, int precision, tdvmc_step_data* out)
     rc = fill_step_data(ctx, ctx->W, out);
     if (rc) return rc;
     return prof_flush(ctx);
+}
+
+int tdvmc_sample_dev(tdvmc_ctx* ctx, uint64_t seed, int precision, double* mean_dev, double* S_dev, double* F_dev,
+                     void* stream)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    int prec, rc = resolve_precision(ctx, precision, &prec);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if ((rc = join_in(ctx, st))) return rc;
+    rc = prec == TDVMC_MIXED ? ensure_tables(ctx, ctx->tf, false) : ensure_tables(ctx, ctx->td, false);
+    if (rc) return rc;
+    if ((rc = enqueue_stage_samples(ctx, prec, seed))) return rc;
+    if ((rc = copy_moments_dev(ctx, mean_dev, S_dev, F_dev))) return rc;
+    return join_out(ctx, st);
+}
+
+int tdvmc_measure_dev(tdvmc_ctx* ctx, int precision, double* mean_dev, double* S_dev, double* F_dev, void* stream)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    int prec, rc = resolve_precision(ctx, precision, &prec);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if ((rc = join_in(ctx, st))) return rc;
+    rc = prec == TDVMC_MIXED ? ensure_tables(ctx, ctx->tf, false) : ensure_tables(ctx, ctx->td, false);
+    if (rc) return rc;
+    if ((rc = enqueue_eval(ctx, prec, 0))) return rc;
+    if ((rc = enqueue_moments(ctx, prec, ctx->W))) return rc;
+    if ((rc = copy_moments_dev(ctx, mean_dev, S_dev, F_dev))) return rc;
+    return join_out(ctx, st);
+}
+
+int tdvmc_set_walkers_dev(tdvmc_ctx* ctx, const double* R_dev, int64_t W, void* stream)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(R_dev != nullptr && W == ctx->W,
+              "R_dev is null or W differs from the walker count (change it with tdvmc_set_walker_count)");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    int rc = join_in(ctx, st);
+    if (rc) return rc;
+    HIPC(hipMemcpyAsync(ctx->dR, R_dev, (size_t)W * ctx->sy.N * ctx->sy.D * sizeof(double),
+                        hipMemcpyDeviceToDevice, ctx->stream));
+    return join_out(ctx, st);
+}
+
+int tdvmc_get_walkers_dev(tdvmc_ctx* ctx, double* R_dev, int64_t W, void* stream)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(R_dev != nullptr && W == ctx->W, "R_dev is null or W differs from the walker count");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    int rc = join_in(ctx, st);
+    if (rc) return rc;
+    HIPC(hipMemcpyAsync(R_dev, ctx->dR, (size_t)W * ctx->sy.N * ctx->sy.D * sizeof(double),
+                        hipMemcpyDeviceToDevice, ctx->stream));
+    return join_out(ctx, st);
 }
 
 int tdvmc_eval_configs_dev(tdvmc_ctx* ctx, const double* R_dev, int64_t W, int precision, double* expo_dev,
@@ -2370,6 +2464,28 @@ int tdvmc_adjoint(tdvmc_ctx* ctx, const double* R, int64_t W, const double* gbar
         ctx->err = std::string("adjoint: ") + hipGetErrorString(e);
         return TDVMC_ERR_HIP;
     }
+    return TDVMC_OK;
+}
+
+int tdvmc_get_rng_state(tdvmc_ctx* ctx, uint64_t* seed, uint32_t* epoch)
+{
+    if (!ctx || !seed || !epoch) return TDVMC_ERR_BAD_ARG;
+    uint32_t e = 0;
+    HIPC(hipMemcpyAsync(&e, ctx->dEpoch, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    *seed = ctx->seed;
+    *epoch = e;
+    return TDVMC_OK;
+}
+
+int tdvmc_set_rng_state(tdvmc_ctx* ctx, uint64_t seed, uint32_t epoch)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    HIPC(hipStreamSynchronize(ctx->stream));
+    HIPC(launch_fill_u32(ctx->dEpoch, epoch, 1, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    if (seed != ctx->seed) drop_graph(ctx);   // a captured step holds the seed in its launch arguments
+    ctx->seed = seed;
     return TDVMC_OK;
 }
 

@@ -42,7 +42,7 @@ struct Config {
     double TRAP_OMEGA = 0;          // external 0.5 omega^2 r^2
     double SPLINE_CUTOFF = 0;       // 0 -> LBOX / 2
     int N_WALKERS = 0;              // 0 -> 1
-    int SAMPLES_PER_WALKER = 0;     // 0 -> 1
+    int SAMPLES_PER_WALKER = 0;     // 0 -> MC_NSTEPS, or 1 without it
     int64_t SEED = 0;               // 0 -> 1
     std::string PRECISION;          // FP64 | MIXED ("" -> FP64)
     double REG_EPS = 0;             // 0 -> 0.001 (src/TDVMC.cpp:1737)

@@ -11,9 +11,14 @@
 #include "../../include/tdvmc.h"
 #include "config.hpp"
 
+#include <cinttypes>
+#include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <exception>
 #include <string>
+#include <sys/stat.h>
+#include <vector>
 
 namespace {
 
@@ -51,9 +56,126 @@ int append_line(const char* dir, const char* name, const double* v, int n)
     return rc ? TDVMC_ERR_BAD_ARG : TDVMC_OK;
 }
 
+// GetCoordinateNIC (src/Utils.cpp:266-281): x - k L with k the quotient
+// x (1/L) rounded half away from zero, and a result of exactly +-L/2 nudged
+// inwards by 1e-10 (the same arithmetic as there: 1/L and L/2 precomputed)
+double nic_coordinate(double x, double L)
+{
+    const double Lr = 1.0 / L, L2 = L / 2.0;
+    const int k = (int)(x * Lr + ((x >= 0.0) ? 0.5 : -0.5));
+    double r = x - k * L;
+    if (r == L2)
+        r -= 1e-10;
+    else if (r == -L2)
+        r += 1e-10;
+    return r;
+}
+
+int ensure_dir(const std::string& d)
+{
+    struct stat st;
+    if (stat(d.c_str(), &st) == 0) return S_ISDIR(st.st_mode) ? 0 : -1;
+    return mkdir(d.c_str(), 0755);
+}
+
 }  // namespace
 
 extern "C" {
+
+int tdvmc_files_write_coordinates(const char* out_dir, const double* R, int64_t W, int N, int D, double L)
+{
+    if (!out_dir || !R || W < 1 || N < 1 || D < 1 || D > 3) return TDVMC_ERR_BAD_ARG;
+    const std::string dir(out_dir);
+    if (ensure_dir(dir + "coords")) return TDVMC_ERR_BAD_ARG;
+    std::vector<double> row((size_t)N * D);
+    for (int64_t w = 0; w < W; w++) {
+        // AlignCoordinates (src/TDVMC.cpp:2569-2582, :787-796): into the first cell
+        for (size_t i = 0; i < row.size(); i++) {
+            const double x = R[(size_t)w * row.size() + i];
+            row[i] = L > 0 ? nic_coordinate(x, L) : x;
+        }
+        const std::string name = "coords/particleconfiguration_" + std::to_string(N) + "_" + std::to_string(D) +
+                                 "D_" + std::to_string((long long)w);
+        std::FILE* f = std::fopen(path_of(out_dir, name.c_str()).c_str(), "w");
+        if (!f) return TDVMC_ERR_BAD_ARG;
+        int rc = put_values(f, row.data(), (int)row.size(), false);
+        rc |= std::fclose(f);
+        if (rc) return TDVMC_ERR_BAD_ARG;
+        if (w == 0) {
+            // the root rank's copy for the next run's config directory (:4019-4022)
+            const std::string fin = "AAFinish_particleconfiguration_" + std::to_string(N) + "_" + std::to_string(D) + "D";
+            std::FILE* g = std::fopen(path_of(out_dir, fin.c_str()).c_str(), "w");
+            if (!g) return TDVMC_ERR_BAD_ARG;
+            int rc2 = put_values(g, row.data(), (int)row.size(), false);
+            rc2 |= std::fclose(g);
+            if (rc2) return TDVMC_ERR_BAD_ARG;
+        }
+    }
+    return TDVMC_OK;
+}
+
+int tdvmc_files_read_coordinates(const char* path, double* R, int N, int D)
+{
+    if (!path || !R || N < 1 || D < 1) return TDVMC_ERR_BAD_ARG;
+    std::FILE* f = std::fopen(path, "r");
+    if (!f) return TDVMC_ERR_BAD_ARG;
+    // the last line of the file (LoadLastPositionsFromFile, src/TDVMC.cpp:640-670)
+    std::string line, last;
+    int c;
+    while ((c = std::fgetc(f)) != EOF) {
+        if (c == '\n') {
+            if (!line.empty()) last = line;
+            line.clear();
+        } else if (c != '\r') {
+            line.push_back((char)c);
+        }
+    }
+    if (!line.empty()) last = line;
+    std::fclose(f);
+    // comma-separated values, DIM per particle in particle order
+    std::vector<double> v;
+    const char* p = last.c_str();
+    while (*p) {
+        char* end = nullptr;
+        const double x = std::strtod(p, &end);
+        if (end == p) break;
+        v.push_back(x);
+        p = end;
+        while (*p == ',' || *p == ' ' || *p == '\t') p++;
+    }
+    if ((int64_t)v.size() != (int64_t)N * D) return TDVMC_ERR_BAD_ARG;
+    for (size_t i = 0; i < v.size(); i++) R[i] = v[i];
+    return TDVMC_OK;
+}
+
+int tdvmc_files_write_rng_state(const char* out_dir, uint64_t seed, uint32_t epoch, int rank)
+{
+    if (!out_dir || rank < 0) return TDVMC_ERR_BAD_ARG;
+    const std::string dir(out_dir);
+    if (ensure_dir(dir + "random")) return TDVMC_ERR_BAD_ARG;
+    const std::string path = dir + "random/state_philox_" + std::to_string(rank) + ".dat";
+    std::FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) return TDVMC_ERR_BAD_ARG;
+    int rc = std::fprintf(f, "philox4x32-10 %" PRIu64 " %" PRIu32 "\n", seed, epoch) < 0 ? -1 : 0;
+    rc |= std::fclose(f);
+    return rc ? TDVMC_ERR_BAD_ARG : TDVMC_OK;
+}
+
+int tdvmc_files_read_rng_state(const char* path, uint64_t* seed, uint32_t* epoch)
+{
+    if (!path || !seed || !epoch) return TDVMC_ERR_BAD_ARG;
+    std::FILE* f = std::fopen(path, "r");
+    if (!f) return TDVMC_ERR_BAD_ARG;
+    char name[32] = {0};
+    uint64_t s = 0;
+    uint32_t e = 0;
+    const int n = std::fscanf(f, "%31s %" SCNu64 " %" SCNu32, name, &s, &e);
+    std::fclose(f);
+    if (n != 3 || std::string(name) != "philox4x32-10") return TDVMC_ERR_BAD_ARG;
+    *seed = s;
+    *epoch = e;
+    return TDVMC_OK;
+}
 
 int tdvmc_files_init(const char* out_dir, const double* uR, const double* uI, int P)
 {

@@ -27,6 +27,8 @@ Outputs (inputs and expected outputs only; no reference source):
                          sides, solutions of both linear solvers (with and
                          without scaling) and phi-dot, real and imaginary time
   driver_cases.json   -- AdjustParameters and LJ::GetPotential values
+  resume_cases.json   -- the driver's successive-run coordinate files (text)
+                         and the coordinates its loader reads from config/
   sample_bulk3d_sw16.json -- <E_R> +- error of 256 reference Metropolis chains
 
     python tests/golden/make_golden.py obs    # only the observables fixtures
@@ -201,6 +203,40 @@ def eval_large():
         np.savez_compressed(os.path.join(HERE, f"eval_{name}.npz"), D=D, N=N, L=L, P=P, sysparams=spa, time=time,
                             R=R, uR=uR, uI=uI, phiR=phiR, phiI=phiI, expo=expo, O=O, ER=ER, EI=EI, other=oth,
                             grad=grad, lap=lap)
+
+
+def vmcsampler_case():
+    """The 64-particle 3-D configuration and the 100 spline parameters held in
+    src/VMCSampler.h:18-29 (a prototype sampler the driver does not use; its
+    own basis differs), evaluated by the reference's BosonsBulk plug-in in a
+    box of L = 4 (the fixture's coordinates lie in [-2, 2], maxDistance = 2):
+    the fixture and three jittered copies, square-well potential, real
+    parameters; gradient / Laplacian contracted as for the large cases."""
+    import re
+    r = o.ref()
+    text = open(os.path.join(REF, "src", "VMCSampler.h")).read()
+    body = text[text.index("vector<vector<double> > R = {"):text.index("vector<double> params")]
+    R0 = np.array([float(x) for x in re.findall(r"-?\d+\.\d+(?:[eE][-+]?\d+)?", body)]).reshape(64, 3)
+    pb = text[text.index("vector<double> params = {"):text.index("int nAcceptances")]
+    uR0 = np.array([float(x) for x in re.findall(r"-?\d+\.\d+(?:[eE][-+]?\d+)?", pb)])
+    assert uR0.size == 100
+    N, D, L, P, W = 64, 3, 4.0, 100, 4
+    rng = np.random.default_rng(19)
+    R = np.ascontiguousarray(np.stack([R0] + [R0 + rng.uniform(-0.02, 0.02, R0.shape) for _ in range(W - 1)]))
+    uR, uI = uR0.copy(), np.zeros(P)
+    phiR, phiI = 0.0, 0.0
+    sp = np.array([0.3, 2.0])
+    S = P + 2
+    expo, O, ER, EI = np.zeros(W), np.zeros((W, P)), np.zeros(W), np.zeros(W)
+    oth, sD, sD2 = np.zeros((W, 9)), np.zeros((W, S, N, D)), np.zeros((W, S, N))
+    r.ref_bosonsbulk_eval(N, D, L, P, f(sp), 2, 0.0, W, f(R), f(uR), f(uI), phiR, phiI, f(expo), f(O), f(ER), f(EI),
+                          f(oth), f(sD), f(sD2))
+    cR = _spline_coeff(uR, P)
+    grad = np.stack([np.einsum("s,wsna->wna", cR, sD), np.zeros((W, N, D))], -1)
+    lap = np.stack([np.einsum("s,wsn->wn", cR, sD2), np.zeros((W, N))], -1)
+    np.savez_compressed(os.path.join(HERE, "eval_vmcsampler_n64_p100.npz"), D=D, N=N, L=L, P=P, sysparams=sp,
+                        time=0.0, R=R, uR=uR, uI=uI, phiR=phiR, phiI=phiI, expo=expo, O=O, ER=ER, EI=EI, other=oth,
+                        grad=grad, lap=lap)
 
 
 def move_cases():
@@ -379,6 +415,35 @@ def tdvp_cases():
                                v_he=v_he.tolist(), sigma_he=2.556, eps_he=10.22)), fh)
 
 
+def resume_cases():
+    """The successive-run coordinate file as the reference driver writes it
+    (MoveCoordinatesToFirstCell + WriteParticleInputFile), as text, for walkers
+    partly outside the box in 1, 2 and 3 dimensions; and the coordinates its
+    LoadLastPositionsFromFile reads from config/particleconfiguration_400.csv
+    (2-D, N = 400) and _64.csv (3-D, N = 64)."""
+    import tempfile
+    L = drv()
+    rng = np.random.default_rng(61)
+    files = []
+    for (n, d, box) in [(5, 1, 6.0), (7, 2, 5.0), (9, 3, 4.0)]:
+        R = np.ascontiguousarray(rng.uniform(-1.2 * box, 1.2 * box, (n, d)))
+        R[0, 0] = box / 2.0          # exactly on the half-box: the nudge of GetCoordinateNIC
+        R[1, 0] = -box / 2.0
+        with tempfile.TemporaryDirectory() as tmp:
+            L.refd_write_particle_file((tmp + "/").encode(), b"coords_case", f(R), n, d, C.c_double(box), 1)
+            text = open(os.path.join(tmp, "coords_case.dat")).read()
+        files.append(dict(N=n, D=d, L=box, R=R.ravel().tolist(), text=text))
+    loads = []
+    for (name, n, d) in [("particleconfiguration_400", 400, 2), ("particleconfiguration_64", 64, 3)]:
+        out = np.zeros(n * d)
+        rc = L.refd_load_positions((REF + "/config/").encode(), name.encode(), f(out), n, d)
+        assert rc == 0
+        loads.append(dict(file="refconf_" + name + ".csv", N=n, D=d, R=out.tolist()))
+    with open(os.path.join(HERE, "resume_cases.json"), "w") as fh:
+        json.dump(dict(source="src/TDVMC.cpp:493-504, :640-670, :787-796 (WriteParticleInputFile, "
+                              "LoadLastPositionsFromFile, MoveCoordinatesToFirstCell)", write=files, load=loads), fh)
+
+
 # Reference Metropolis chains of a 3-D square-well system (BosonsBulk with its
 # own incremental move ratio, src/PhysicalSystems/BosonsBulk.cpp:553-665, and
 # the move loop of DoMetropolisStep, src/TDVMC.cpp:858-916): independent
@@ -420,9 +485,11 @@ if __name__ == "__main__":
         sys.exit("oracle/_ref/libref.so missing: run `make -C oracle ref` in a container with the reference")
     if sys.argv[1:] == ["drv"]:
         tdvp_cases()
+        resume_cases()
         sys.exit(0)
     if sys.argv[1:] == ["eval_large"]:
         eval_large()
+        vmcsampler_case()
         sys.exit(0)
     if sys.argv[1:] == ["sample"]:
         sample_case()

@@ -164,3 +164,57 @@ def test_final_config_matches_reference_format(tmp_path, golden_dir):
     back = json.loads(text)
     np.testing.assert_allclose(back["PARAMS_REAL"], uR, rtol=1e-14)
     assert back["PARAM_PHIR"] == float("%f" % phiR)
+
+
+# ---------------------------------------------------------------- resume files
+
+def test_coordinate_files_vs_reference_writer(tmp_path, golden_dir):
+    """The successive-run coordinate files (src/TDVMC.cpp:4007-4022) against the
+    text the reference driver's own MoveCoordinatesToFirstCell +
+    WriteParticleInputFile wrote (tests/golden/resume_cases.json), including
+    coordinates on exactly +-L/2 (the 1e-10 nudge) and far outside the box."""
+    cases = json.load(open(os.path.join(golden_dir, "resume_cases.json")))["write"]
+    L = tdvmc.lib()
+    for i, c in enumerate(cases):
+        d = tmp_path / f"case{i}"
+        d.mkdir()
+        R = np.array(c["R"], dtype=np.float64)
+        N, D = c["N"], c["D"]
+        assert L.tdvmc_files_write_coordinates((str(d) + "/").encode(), _dp(R), 1, N, D, c["L"]) == 0
+        got = (d / "coords" / f"particleconfiguration_{N}_{D}D_0.dat").read_text()
+        assert got == c["text"], (got, c["text"])
+        assert (d / f"AAFinish_particleconfiguration_{N}_{D}D.dat").read_text() == c["text"]
+    # several walkers: one file per walker, as one per MPI rank
+    d = tmp_path / "multi"
+    d.mkdir()
+    R = np.arange(3 * 4 * 2, dtype=np.float64).reshape(3, 4, 2) * 0.37
+    assert L.tdvmc_files_write_coordinates((str(d) + "/").encode(), _dp(R), 3, 4, 2, 0.0) == 0
+    for w in range(3):
+        vals = [float(x) for x in (d / "coords" / f"particleconfiguration_4_2D_{w}.dat").read_text().split()]
+        assert np.allclose(vals, R[w].ravel(), rtol=1e-15, atol=0)   # open space: as they are (16 digits)
+
+
+def test_read_coordinates_vs_reference_loader(golden_dir):
+    """LoadLastPositionsFromFile (src/TDVMC.cpp:640-670) on the reference's own
+    config/particleconfiguration_400.csv and _64.csv: the same doubles."""
+    cases = json.load(open(os.path.join(golden_dir, "resume_cases.json")))["load"]
+    for c in cases:
+        R = np.zeros(c["N"] * c["D"])
+        rc = tdvmc.lib().tdvmc_files_read_coordinates(os.path.join(golden_dir, c["file"]).encode(), _dp(R),
+                                                       c["N"], c["D"])
+        assert rc == 0
+        assert np.array_equal(R, np.array(c["R"]))
+    # a file with the wrong count is refused
+    R = np.zeros(12)
+    assert tdvmc.lib().tdvmc_files_read_coordinates(os.path.join(golden_dir, cases[0]["file"]).encode(), _dp(R),
+                                                     4, 3) == tdvmc.ERR_BAD_ARG
+
+
+def test_rng_state_file_roundtrip(tmp_path):
+    L = tdvmc.lib()
+    seed, epoch = 0xDEADBEEFCAFEF00D, 123456
+    assert L.tdvmc_files_write_rng_state((str(tmp_path) + "/").encode(), seed, epoch, 5) == 0
+    s, e = C.c_uint64(), C.c_uint32()
+    path = tmp_path / "random" / "state_philox_5.dat"
+    assert L.tdvmc_files_read_rng_state(str(path).encode(), C.byref(s), C.byref(e)) == 0
+    assert (s.value, e.value) == (seed, epoch)

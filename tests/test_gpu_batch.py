@@ -312,6 +312,9 @@ def test_batch_member_walker_count_change(prec):
             ctx.set_params(uR, uI)
             ctx.set_walkers(starts[i])
             ctx.set_sweep_lanes(32)
+            # the batch's stream epoch before this sweep: its first sweep and the
+            # step's sweep launch advanced it once each
+            ctx.set_rng_state(ctx.get_rng_state()[0], 2)
             ctx.sweep(30, seed=tdvmc.batch_system_seed(seed + 1, i), precision=prec)
             assert np.array_equal(ctx.get_walkers(), got[i]), f"system {i}"
         out = o.eval_configs(s, got[i], uR, uI)

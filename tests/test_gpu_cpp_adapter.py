@@ -69,3 +69,42 @@ def test_cpp_adapter_vs_oracle(tmp_path):
     assert rel(res["S_diag"], ref["O"].var(0)) < 1e-9
     assert int(res["n_samples"][0]) == W
     assert int(res["error_caught"][0]) == 1       # TDVMC_ERR_BAD_ARG surfaced as tdvmc::Error
+
+
+IPHYS = os.path.join(ROOT, "tests", "cpp", "build", "iphys_check")
+
+
+def test_iphysicalsystem_adapter_vs_reference_bosonsbulk(tmp_path):
+    """PhysicalSystems::GpuBulkBosons (include/tdvmc_iphysicalsystem.hpp)
+    against the reference's own BosonsBulk, both driven through the
+    reference's IPhysicalSystem* by tests/cpp/iphys_check.cpp (built with the
+    reference's headers and its compiled plug-in, oracle/_ref/libref.so):
+    wave function and exponent, a Metropolis chain of trial quotients and
+    accepted moves, expectation values and their products, the nine 'other'
+    values, g(r) and S(k) in the plug-ins' observables, and the
+    correlated-sampling overloads."""
+    assert os.path.exists(IPHYS), "tests/cpp/build/iphys_check is not built (make -C oracle iphys)"
+    rng = np.random.default_rng(4)
+    N, D, L, P = 27, 3, 4.5, 10
+    cfg = make_cfg(N=N, D=D, L=L, P_s=P, pot="SQUARE_WELL", sysparams=(0.7, 2.0), W=1, GR_BIN_COUNT=40,
+                   PARAMS_REAL=rng.normal(0, 0.3, P).tolist(), PARAMS_IMAGINARY=rng.normal(0, 0.2, P).tolist())
+    path = tmp_path / "sys.config"
+    path.write_text(json.dumps(cfg))
+    print("iphys_check", flush=True)
+    r = subprocess.run([IPHYS, str(path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = {}
+    for line in r.stdout.splitlines():
+        k, *v = line.split()
+        out[k] = [float(x) for x in v]
+    print(r.stdout)
+    assert 0 < out["accepted"][0] < out["moves"][0]
+    # the reference's incremental sums (clamped, src/PhysicalSystems/BosonsBulk.cpp:617-621)
+    # drift from a full evaluation by round-off only
+    for k in ("err_expo", "err_wf", "err_quotient", "err_O", "err_OO", "err_OE", "err_ER", "err_EI", "err_other"):
+        assert out[k][0] < 1e-9, k
+    assert out["gr_bins"][0] == out["gr_bins"][1] > 0
+    assert out["gr_grid_err"][0] < 1e-14
+    assert out["err_gr"][0] < 1e-12 and out["err_sk"][0] < 1e-10
+    assert out["err_cs"][0] < 1e-12 and out["com"][0] == 0.0
+    assert out["nic"][0] == 1 and out["normalization"][0] == 1 and out["n_other"][0] == 9

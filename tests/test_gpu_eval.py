@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as o
-from tests.helpers import energy_scale, make_cfg, oracle_system, rel
+from tests.helpers import energy_scale, make_cfg, oracle_system, ref_golden_tol, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -56,7 +56,7 @@ def test_fp64_vs_reference_golden(path):
             continue
         assert rel(out["other"][:, k], ref["other"][:, k]) < 1e-10, k
     # against the reference's golden vectors directly
-    tol = 1e-10 if P <= 16 else 2e-7
+    tol = ref_golden_tol(P)
     assert rel(out["expo"], z["expo"]) < tol
     assert rel(out["O"], z["O"]) < tol
     assert rel(out["ER"], z["ER"]) < tol

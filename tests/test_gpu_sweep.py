@@ -189,3 +189,36 @@ def test_sampled_energy_vs_reference_chains(golden_dir, precision):
     print(f"GPU <E_R> {mean:.4f} +- {sigma:.4f}, reference {g['mean']:.4f} +- {g['sigma']:.4f}: "
           f"{diff / comb:.2f} sigma", flush=True)
     assert diff < 4.5 * comb
+
+
+def test_rng_state_resume(tmp_path):
+    """A chain stopped and resumed from saved walkers and a saved (seed, epoch)
+    (tdvmc_get_rng_state, tdvmc_files_write_rng_state / read, tdvmc_set_rng_state)
+    continues bit for bit as the uninterrupted chain."""
+    import ctypes as C
+
+    cfg = make_cfg(N=16, D=3, L=3.0, P_s=8, pot="GAUSSIAN", sysparams=(0.5, 1.0), K=6, hbar2_2m=0.5, W=256,
+                   step=0.3, SEED=17)
+    uR = np.random.default_rng(2).normal(0, 0.1, 14)
+    print("rng resume", flush=True)
+    with tdvmc.Context(cfg) as a:
+        a.set_params(uR)
+        a.sweep(160, seed=5)
+        R = a.get_walkers()
+        seed, epoch = a.get_rng_state()
+        L = tdvmc.lib()
+        assert L.tdvmc_files_write_rng_state((str(tmp_path) + "/").encode(), seed, epoch, 0) == 0
+        a.sweep(96, seed=9)
+        a.sweep(32, seed=11)
+        R_cont = a.get_walkers()
+    s, e = C.c_uint64(), C.c_uint32()
+    assert L.tdvmc_files_read_rng_state(str(tmp_path / "random" / "state_philox_0.dat").encode(), C.byref(s),
+                                        C.byref(e)) == 0
+    assert (s.value, e.value) == (17, epoch) and epoch == 1
+    with tdvmc.Context(cfg) as b:
+        b.set_params(uR)
+        b.set_walkers(R)
+        b.set_rng_state(s.value, e.value)
+        b.sweep(96, seed=9)
+        b.sweep(32, seed=11)
+        assert np.array_equal(b.get_walkers(), R_cont)

@@ -264,3 +264,24 @@ def test_trial_counts_graph_and_eager(ode):
             counts.append(trials)
             assert 0 < acc <= trials
     assert counts[0] == counts[1] == 3 * stages * (init + therm) * W
+
+
+def test_reference_config_samples_mc_nsteps(golden_dir):
+    """The reference's own config/BosonsBulk1D.config (1-D, N = 20, 100
+    converged parameters, MC_NSTEPS = 5000, MC_NTHERMSTEPS = 100,
+    MC_NINITIALIZATIONSTEPS = 200): one sampling stage draws MC_NSTEPS
+    samples per walker, as the reference does per rank (src/TDVMC.cpp:1073-1080),
+    each after MC_NTHERMSTEPS moves."""
+    import os
+
+    path = os.path.join(golden_dir, "config_BosonsBulk1D.json")
+    print("reference config sampling", flush=True)
+    with tdvmc.Context(path) as ctx:
+        info = ctx.get_info()
+        assert info.samples_per_walker == 5000 and info.n_walkers == 1
+        d = ctx.sample(seed=4, precision=tdvmc.FP64)
+        acc, trials = ctx.acceptance()
+    assert d["n_samples"] == 5000
+    assert trials == 200 + 5000 * 100
+    assert 0 < acc < trials
+    assert np.isfinite(d["localEnergyR"]) and np.all(np.isfinite(d["S"]))

@@ -34,6 +34,17 @@ def test_parse_reference_config_file(golden_dir):
     assert info.imaginary_time == 1
     assert info.therm_moves == 100 and info.init_moves == 200
     assert info.hbar2_2m == 1.0          # src/Constants.h:12
+    # MC_NSTEPS = 5000 samples per stage on the reference's one walker per rank
+    # (src/TDVMC.cpp:1073-1080): 5000 samples per walker, one walker by default
+    assert info.samples_per_walker == 5000 and info.n_walkers == 1
+
+
+def test_samples_per_walker_from_mc_nsteps():
+    base = {"N": 8, "DIM": 3, "LBOX": 4.0, "N_PARAM": 6, "SYSTEM_PARAMS": [0.5, 1.0]}
+    assert tdvmc.parse_config(json.dumps(base)).samples_per_walker == 1
+    assert tdvmc.parse_config(json.dumps(dict(base, MC_NSTEPS=40))).samples_per_walker == 40
+    info = tdvmc.parse_config(json.dumps(dict(base, MC_NSTEPS=40, SAMPLES_PER_WALKER=3)))
+    assert info.samples_per_walker == 3
 
 
 def test_parse_config_item_semantics():

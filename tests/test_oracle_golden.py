@@ -47,12 +47,9 @@ CASES = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "eval
 
 
 def _tol(P):
-    # The reference evaluates each cubic piece in the monomial basis of the
-    # absolute distance (coefficients ~ (knot index)^3), which carries a
-    # relative roundoff of order 1e-16 * P^3 * (number of pairs)^0.5; below
-    # P = 16 that is under 1e-11.  Larger cases are also checked against a
-    # 40-digit evaluation in test_oracle_math.py.
-    return 1e-10 if P <= 16 else 2e-7
+    from tests.helpers import ref_golden_tol
+
+    return ref_golden_tol(P)
 
 
 @pytest.mark.parametrize("path", CASES, ids=[os.path.basename(p) for p in CASES])
