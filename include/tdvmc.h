@@ -306,6 +306,19 @@ int tdvmc_get_step_status(tdvmc_ctx* ctx, int* n_bad);
 int tdvmc_adjoint(tdvmc_ctx* ctx, const double* R, int64_t W, const double* gbar, int precision, double* dR,
                   double* dalpha);
 
+/* Allocation-free, device-pointer variant of tdvmc_adjoint: the same kernels,
+ * with R_dev [W][N][DIM], gbar_dev [W], dR_dev [W][N][DIM] and dalpha_dev [P]
+ * in device memory (either output may be NULL), enqueued on `stream` (a
+ * hipStream_t; NULL: the context stream), returning without synchronising.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).  It calls
+ * no hipMalloc / hipFree; its scratch is one fixed 64 MiB device array per GPU,
+ * so W * (P + 2 N DIM + 2 N + 4) * 8 bytes must fit in 64 MiB (larger W:
+ * TDVMC_ERR_BAD_ARG).  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other streams, other contexts, other threads) race. */
+int tdvmc_adjoint_noalloc(tdvmc_ctx* ctx, const double* R_dev, int64_t W, const double* gbar_dev, int precision,
+                          double* dR_dev, double* dalpha_dev, void* stream);
+
 /* Kernel timing with HIP events on the context stream (Timer, src/Timer.h).
  * names: "sweep", "eval", "moments", "solve".  Accumulated since enable. */
 int tdvmc_profile_enable(tdvmc_ctx* ctx, int enable);

@@ -106,6 +106,8 @@ SYMBOLS = {
     "tdvmc_get_step_status": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "tdvmc_tdvp_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int]),
     "tdvmc_adjoint": (C.c_int, [C.c_void_p, _dp, C.c_int64, _dp, C.c_int, _dp, _dp]),
+    "tdvmc_adjoint_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_void_p,
+                                        C.c_void_p, C.c_void_p]),
     "tdvmc_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "tdvmc_profile_get": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "tdvmc_set_sweep_lanes": (C.c_int, [C.c_void_p, C.c_int]),

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -19,10 +20,20 @@
 
 using namespace tdvmc;
 
+// Fixed device scratch of tdvmc_adjoint_noalloc (one copy per GPU, held until
+// the process ends): the per-walker evaluation outputs the adjoint contracts.
+// In host code `scratch` names a host-side stand-in; its device address comes
+// from hipGetSymbolAddress, once per device (init_scratch).
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
 namespace {
 
 thread_local std::string g_create_error;
 
+constexpr int kMaxDevices = 64;
+double* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
 
 struct Prof {
     double ms = 0.0;
@@ -1043,6 +1054,22 @@ int fill_step_data(tdvmc_ctx* ctx, int64_t n, tdvmc_step_data* out)
     return TDVMC_OK;
 }
 
+// The device address of this device's copy of `scratch`, looked up once per
+// device with that device current (the context's device at creation).
+int init_scratch(tdvmc_ctx* ctx)
+{
+    if (ctx->device < 0 || ctx->device >= kMaxDevices) {
+        ctx->err = "device index " + std::to_string(ctx->device) + " beyond the scratch table";
+        return TDVMC_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[ctx->device]) return TDVMC_OK;
+    void* p = nullptr;
+    HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[ctx->device] = static_cast<double*>(p);
+    return TDVMC_OK;
+}
+
 // Device-pointer entry points run on the context stream, ordered against the
 // caller's stream st: the context stream first waits for what st holds
 // (join_in), and st then waits for the context's work (join_out); no host
@@ -1079,6 +1106,7 @@ int alloc_ctx(tdvmc_ctx* ctx)
 {
     const int P = P_of(ctx);
     HIPC(hipGetDevice(&ctx->device));
+    if (int rc = init_scratch(ctx)) return rc;
     HIPC(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     HIPC(dalloc(&ctx->dUR, P + 1));
     HIPC(dalloc(&ctx->dUI, P + 1));
@@ -2467,6 +2495,60 @@ int tdvmc_adjoint(tdvmc_ctx* ctx, const double* R, int64_t W, const double* gbar
     return TDVMC_OK;
 }
 
+int tdvmc_adjoint_noalloc(tdvmc_ctx* ctx, const double* R_dev, int64_t W, const double* gbar_dev, int precision,
+                          double* dR_dev, double* dalpha_dev, void* stream)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(R_dev && gbar_dev && W >= 1, "tdvmc_adjoint_noalloc: null device pointer or W < 1");
+    CHECK_ARG(dR_dev || dalpha_dev, "tdvmc_adjoint_noalloc: no output requested");
+    int prec, rc = resolve_precision(ctx, precision, &prec);
+    if (rc) return rc;
+    const int N = ctx->sy.N, D = ctx->sy.D, P = P_of(ctx);
+    // scratch per walker: O [P], grad [N D 2], lap [N 2], ln psi, E_R, E_I, outer count
+    const size_t per = (size_t)P + 2 * (size_t)N * D + 2 * (size_t)N + 4;
+    const int64_t wmax = (int64_t)(kScratchBytes / sizeof(double) / per);
+    if (W > wmax) {
+        ctx->err = "tdvmc_adjoint_noalloc: W = " + std::to_string(W) + " exceeds the " +
+                   std::to_string(kScratchBytes >> 20) + " MiB scratch, which holds " + std::to_string(wmax) +
+                   " walkers of this system";
+        return TDVMC_ERR_BAD_ARG;
+    }
+    double* base = g_scratch[ctx->device];
+    if (!base) {
+        ctx->err = "tdvmc_adjoint_noalloc: scratch not set up for this device";
+        return TDVMC_ERR_STATE;
+    }
+    // every scratch entry read below is written first by this call's
+    // evaluation, so nothing needs clearing between calls
+    double* dO = base;
+    double* dG = dO + (size_t)W * P;
+    double* dL = dG + (size_t)W * N * D * 2;
+    double* dEx = dL + (size_t)W * N * 2;
+    double* dER = dEx + W;
+    double* dEI = dER + W;
+    double* dOuter = dEI + W;
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    if (st != ctx->stream) HIPC(hipStreamSynchronize(ctx->stream));
+    hipStream_t keep = ctx->stream;
+    ctx->stream = st;
+    rc = prec == TDVMC_MIXED ? ensure_tables(ctx, ctx->tf, false) : ensure_tables(ctx, ctx->td, false);
+    ctx->stream = keep;
+    if (rc) return rc;
+    if (prec == TDVMC_MIXED) {
+        HIPC((launch_eval<float, double>(ctx->sy, tables_of(ctx, ctx->tf), R_dev, W, dEx, dER, dEI, dO, nullptr, dG,
+                                         dL, dOuter, nullptr, st)));
+    } else {
+        HIPC((launch_eval<double, double>(ctx->sy, tables_of(ctx, ctx->td), R_dev, W, dEx, dER, dEI, dO, nullptr, dG,
+                                          dL, dOuter, nullptr, st)));
+    }
+    if (dR_dev) HIPC(launch_adjoint_scale(dG, gbar_dev, W, N * D, dR_dev, st));
+    if (dalpha_dev) {
+        HIPC(launch_adjoint_reduce(dO, gbar_dev, W, P, dalpha_dev, st));
+        HIPC(launch_adjoint_outer(dOuter, gbar_dev, W, ctx->sy.outer_idx, dalpha_dev, st));
+    }
+    return TDVMC_OK;
+}
+
 int tdvmc_get_rng_state(tdvmc_ctx* ctx, uint64_t* seed, uint32_t* epoch)
 {
     if (!ctx || !seed || !epoch) return TDVMC_ERR_BAD_ARG;
@@ -2541,8 +2623,8 @@ int tdvmc_autotune(tdvmc_ctx* ctx, int precision, int* lanes_out)
     if (rc) return rc;
     const int N = ctx->sy.N, D = ctx->sy.D;
     const int64_t Wt = std::min<int64_t>(ctx->W, 16384);
-    double* scratch = nullptr;
-    HIPC(dalloc(&scratch, (size_t)Wt * N * D));
+    double* dRt = nullptr;
+    HIPC(dalloc(&dRt, (size_t)Wt * N * D));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
@@ -2552,7 +2634,7 @@ int tdvmc_autotune(tdvmc_ctx* ctx, int precision, int* lanes_out)
         if (e != hipSuccess) break;
         if ((ctx->sy.Kh + lanes - 1) / lanes > 8) continue;
         SweepArgs a;
-        a.R = scratch;
+        a.R = dRt;
         a.W = Wt;
         a.n_moves = N;
         a.seed = 0x9e3779b97f4a7c15ull;
@@ -2562,7 +2644,7 @@ int tdvmc_autotune(tdvmc_ctx* ctx, int precision, int* lanes_out)
         a.rho_out = nullptr;
         float ms = 0.f;
         for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
-            e = hipMemcpyAsync(scratch, ctx->dR, (size_t)Wt * N * D * sizeof(double), hipMemcpyDeviceToDevice,
+            e = hipMemcpyAsync(dRt, ctx->dR, (size_t)Wt * N * D * sizeof(double), hipMemcpyDeviceToDevice,
                                ctx->stream);
             if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
             if (e == hipSuccess)
@@ -2581,7 +2663,7 @@ int tdvmc_autotune(tdvmc_ctx* ctx, int precision, int* lanes_out)
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    dfree(scratch);
+    dfree(dRt);
     if (e != hipSuccess) {
         ctx->err = std::string("autotune: ") + hipGetErrorString(e);
         return TDVMC_ERR_HIP;
