@@ -36,7 +36,7 @@ class _CSystem(C.Structure):
         ("N", C.c_int), ("D", C.c_int), ("L", C.c_double), ("P_s", C.c_int), ("rc", C.c_double),
         ("K", C.c_int), ("kvec", _dp), ("one_body", C.c_int), ("hbar2_2m", C.c_double),
         ("pot_type", C.c_int), ("n_pot", C.c_int), ("pot", C.c_double * 5),
-        ("pot_cutoff", C.c_double), ("trap_omega", C.c_double),
+        ("pot_cutoff", C.c_double), ("trap_omega", C.c_double), ("grid", _dp),
     ]
 
 
@@ -58,8 +58,13 @@ class System:
     pot: tuple = ()
     pot_cutoff: float = 0.0   # default: rc
     trap_omega: float = 0.0
+    grid: np.ndarray = None   # non-uniform knots (NURBS_GRID, P_s values from 0); rc = grid[-1]
 
     def __post_init__(self):
+        if self.grid is not None:
+            self.grid = np.ascontiguousarray(np.asarray(self.grid, dtype=np.float64))
+            assert self.grid.size == self.P_s
+            self.rc = float(self.grid[-1])
         if self.rc <= 0:
             self.rc = self.L / 2.0
         if self.pot_cutoff <= 0:
@@ -84,7 +89,8 @@ class System:
         for i, v in enumerate(self.pot):
             s.pot[i] = v
         s.pot_cutoff, s.trap_omega = self.pot_cutoff, self.trap_omega
-        s._keep = self.kvec
+        s.grid = _ptr(self.grid) if self.grid is not None else None
+        s._keep = (self.kvec, self.grid)
         return s
 
 
@@ -101,6 +107,8 @@ def lib():
         L.tdo_displacement.argtypes = [_dp, _dp, C.c_int, C.c_double, _dp]
         L.tdo_bspline.restype = C.c_int
         L.tdo_bspline.argtypes = [C.c_double, C.c_double, C.c_int, _dp, _dp, _dp]
+        L.tdo_bspline_nu.restype = C.c_int
+        L.tdo_bspline_nu.argtypes = [C.c_double, _dp, C.c_int, _dp, _dp, _dp]
         L.tdo_pair_potential.restype = C.c_double
         L.tdo_pair_potential.argtypes = [C.POINTER(_CSystem), C.c_double, C.c_double]
         L.tdo_eval_batch.restype = C.c_int
@@ -189,6 +197,15 @@ def displacement(ri, rj, L):
 def bspline(r, h, P_s):
     b, db, d2b = np.zeros(4), np.zeros(4), np.zeros(4)
     j = lib().tdo_bspline(float(r), float(h), int(P_s), _ptr(b), _ptr(db), _ptr(d2b))
+    return j, b, db, d2b
+
+
+def bspline_nu(r, grid, P_s):
+    """The four non-zero cubic B-splines (values, d/dr, d2/dr2) at r on the
+    mirrored non-uniform knots of grid (tdo_bspline_nu); returns (j, b, db, d2b)."""
+    g = _f64(grid)
+    b, db, d2b = np.zeros(4), np.zeros(4), np.zeros(4)
+    j = lib().tdo_bspline_nu(float(r), _ptr(g), int(P_s), _ptr(b), _ptr(db), _ptr(d2b))
     return j, b, db, d2b
 
 

@@ -124,7 +124,7 @@ struct Setup {
 };
 
 Setup make_system(int n, int dim, double L, int nparam, const double* sysparams, int nsys, double time,
-                  int gr_bins = 50)
+                  int gr_bins = 50, const double* grid = nullptr)
 {
     N = n;
     DIM = dim;
@@ -138,6 +138,8 @@ Setup make_system(int n, int dim, double L, int nparam, const double* sysparams,
     s.dir = write_kvector_dir(dim);
     vector<double> p(sysparams, sysparams + nsys);
     s.sys = new PhysicalSystems::BosonsBulk(p, s.dir);
+    // USE_NURBS: the factory hands NURBS_GRID to SetNodes before InitSystem (src/TDVMC.cpp:2845-2850)
+    if (grid) s.sys->SetNodes(vector<double>(grid, grid + nparam));
     s.sys->SetPairDistributionBinCount(gr_bins);
     s.sys->InitSystem();
     s.sys->SetTime(time);
@@ -228,6 +230,39 @@ void* ref_new_bosonsbulk(int n, int dim, double L, int nparam, const double* sys
 {
     Setup s = make_system(n, dim, L, nparam, sysparams, nsys, 0.0, gr_bins);
     return static_cast<PhysicalSystems::IPhysicalSystem*>(s.sys);
+}
+
+// ref_bosonsbulk_eval on the non-uniform knots of grid [nparam] (USE_NURBS with
+// NURBS_GRID; BosonsBulk::SetNodes, BosonsBulk.cpp:35-44).
+int ref_bosonsbulk_eval_nurbs(int n, int dim, double L, int nparam, const double* grid, const double* sysparams,
+                              int nsys, double time, int64_t W, const double* R, const double* uR, const double* uI,
+                              double phiR, double phiI, double* expo, double* O, double* ER, double* EI,
+                              double* other, double* sD, double* sD2)
+{
+    Setup s = make_system(n, dim, L, nparam, sysparams, nsys, time, 50, grid);
+    vector<double> vR(uR, uR + nparam), vI(uI, uI + nparam);
+    const int S = nparam + 2;
+    for (int64_t w = 0; w < W; w++) {
+        auto Rw = to_R(R + w * n * dim);
+        s.sys->CalculateWavefunction(Rw, vR, vI, phiR, phiI);
+        s.sys->CalculateExpectationValues(Rw, vR, vI, phiR, phiI);
+        expo[w] = s.sys->GetExponent();
+        auto lo = s.sys->GetLocalOperators();
+        for (int k = 0; k < nparam; k++) O[w * nparam + k] = lo[k];
+        ER[w] = s.sys->GetLocalEnergyR();
+        EI[w] = s.sys->GetLocalEnergyI();
+        auto ov = s.sys->GetOtherExpectationValues();
+        for (int k = 0; k < 9; k++) other[w * 9 + k] = ov[k];
+        CSDataBulkSplines cs;
+        s.sys->FillCorrelatedSamplingData(&cs);
+        for (int sp = 0; sp < S; sp++)
+            for (int i = 0; i < n; i++) {
+                for (int a = 0; a < dim; a++) sD[((w * S + sp) * n + i) * dim + a] = cs.splineSumsD[sp][i][a];
+                sD2[(w * S + sp) * n + i] = cs.splineSumsD2[sp][i];
+            }
+    }
+    free_system(s);
+    return 0;
 }
 
 // Incremental move of one particle (CalculateWFQuotient, BosonsBulk.cpp:553-657):

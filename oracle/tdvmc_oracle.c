@@ -59,6 +59,8 @@ typedef struct {
     double pot[5];      /* square/Gaussian: a, b, t_quench, a', b';  LJ: sigma, eps */
     double pot_cutoff;  /* pair potential applies for r <= pot_cutoff */
     double trap_omega;  /* external 0.5*omega^2*|r|^2 per particle (0 = none) */
+    const double* grid; /* NULL: uniform knots; else the P_s knots of NURBS_GRID
+                           (0 = g_0 < ... < g_{P_s-1}, rc = g_{P_s-1}) */
 } tdo_system;
 
 /* ------------------------------------------------------------------ */
@@ -130,6 +132,76 @@ int tdo_bspline(double r, double h, int P_s, double* b, double* db, double* d2b)
         d2b[1] = (3.0 * t - 2.0) * ih2;
         d2b[2] = (-3.0 * t + 1.0) * ih2;
         d2b[3] = t * ih2;
+    }
+    return j;
+}
+
+/* Non-uniform knots (BosonsBulk::SetNodes, src/PhysicalSystems/BosonsBulk.cpp:35-44):
+ * the P_s grid values g, extended by three mirrored knots at each end,
+ *   n = (-g_3, -g_2, -g_1, g_0, ..., g_{P-1}, 2g_{P-1}-g_{P-2}, 2g_{P-1}-g_{P-3}, 2g_{P-1}-g_{P-4}),
+ * cubic B-spline s on n_s..n_{s+4}.  r lies in grid interval j (the knot
+ * interval (g_j, g_{j+1}] of lower_bound - 1, src/PhysicalSystems/BosonsBulk.cpp:197,
+ * clamped to 0..P-2), where splines s = j..j+3 are non-zero.  The values come
+ * from the Cox-de Boor recursion on that interval (C. de Boor, "A Practical
+ * Guide to Splines", ch. IX: B_{i,k} from B_{i,k-1} and B_{i+1,k-1}), the
+ * derivatives from B'_{i,3} = 3 (B_{i,2}/(n_{i+3}-n_i) - B_{i+1,2}/(n_{i+4}-n_{i+1}))
+ * and its analogue for B''.  b/db/d2b as in tdo_bspline.  Returns j. */
+static double tdo_ext_knot(const double* g, int P, int i)
+{
+    if (i < 3) return -g[3 - i];
+    if (i < P + 3) return g[i - 3];
+    return 2.0 * g[P - 1] - g[P - 1 - (i - (P + 2))];
+}
+
+int tdo_bspline_nu(double r, const double* g, int P_s, double* b, double* db, double* d2b)
+{
+    int k = 0;
+    while (k < P_s && g[k] < r) k++;          /* lower_bound */
+    int j = k - 1;
+    if (j < 0) j = 0;
+    if (j > P_s - 2) j = P_s - 2;
+    const int e = j + 3;                      /* extended interval n_e .. n_{e+1} */
+    double n[8];
+    for (int q = 0; q < 8; q++) n[q] = tdo_ext_knot(g, P_s, e - 3 + q);   /* n_{e-3} .. n_{e+4} */
+    /* B[o][m]: order-o spline index e - o + m (m = 0..o), non-zero on the interval */
+    double B[4][4] = {{0}};
+    B[0][0] = 1.0;
+    for (int o = 1; o <= 3; o++) {
+        for (int m = 0; m <= o; m++) {
+            const int i = e - o + m;          /* spline index, knots n_i .. n_{i+o+1} */
+            const int li = i - (e - 3);       /* local index into n[] */
+            double v = 0.0;
+            if (m > 0) v += (r - n[li]) / (n[li + o] - n[li]) * B[o - 1][m - 1];
+            if (m < o) v += (n[li + o + 1] - r) / (n[li + o + 1] - n[li + 1]) * B[o - 1][m];
+            B[o][m] = v;
+        }
+    }
+    for (int m = 0; m < 4; m++) b[m] = B[3][m];
+    if (db || d2b) {
+        /* first derivatives of the order-3 splines from the order-2 values, and
+         * the order-2 splines' derivatives from the order-1 values */
+        double d2[3] = {0, 0, 0};             /* B'_{i,2}, i = e-2 .. e */
+        for (int m = 0; m <= 2; m++) {
+            const int i = e - 2 + m, li = i - (e - 3);
+            double v = 0.0;
+            if (m > 0) v += 2.0 * B[1][m - 1] / (n[li + 2] - n[li]);
+            if (m < 2) v -= 2.0 * B[1][m] / (n[li + 3] - n[li + 1]);
+            d2[m] = v;
+        }
+        for (int m = 0; m < 4; m++) {
+            const int i = e - 3 + m, li = i - (e - 3);
+            double v1 = 0.0, v2 = 0.0;
+            if (m > 0) {
+                v1 += 3.0 * B[2][m - 1] / (n[li + 3] - n[li]);
+                v2 += 3.0 * d2[m - 1] / (n[li + 3] - n[li]);
+            }
+            if (m < 3) {
+                v1 -= 3.0 * B[2][m] / (n[li + 4] - n[li + 1]);
+                v2 -= 3.0 * d2[m] / (n[li + 4] - n[li + 1]);
+            }
+            if (db) db[m] = v1;
+            if (d2b) d2b[m] = v2;
+        }
     }
     return j;
 }
@@ -245,7 +317,7 @@ int tdo_eval(const tdo_system* s, const double* R, const double* uR, const doubl
             if (r <= s->pot_cutoff)
                 Vint += tdo_pair_potential(s, r, time);
             if (r <= s->rc) {
-                int j = tdo_bspline(r, h, Ps, b, db, d2b);
+                int j = s->grid ? tdo_bspline_nu(r, s->grid, Ps, b, db, d2b) : tdo_bspline(r, h, Ps, b, db, d2b);
                 double upR = 0, upI = 0, uppR = 0, uppI = 0;
                 for (int m = 0; m < 4; m++) {
                     B[j + m] += b[m];
@@ -421,7 +493,7 @@ static double tdo_log_psi_re(const tdo_system* s, const double* R, const double*
         for (int i = 0; i < n; i++) {
             double r = tdo_displacement(R + n * D, R + i * D, D, s->L, d);
             if (r <= s->rc) {
-                int j = tdo_bspline(r, h, Ps, b, NULL, NULL);
+                int j = s->grid ? tdo_bspline_nu(r, s->grid, Ps, b, NULL, NULL) : tdo_bspline(r, h, Ps, b, NULL, NULL);
                 for (int m = 0; m < 4; m++) sum += cR[j + m] * b[m];
             } else {
                 sum += uR[off_pair + Ps - 1];

@@ -23,6 +23,12 @@ struct KSys {
     double trap_omega;
     double pot_cutoff, pot_a, pot_b;   // pair potential resolved for the current time
     double step;           // Gaussian Metropolis step (MC_STEP)
+    // non-uniform knots (USE_NURBS): interval lookup by cells, pieces in
+    // s = r - g_j (KTables::nucell / nubeta); rc is then the last knot
+    int nu;                // 0 uniform knots, 1 NURBS_GRID
+    int ncell;             // lookup cells, the last one for every r beyond rc
+    double cinv;           // cells per unit length
+    double hlast;          // last knot gap g_{P_s-1} - g_{P_s-2}
 };
 
 // Per-precision device tables derived from the parameters (written by
@@ -34,7 +40,8 @@ struct KSys {
 // parameter slots.
 template <typename T>
 struct KTables {
-    const T* spl;     // [(P_s-1)][8]: cubic-in-t coefficients A0..A3 of Re u, then of Im u
+    const T* spl;     // [P_s][8]: cubic-in-t coefficients A0..A3 of Re u, then of Im u (non-uniform
+                      //   knots: cubics in s = r - g_j; row P_s-1: the constant beyond rc)
     const T* dspl;    // [P_s][kDsplStride]: derivatives per unit length on knot interval j,
                       //   u' = d1 + t (d2 + t d3), u'' = hinv (d2 + t e3), e3 = 2 d3, stored as
                       //   (d1, d2, d3, e3) of Re u, then of Im u; row P_s-1 holds u'(rc)
@@ -45,6 +52,9 @@ struct KTables {
     const int* kmap;  // [Kh][2]: parameter slots (relative to off_k) of k and of -k (-1: unpaired)
     const double* uR; // [P + 1] parameters (FP64 master copy), then phi R
     const double* uI; // [P + 1] parameters, then phi I
+    // non-uniform knots only (null otherwise); lengths in real units
+    const double* nucell;   // [ncell][4]: knot b, interval j, g_j, g_{j+1}: r <= b -> j, else j + 1
+    const double* nubeta;   // [P_s][4][4]: piece m of interval j as sum_p beta[j][m][p] s^p
 };
 
 constexpr int kMaxD = 3;
@@ -193,6 +203,8 @@ struct Coord<float, true> {
     }
     __device__ static double store(S p) { return (double)(int32_t)p * kUnit; }   // [-1/2, 1/2)
     __device__ static float diff(S a, S b) { return (float)(int32_t)(a - b); }
+    // the same minimum-image difference without rounding (32 significant bits)
+    __device__ static double diff_exact(S a, S b) { return (double)(int32_t)(a - b); }
     using Inc = uint32_t;
     __device__ static Inc inc(float dz)
     {

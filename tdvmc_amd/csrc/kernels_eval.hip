@@ -160,6 +160,15 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
     const double unit = C::kUnit / (PER ? sy.Linv : 1.0);   // length of one coordinate-difference unit
     const T hsc = (T)(sy.hinv * unit), rc = (T)(sy.rc / unit);
     const T hinvT = (T)sy.hinv;
+    // Cutoff test of the FP32 form in a periodic box.  O_{P-1} and O_{P-2}
+    // jump by 2/3 and 1/3 where a pair crosses rc, so a pair whose FP32 r^2
+    // (a few ulp off) lands on the other side of rc than the exact distance
+    // shifts O by a whole basis value.  Outside a band of relative width
+    // 2^-18 around rc^2 the FP32 r^2 decides; inside it, the exact integer
+    // coordinate differences do (in FP64, rounding 2^-53).
+    constexpr bool kExactCut = sizeof(T) == 4 && PER;
+    const double rc2x = (sy.rc / unit) * (sy.rc / unit);
+    const T rc2lo = (T)(rc2x * (1.0 - 0x1p-18)), rc2hi = (T)(rc2x * (1.0 + 0x1p-18));
     const double lapscD = (sy.D - 1) / unit;   // (D-1)/r = lapsc * rinv
     const T lapsc = (T)lapscD;
     PairConsts<T> pc{};
@@ -236,7 +245,25 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
                 T r, rinv;
                 Prec<T>::r_and_rinv(r2, &r, &rinv);
                 if constexpr (!PER && POT != 0) vs += pair_pot<T, POT>(pc, r, r2, rinv);
-                if (r <= rc) {
+                bool inside;
+                if constexpr (kExactCut) {
+                    inside = r2 <= rc2lo;
+                    const bool band = r2 <= rc2hi && !inside;
+                    if (__builtin_amdgcn_ballot_w64(band)) {   // rare: some lane near rc
+                        if (band) {
+                            double e2 = 0.0;
+#pragma unroll
+                            for (int a = 0; a < D; a++) {
+                                const double de = C::diff_exact(rn[a], cp[a]);
+                                e2 = fma(de, de, e2);
+                            }
+                            inside = e2 <= rc2x;
+                        }
+                    }
+                } else {
+                    inside = r <= rc;
+                }
+                if (inside) {
                     if constexpr (PER && POT != 0) vs += pair_pot<T, POT>(pc, r, r2, rinv);
                     // knot interval j in [0, P_s-1] (P_s-1 only at r = rc, t = 0)
                     const T x = r * hsc;

@@ -6,15 +6,17 @@
 // centred column-wise (two passes: column means in FP64, then centring while
 // packing), and S_aug = Xc^T Xc / W is a dense contraction over walkers:
 //   mode 0 (FP64):  v_mfma_f64_16x16x4 on FP64 operands;
-//   mode 1 (mixed): every centred entry is split into two bf16 terms
-//                   x = hi + lo (16 significant bits) and each product is
-//                   formed from the split terms on v_mfma_f32_32x32x16_bf16:
-//                   hi*hi + hi*lo + lo*hi (the "bf16x3" products) plus the
-//                   lo*lo term -- without it the diagonal of S carries a
-//                   one-signed bias of ~(2^-8)^2/3 relative, above the 1e-6
-//                   target; FP32 accumulation is folded into FP64 registers
-//                   every 256 walkers, and the walker slices are reduced in
-//                   FP64.
+//   mode 1 (mixed): every centred entry (rounded to FP32) is split into
+//                   three bf16 terms x = x0 + x1 + x2 (exact: 24 significant
+//                   bits) and each product is formed from the split terms on
+//                   v_mfma_f32_32x32x16_bf16: x0*y0 + x0*y1 + x1*y0 (the
+//                   "bf16x3" products) + x1*y1 + x0*y2 + x2*y0, leaving out
+//                   terms of relative size 2^-24 and below.  With two-term
+//                   splits (16 bits) the per-entry residual of ~2^-18 left
+//                   F = <O E> - <O><E> 1.4e-6 off FP64 on sampled N = 343
+//                   data, where O and E_L correlate weakly.  FP32
+//                   accumulation is folded into FP64 registers every 256
+//                   walkers, and the walker slices are reduced in FP64.
 // Only tiles on or above the diagonal are computed; F is column Pa-2 / Pa-1.
 
 #include "common.hpp"
@@ -140,8 +142,9 @@ __global__ void pack64_kernel(const TO* __restrict__ O, const double* __restrict
     }
 }
 
-// ---- pass 2b (mode 1): pack centred operands split into bf16 hi / lo as
+// ---- pass 2b (mode 1): pack centred operands split into three bf16 terms as
 //      [cb][s][part][h][l32][j], column c = 32 cb + l32, walker w = 16 s + 8 h + j
+constexpr int kSplit = 3;
 template <typename TO>
 __global__ void pack_bf16_kernel(const TO* __restrict__ O, const double* __restrict__ ER, const double* __restrict__ EI,
                                  int64_t W, int P, const double* __restrict__ mean, int64_t Wp, int Pp,
@@ -158,19 +161,23 @@ __global__ void pack_bf16_kernel(const TO* __restrict__ O, const double* __restr
         const int64_t s = rest % S16;
         const int cb = (int)(rest / S16);
         const int c = 32 * cb + l32;
-        __bf16* base = xt + ((((size_t)cb * S16 + s) * 2) * 64) * 8;
-        // 8 walkers of hi and of lo: one 16-byte store each
-        bf16x8 vh, vl;
+        __bf16* base = xt + ((((size_t)cb * S16 + s) * kSplit) * 64) * 8;
+        // 8 walkers of each term: one 16-byte store per term
+        bf16x8 v[kSplit];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int64_t w = 16 * s + 8 * h + j;
-            const float x = (float)xval(O, ER, EI, W, P, mean, w, c);
-            const __bf16 xh = (__bf16)x;
-            vh[j] = xh;
-            vl[j] = (__bf16)(x - (float)xh);
+            float x = (float)xval(O, ER, EI, W, P, mean, w, c);
+#pragma unroll
+            for (int q = 0; q < kSplit; q++) {
+                const __bf16 xq = (__bf16)x;
+                v[q][j] = xq;
+                x -= (float)xq;   // exact: the remainder has at most 16 significant bits
+            }
         }
-        *reinterpret_cast<bf16x8*>(base + ((size_t)h * 32 + l32) * 8) = vh;
-        *reinterpret_cast<bf16x8*>(base + ((size_t)64 + h * 32 + l32) * 8) = vl;
+#pragma unroll
+        for (int q = 0; q < kSplit; q++)
+            *reinterpret_cast<bf16x8*>(base + ((size_t)q * 64 + h * 32 + l32) * 8) = v[q];
     }
 }
 
@@ -225,7 +232,7 @@ __global__ __launch_bounds__(64) void gemm64_kernel(const double* __restrict__ x
     for (int v = 0; v < 4; v++) out[(4 * v + lane / 16) * 16 + (lane % 16)] = acc[v];
 }
 
-// ---- pass 3b (mode 1): split-bf16 MFMA (hi*hi + hi*lo + lo*hi + lo*lo)
+// ---- pass 3b (mode 1): split-bf16 MFMA, six products per k-step
 __global__ __launch_bounds__(64) void gemm_bf16x3_kernel(const __bf16* __restrict__ xt, int64_t ksteps, int nb,
                                                          int groups, double* __restrict__ partial)
 {
@@ -233,8 +240,8 @@ __global__ __launch_bounds__(64) void gemm_bf16x3_kernel(const __bf16* __restric
     int t, g, I, J;
     tile_group_of(blockIdx.x, ntiles, groups, &t, &g);
     tile_of(t, nb, &I, &J);
-    // per k-step block: 2 parts x 64 lanes x 8 bf16
-    const size_t step_elems = 2 * 64 * 8;
+    // per k-step block: kSplit parts x 64 lanes x 8 bf16
+    const size_t step_elems = kSplit * 64 * 8;
     const bf16x8* A = reinterpret_cast<const bf16x8*>(xt + (size_t)I * ksteps * step_elems);
     const bf16x8* B = reinterpret_cast<const bf16x8*>(xt + (size_t)J * ksteps * step_elems);
     // groups split the k-steps in whole folds of 16 steps (256 walkers)
@@ -249,12 +256,17 @@ __global__ __launch_bounds__(64) void gemm_bf16x3_kernel(const __bf16* __restric
 #pragma unroll
         for (int v = 0; v < 16; v++) acc[v] = 0.0f;
         for (int64_t s = f * 16; s < f * 16 + 16; s++) {
-            const bf16x8 ah = A[s * 128 + lane], al = A[s * 128 + 64 + lane];
-            const bf16x8 bh = B[s * 128 + lane], bl = B[s * 128 + 64 + lane];
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bl, acc, 0, 0, 0);
+            const bf16x8* a = A + s * (kSplit * 64) + lane;
+            const bf16x8* b = B + s * (kSplit * 64) + lane;
+            const bf16x8 a0 = a[0], a1 = a[64], a2 = a[128];
+            const bf16x8 b0 = b[0], b1 = b[64], b2 = b[128];
+            // smallest terms first
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b2, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, b0, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc, 0, 0, 0);
         }
 #pragma unroll
         for (int v = 0; v < 16; v++) acc64[v] += (double)acc[v];
@@ -298,7 +310,7 @@ void moments_work_bytes(int64_t W, int P, int mode, size_t* xt, size_t* partial,
     if (mode == 0)
         *xt = (size_t)g.Wp * g.Pp * sizeof(double);
     else
-        *xt = (size_t)g.Wp * g.Pp * 2 * sizeof(__bf16);
+        *xt = (size_t)g.Wp * g.Pp * kSplit * sizeof(__bf16);
     *partial = (size_t)g.groups * g.ntiles * g.tile * g.tile * sizeof(double);
     *colsum = (size_t)g.chunks * g.Pa * sizeof(double);
 }
