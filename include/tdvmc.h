@@ -74,6 +74,7 @@ typedef struct {
     int precondition;
     int K_terms;           /* k-space terms the kernels evaluate: one per +-k pair, plus unpaired k */
     int gr_bin_count;      /* GR_BIN_COUNT: g(r) bins requested */
+    int use_nurbs;         /* USE_NURBS: pair spline on the knots of NURBS_GRID (rc = last knot) */
 } tdvmc_info;
 
 /* Moments of one sampling stage; fields mirror SimulationStepData
@@ -145,6 +146,14 @@ int tdvmc_measure(tdvmc_ctx* ctx, int precision, tdvmc_step_data* out);
  * (Re, Im of grad ln psi), lap [W][N][2], other [W][9]. */
 int tdvmc_eval_configs(tdvmc_ctx* ctx, const double* R, int64_t W, int precision, double* expo, double* ER,
                        double* EI, double* O, double* grad, double* lap, double* other);
+
+/* Basis sums B_s = sum over pairs r_ij <= rc of B_s(r_ij), s = 0 .. P_pair + 1
+ * (the cubic B-splines on the knot vector, before the boundary-condition map
+ * that gives O), of W fed configurations R [W][N][DIM]; B [W][P_pair + 2]
+ * (host memory).  BosonsBulk::splineSums after CalculateLocalOperators
+ * (src/PhysicalSystems/BosonsBulk.cpp:179-218); tdvmc_splined_function.hpp
+ * exposes them as WFParts::SplinedFunction::splineSums. */
+int tdvmc_basis_sums(tdvmc_ctx* ctx, const double* R, int64_t W, int precision, double* B);
 
 /* Device-pointer variant of tdvmc_eval_configs (no host copies; enqueued on
  * `stream` (a hipStream_t) or the context stream when NULL; returns without

@@ -109,6 +109,8 @@ def lib():
         L.tdo_bspline.argtypes = [C.c_double, C.c_double, C.c_int, _dp, _dp, _dp]
         L.tdo_bspline_nu.restype = C.c_int
         L.tdo_bspline_nu.argtypes = [C.c_double, _dp, C.c_int, _dp, _dp, _dp]
+        L.tdo_basis_sums.restype = C.c_int
+        L.tdo_basis_sums.argtypes = [C.POINTER(_CSystem), _dp, _dp]
         L.tdo_pair_potential.restype = C.c_double
         L.tdo_pair_potential.argtypes = [C.POINTER(_CSystem), C.c_double, C.c_double]
         L.tdo_eval_batch.restype = C.c_int
@@ -164,6 +166,10 @@ def ref():
         L.ref_bosonsbulk_eval.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _dp, C.c_int, C.c_double,
                                           C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double,
                                           _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.ref_bosonsbulk_eval_nurbs.restype = C.c_int
+        L.ref_bosonsbulk_eval_nurbs.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp, C.c_int,
+                                                C.c_double, C.c_int64, _dp, _dp, _dp, C.c_double, C.c_double,
+                                                _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.ref_bosonsbulk_move.restype = C.c_double
         L.ref_bosonsbulk_move.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _dp, C.c_int, _dp, _dp,
                                           C.c_int, _dp, _dp]
@@ -207,6 +213,21 @@ def bspline_nu(r, grid, P_s):
     b, db, d2b = np.zeros(4), np.zeros(4), np.zeros(4)
     j = lib().tdo_bspline_nu(float(r), _ptr(g), int(P_s), _ptr(b), _ptr(db), _ptr(d2b))
     return j, b, db, d2b
+
+
+def basis_sums(sysd: System, R):
+    """B_s (s = 0 .. P_s + 1) of each configuration R [W][N][D] and the
+    number of pairs inside the cutoff (tdo_basis_sums)."""
+    R = _f64(R).reshape(-1, sysd.N, sysd.D)
+    B = np.zeros((R.shape[0], sysd.P_s + 2))
+    n_in = np.zeros(R.shape[0], dtype=np.int64)
+    cs = sysd.c()
+    for w in range(R.shape[0]):
+        Rw = np.ascontiguousarray(R[w])
+        Bw = np.zeros(sysd.P_s + 2)
+        n_in[w] = lib().tdo_basis_sums(C.byref(cs), _ptr(Rw), _ptr(Bw))
+        B[w] = Bw
+    return B, n_in
 
 
 def pair_potential(sysd: System, r, time=0.0):

@@ -13,6 +13,7 @@
 // plug-in can be constructed without the reference's data directory.
 
 #include "PhysicalSystems/BosonsBulk.h"
+#include "SplineFactory.h"
 #include "CSDataBulkSplines.h"
 #include "ConfigItem.h"
 #include "Utils.h"
@@ -362,6 +363,28 @@ int ref_bosonsbulk_observables(int n, int dim, double L, int nparam, const doubl
 }
 
 // The reference's file writers (src/Utils.cpp:818-1016) with OUT_DIR = dir.
+// SplineFactory::GetWeights (GetWeights3, src/SplineFactory.cpp:54-101) and the
+// zero-slope boundary-condition factors with uniform = true
+// (SetBoundaryConditions3_1D_OR_2 / _CO_2, :423-456, :563-596) on the knot
+// vector nodes [n]: w [n - 4][4][4], bcs [2][3], bce [2][3].
+int ref_spline_weights(const double* nodes, int n, double* w, double* bcs, double* bce)
+{
+    vector<double> nv(nodes, nodes + n);
+    auto W = SplineFactory::GetWeights(nv);
+    for (size_t s = 0; s < W.size(); s++)
+        for (int p = 0; p < 4; p++)
+            for (int c = 0; c < 4; c++) w[(s * 4 + p) * 4 + c] = W[s][p][c];
+    vector<vector<double> > a, b;
+    SplineFactory::SetBoundaryConditions3_1D_OR_2(nv, a, true);
+    SplineFactory::SetBoundaryConditions3_1D_CO_2(nv, b, nv[n - 4], true);
+    for (int i = 0; i < 2; i++)
+        for (int k = 0; k < 3; k++) {
+            bcs[i * 3 + k] = a[i][k];
+            bce[i * 3 + k] = b[i][k];
+        }
+    return (int)W.size();
+}
+
 int ref_write_vector(const char* dir, const char* name, const double* v, int n, const char* header)
 {
     OUT_DIR = dir;

@@ -1030,3 +1030,26 @@ int tdo_observables(int N, int D, double L, int P_s, double rc, int bins, int ns
     free(ss);
     return count;
 }
+
+/* Basis sums B_s = sum over pairs r_ij <= rc of B_s(r_ij), s = 0 .. P_s + 1,
+ * before the boundary-condition map (BosonsBulk::CalculateLocalOperators,
+ * src/PhysicalSystems/BosonsBulk.cpp:179-218: splineSums), for one
+ * configuration R [N][D].  B [P_s + 2].  Returns the number of pairs inside. */
+int tdo_basis_sums(const tdo_system* s, const double* R, double* B)
+{
+    const int N = s->N, D = s->D, Ps = s->P_s;
+    const double h = s->rc / (Ps - 1);
+    double b[4], d[3];
+    int inside = 0;
+    for (int k = 0; k < Ps + 2; k++) B[k] = 0.0;
+    for (int n = 0; n < N; n++)
+        for (int i = 0; i < n; i++) {
+            const double r = tdo_displacement(R + n * D, R + i * D, D, s->L, d);
+            if (r > s->rc) continue;
+            const int j = s->grid ? tdo_bspline_nu(r, s->grid, Ps, b, NULL, NULL) : tdo_bspline(r, h, Ps, b, NULL, NULL);
+            for (int m = 0; m < 4; m++)
+                if (j + m < Ps + 2) B[j + m] += b[m];
+            inside++;
+        }
+    return inside;
+}

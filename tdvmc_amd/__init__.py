@@ -48,6 +48,7 @@ class Info(C.Structure):
         ("precision", C.c_int), ("samples_per_walker", C.c_int), ("timestep", C.c_double),
         ("mc_step", C.c_double), ("therm_moves", C.c_int64), ("init_moves", C.c_int64), ("time", C.c_double),
         ("reg_eps", C.c_double), ("precondition", C.c_int), ("K_terms", C.c_int), ("gr_bin_count", C.c_int),
+        ("use_nurbs", C.c_int),
     ]
 
 
@@ -106,6 +107,7 @@ SYMBOLS = {
     "tdvmc_get_step_status": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "tdvmc_tdvp_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int]),
     "tdvmc_adjoint": (C.c_int, [C.c_void_p, _dp, C.c_int64, _dp, C.c_int, _dp, _dp]),
+    "tdvmc_basis_sums": (C.c_int, [C.c_void_p, _dp, C.c_int64, C.c_int, _dp]),
     "tdvmc_adjoint_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_void_p,
                                         C.c_void_p, C.c_void_p]),
     "tdvmc_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),

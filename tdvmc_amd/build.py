@@ -24,6 +24,7 @@ SOURCES = [
     "capi.cpp",
     "config.cpp",
     "writer.cpp",
+    "knots.cpp",
     "kernels_eval.hip",
     "kernels_sweep.hip",
     "kernels_moments.hip",
@@ -85,6 +86,25 @@ def build_cpp_check(verbose: bool = True) -> str:
         raise RuntimeError(f"C++ check build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
         print("built", CPP_CHECK_BIN)
+    # the non-uniform knot tables looked up as the kernels do (host only)
+    kc = os.path.join(os.path.dirname(CPP_CHECK_BIN), "knots_check")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", os.path.join(ROOT, "tests", "cpp", "knots_check.cpp"),
+           os.path.join(CSRC, "knots.cpp"), "-o", kc]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"knots check build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print("built", kc)
+    # the SplinedFunction-shaped view of include/tdvmc_splined_function.hpp
+    sc = os.path.join(os.path.dirname(CPP_CHECK_BIN), "splined_check")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "splined_check.cpp"), "-L" + HERE, "-ltdvmc",
+           "-Wl,-rpath,$ORIGIN/../../../tdvmc_amd", "-o", sc]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"SplinedFunction check build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print("built", sc)
     return CPP_CHECK_BIN
 
 

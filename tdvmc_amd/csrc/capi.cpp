@@ -17,6 +17,7 @@
 
 #include "config.hpp"
 #include "kernels.hpp"
+#include "knots.hpp"
 
 using namespace tdvmc;
 
@@ -55,6 +56,9 @@ struct DevTables {
 struct tdvmc_ctx {
     Config cfg;
     KSys sy{};
+    NuKnots knots;                    // non-uniform knots (USE_NURBS)
+    double* dNuCell = nullptr;        // device copies of knots.cells / knots.beta
+    double* dNuBeta = nullptr;
     std::vector<double> pot;          // SYSTEM_PARAMS
     int pot_type = 0;
     double time = 0.0;
@@ -265,10 +269,6 @@ int build_system(tdvmc_ctx* ctx)
         ctx->err = "N_PARAM (pair-spline parameters) must be >= 4";
         return TDVMC_ERR_BAD_CONFIG;
     }
-    if (c.USE_NURBS) {
-        ctx->err = "USE_NURBS: only uniform knots are implemented";
-        return TDVMC_ERR_BAD_CONFIG;
-    }
     if (c.USE_PARAM_START != 0 || c.USE_PARAM_END != 0) {
         ctx->err = "USE_PARAM_START/END: only the full parameter set is implemented";
         return TDVMC_ERR_BAD_CONFIG;
@@ -308,8 +308,33 @@ int build_system(tdvmc_ctx* ctx)
         ctx->err = "K_VECTORS must be in [0, 512]";
         return TDVMC_ERR_BAD_CONFIG;
     }
+    s.nu = 0;
+    s.ncell = 0;
+    s.cinv = 0.0;
+    if (c.USE_NURBS) {
+        // USE_NURBS with NURBS_GRID (BosonsBulk::SetNodes, src/TDVMC.cpp:2845-2850):
+        // the spline ends at the last knot (maxDistance, BosonsBulk.cpp:72)
+        if (!s.periodic) {
+            ctx->err = "USE_NURBS: non-uniform knots are implemented for the periodic bulk system";
+            return TDVMC_ERR_BAD_CONFIG;
+        }
+        if ((int)c.NURBS_GRID.size() != c.N_PARAM) {
+            ctx->err = "USE_NURBS: NURBS_GRID must hold N_PARAM knots";
+            return TDVMC_ERR_BAD_CONFIG;
+        }
+        std::string e;
+        if (!nu_build(c.NURBS_GRID, L / 2.0, &ctx->knots, &e)) {
+            ctx->err = "USE_NURBS: " + e;
+            return TDVMC_ERR_BAD_CONFIG;
+        }
+        s.nu = 1;
+        s.ncell = ctx->knots.ncell;
+        s.cinv = ctx->knots.cinv;
+        s.rc = c.NURBS_GRID.back();
+    }
     s.h = s.rc / (s.P_s - 1);
     s.hinv = 1.0 / s.h;
+    s.hlast = s.nu ? c.NURBS_GRID[s.P_s - 1] - c.NURBS_GRID[s.P_s - 2] : s.h;
     s.hbar2_2m = c.HBAR2_2M > 0 ? c.HBAR2_2M : 1.0;
     s.trap_omega = c.TRAP_OMEGA;
     std::string pp = c.PAIR_POTENTIAL;
@@ -465,6 +490,8 @@ KTables<T> tables_of(tdvmc_ctx* ctx, DevTables<T>& t)
     k.kmap = ctx->dKmap;
     k.uR = ctx->dUR;
     k.uI = ctx->dUI;
+    k.nucell = ctx->dNuCell;
+    k.nubeta = ctx->dNuBeta;
     return k;
 }
 
@@ -473,7 +500,7 @@ int ensure_tables(tdvmc_ctx* ctx, DevTables<T>& t, bool force)
 {
     if (!t.dirty && !force) return TDVMC_OK;
     HIPC(launch_prep_tables<T>(ctx->sy, ctx->dUR, ctx->dUI, ctx->dKmap, t.kv, t.spl, t.dspl, t.alpha, t.kag,
-                               ctx->stream));
+                               ctx->dNuBeta, ctx->stream));
     t.dirty = false;
     return TDVMC_OK;
 }
@@ -1120,13 +1147,21 @@ int alloc_ctx(tdvmc_ctx* ctx)
     HIPC(dalloc(&ctx->dKmap, ctx->kmap.size()));
     HIPC(hipMemcpyAsync(ctx->dKmap, ctx->kmap.data(), ctx->kmap.size() * sizeof(int), hipMemcpyHostToDevice,
                         ctx->stream));
-    HIPC(dalloc(&ctx->tf.spl, (size_t)(Ps - 1) * 8));
+    HIPC(dalloc(&ctx->tf.spl, (size_t)Ps * 8));
+    if (ctx->sy.nu) {
+        HIPC(dalloc(&ctx->dNuCell, ctx->knots.cells.size()));
+        HIPC(dalloc(&ctx->dNuBeta, ctx->knots.beta.size()));
+        HIPC(hipMemcpyAsync(ctx->dNuCell, ctx->knots.cells.data(), ctx->knots.cells.size() * sizeof(double),
+                            hipMemcpyHostToDevice, ctx->stream));
+        HIPC(hipMemcpyAsync(ctx->dNuBeta, ctx->knots.beta.data(), ctx->knots.beta.size() * sizeof(double),
+                            hipMemcpyHostToDevice, ctx->stream));
+    }
     HIPC(dalloc(&ctx->tf.dspl, (size_t)Ps * kDsplStride));
     HIPC(dalloc(&ctx->td.dspl, (size_t)Ps * kDsplStride));
     HIPC(dalloc(&ctx->tf.kv, 4 * (size_t)K));
     HIPC(dalloc(&ctx->tf.alpha, 2 * (size_t)K));
     HIPC(dalloc(&ctx->tf.kag, 8 * (size_t)K));
-    HIPC(dalloc(&ctx->td.spl, (size_t)(Ps - 1) * 8));
+    HIPC(dalloc(&ctx->td.spl, (size_t)Ps * 8));
     HIPC(dalloc(&ctx->td.kv, 4 * (size_t)K));
     HIPC(dalloc(&ctx->td.alpha, 2 * (size_t)K));
     HIPC(dalloc(&ctx->td.kag, 8 * (size_t)K));
@@ -1175,6 +1210,8 @@ void release(tdvmc_ctx* ctx)
     dfree(ctx->dXprev);
     for (auto& k : ctx->dK) dfree(k);
     dfree(ctx->tf.spl);
+    dfree(ctx->dNuCell);
+    dfree(ctx->dNuBeta);
     dfree(ctx->dKmap);
     dfree(ctx->tf.dspl);
     dfree(ctx->td.dspl);
@@ -1791,6 +1828,7 @@ int tdvmc_get_info(tdvmc_ctx* ctx, tdvmc_info* info)
     info->precondition = ctx->precondition;
     info->K_terms = ctx->sy.Kh;
     info->gr_bin_count = ctx->cfg.GR_BIN_COUNT;
+    info->use_nurbs = ctx->sy.nu;
     return TDVMC_OK;
 }
 
@@ -2062,19 +2100,23 @@ int tdvmc_eval_configs_dev(tdvmc_ctx* ctx, const double* R_dev, int64_t W, int p
     return TDVMC_OK;
 }
 
-int tdvmc_eval_configs(tdvmc_ctx* ctx, const double* R, int64_t W, int precision, double* expo, double* ER,
-                       double* EI, double* O, double* grad, double* lap, double* other)
+namespace {
+
+// Fed configurations from host memory: tdvmc_eval_configs, and (bsum != null)
+// the basis sums of tdvmc_basis_sums.
+int eval_configs_host(tdvmc_ctx* ctx, const double* R, int64_t W, int precision, double* expo, double* ER,
+                      double* EI, double* O, double* grad, double* lap, double* other, double* bsum)
 {
     if (!ctx) return TDVMC_ERR_BAD_ARG;
     CHECK_ARG(R != nullptr && W >= 1, "R is null or W < 1");
     int prec, rc = resolve_precision(ctx, precision, &prec);
     if (rc) return rc;
-    const int N = ctx->sy.N, D = ctx->sy.D, P = P_of(ctx);
+    const int N = ctx->sy.N, D = ctx->sy.D, P = P_of(ctx), S = ctx->sy.P_s + 2;
     const bool pp = grad != nullptr || lap != nullptr;
     double *dR = nullptr, *dEx = nullptr, *dER = nullptr, *dEI = nullptr, *dO = nullptr, *dG = nullptr,
-           *dL = nullptr, *dOt = nullptr;
+           *dL = nullptr, *dOt = nullptr, *dB = nullptr;
     auto cleanup = [&]() {
-        dfree(dR); dfree(dEx); dfree(dER); dfree(dEI); dfree(dO); dfree(dG); dfree(dL); dfree(dOt);
+        dfree(dR); dfree(dEx); dfree(dER); dfree(dEI); dfree(dO); dfree(dG); dfree(dL); dfree(dOt); dfree(dB);
     };
     auto hipc = [&](hipError_t e, const char* what) {
         if (e != hipSuccess) {
@@ -2087,6 +2129,7 @@ int tdvmc_eval_configs(tdvmc_ctx* ctx, const double* R, int64_t W, int precision
               hipc(dalloc(&dER, (size_t)W), "hipMalloc") && hipc(dalloc(&dEI, (size_t)W), "hipMalloc") &&
               hipc(dalloc(&dO, (size_t)W * P), "hipMalloc") && hipc(dalloc(&dOt, (size_t)W * 9), "hipMalloc");
     if (ok && pp) ok = hipc(dalloc(&dG, (size_t)W * N * D * 2), "hipMalloc") && hipc(dalloc(&dL, (size_t)W * N * 2), "hipMalloc");
+    if (ok && bsum) ok = hipc(dalloc(&dB, (size_t)W * S), "hipMalloc");
     if (ok) ok = hipc(hipMemcpyAsync(dR, R, (size_t)W * N * D * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "hipMemcpy");
     if (ok) {
         rc = prec == TDVMC_MIXED ? ensure_tables(ctx, ctx->tf, false) : ensure_tables(ctx, ctx->td, false);
@@ -2098,10 +2141,10 @@ int tdvmc_eval_configs(tdvmc_ctx* ctx, const double* R, int64_t W, int precision
         prof_begin(ctx, "eval", &ev);
         if (prec == TDVMC_MIXED)
             ok = hipc((launch_eval<float, double>(ctx->sy, tables_of(ctx, ctx->tf), dR, W, dEx, dER, dEI, dO, dOt, dG,
-                                                  dL, nullptr, nullptr, ctx->stream)), "eval kernel");
+                                                  dL, nullptr, nullptr, ctx->stream, dB)), "eval kernel");
         else
             ok = hipc((launch_eval<double, double>(ctx->sy, tables_of(ctx, ctx->td), dR, W, dEx, dER, dEI, dO, dOt,
-                                                   dG, dL, nullptr, nullptr, ctx->stream)), "eval kernel");
+                                                   dG, dL, nullptr, nullptr, ctx->stream, dB)), "eval kernel");
         prof_end(ctx, "eval", ev);
     }
     auto down = [&](double* h, double* d, size_t n) {
@@ -2116,10 +2159,26 @@ int tdvmc_eval_configs(tdvmc_ctx* ctx, const double* R, int64_t W, int precision
         down(grad, dG, (size_t)W * N * D * 2);
         down(lap, dL, (size_t)W * N * 2);
     }
+    down(bsum, dB, (size_t)W * S);
     if (ok) ok = hipc(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     cleanup();
     if (!ok) return TDVMC_ERR_HIP;
     return prof_flush(ctx);
+}
+
+}  // namespace
+
+int tdvmc_eval_configs(tdvmc_ctx* ctx, const double* R, int64_t W, int precision, double* expo, double* ER,
+                       double* EI, double* O, double* grad, double* lap, double* other)
+{
+    return eval_configs_host(ctx, R, W, precision, expo, ER, EI, O, grad, lap, other, nullptr);
+}
+
+int tdvmc_basis_sums(tdvmc_ctx* ctx, const double* R, int64_t W, int precision, double* B)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(B != nullptr, "tdvmc_basis_sums: B is null");
+    return eval_configs_host(ctx, R, W, precision, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, B);
 }
 
 int tdvmc_observables_info(tdvmc_ctx* ctx, int* n_gr, int* n_sk, double* gr_grid, double* sk_grid)
@@ -2712,6 +2771,7 @@ int tdvmc_batch_create(const char* const* configs, int n_systems, tdvmc_batch** 
             return fail(TDVMC_ERR_BAD_CONFIG, "USE_NORMALIZE_WF, USE_ADJUST_PARAMETERS and the parameter acceptance "
                                               "check are not available for batches");
         if (c->spw != 1) return fail(TDVMC_ERR_BAD_CONFIG, "batches take one sample per walker and stage");
+        if (c->sy.nu) return fail(TDVMC_ERR_BAD_CONFIG, "USE_NURBS (non-uniform knots) is not available for batches");
         // the batched solve sizes its LDS for the Cholesky panel of the largest system
         if (c->sy.P > solve_max_params(0))
             return fail(TDVMC_ERR_BAD_CONFIG, "system " + std::to_string(s) + ": P = " + std::to_string(c->sy.P) +

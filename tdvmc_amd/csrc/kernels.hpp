@@ -11,7 +11,7 @@ namespace tdvmc {
 // ---- parameter-derived tables (kernels_misc.hip)
 template <typename T>
 hipError_t launch_prep_tables(const KSys& sy, const double* uR, const double* uI, const int* kmap, const T* kv, T* spl,
-                              T* dspl, T* alpha, T* kag, hipStream_t st);
+                              T* dspl, T* alpha, T* kag, const double* nubeta, hipStream_t st);
 
 // ---- evaluation (kernels_eval.hip); grad/lap non-null selects the per-particle variant
 // LDS of one walker with wpw wavefronts (single contexts: 4)
@@ -22,7 +22,7 @@ size_t eval_lds_bytes(const KSys& sy, bool per_particle, int wpw = 4);
 template <typename T, typename TO>
 hipError_t launch_eval(const KSys& sy, const KTables<T>& tb, const double* R, int64_t W, double* expo, double* ER,
                        double* EI, TO* O, double* other, double* grad, double* lap, double* outer,
-                       const T* rho_in, hipStream_t st);
+                       const T* rho_in, hipStream_t st, double* bsum = nullptr);
 
 // ---- Metropolis sweeps (kernels_sweep.hip)
 struct SweepArgs {

@@ -52,6 +52,7 @@ struct EvalOut {
     double* grad;     // [W][N][D][2] or null (PP)
     double* lap;      // [W][N][2] or null (PP)
     double* outer;    // [W] count of pairs beyond the cutoff, or null
+    double* bsum = nullptr;   // [W][P_s + 2] basis sums B_s before the BC map, or null
     __device__ __forceinline__ void O_store(int64_t w, int P, int k, double v) const { O[w * P + k] = (TO)v; }
 };
 
@@ -148,7 +149,7 @@ __device__ __forceinline__ T pair_pot(const PairConsts<T>& pc, T r, T r2, T rinv
 // here: their number is N (N - 1) / 2 minus the histogram count.  CPLX false:
 // every imaginary parameter is zero, so the imaginary parts (exact zeros) are
 // not computed.
-template <typename T, int D, bool PP, bool PER, int POT, bool CPLX, int NW>
+template <typename T, int D, bool PP, bool PER, int POT, bool CPLX, int NW, bool NU>
 __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<T, PER>::S* __restrict__ pos,
                                            const T* __restrict__ dsp, double* __restrict__ hist,
                                            double* __restrict__ gsum, double* __restrict__ lsum, int wave,
@@ -159,7 +160,15 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
     const int N = sy.N;
     const double unit = C::kUnit / (PER ? sy.Linv : 1.0);   // length of one coordinate-difference unit
     const T hsc = (T)(sy.hinv * unit), rc = (T)(sy.rc / unit);
-    const T hinvT = (T)sy.hinv;
+    // u'' = hl (d2 + t e3): t in knot units (uniform knots), s in real length
+    // with the derivatives taken in s (non-uniform knots)
+    const double hl = NU ? 1.0 : sy.hinv;
+    const T hinvT = (T)hl;
+    // non-uniform knots: lookup cells after the derivative table (coordinate-
+    // difference units; see knots.hpp)
+    const T* ncel = dsp + (size_t)sy.P_s * kDsplStride;
+    const T cin = (T)(sy.cinv * unit), unitR = (T)unit;
+    const int cmax = sy.ncell - 1;
     // Cutoff test of the FP32 form in a periodic box.  O_{P-1} and O_{P-2}
     // jump by 2/3 and 1/3 where a pair crosses rc, so a pair whose FP32 r^2
     // (a few ulp off) lands on the other side of rc than the exact distance
@@ -266,9 +275,20 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
                 if (inside) {
                     if constexpr (PER && POT != 0) vs += pair_pot<T, POT>(pc, r, r2, rinv);
                     // knot interval j in [0, P_s-1] (P_s-1 only at r = rc, t = 0)
-                    const T x = r * hsc;
-                    const int j = (int)x;
-                    const T t = Prec<T>::fract_(x);
+                    int j;
+                    T t;
+                    if constexpr (NU) {
+                        int c = (int)(r * cin);
+                        c = c < cmax ? c : cmax;
+                        const T* e = ncel + 4 * c;
+                        const bool right = r > e[0];
+                        j = (int)e[1] + (right ? 1 : 0);
+                        t = (r - (right ? e[3] : e[2])) * unitR;
+                    } else {
+                        const T x = r * hsc;
+                        j = (int)x;
+                        t = Prec<T>::fract_(x);
+                    }
                     // moments 1, t, t^2, t^3 of the pair's position in knot
                     // interval j (FP64); the basis sums B_s are rebuilt from
                     // them at the end
@@ -377,8 +397,8 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
                     if (CPLX) lds_add(lsum + N + ra, (double)lI);
                 } else {
                     // every pair term enters the Laplacians of both particles
-                    lap_tot_R += 2.0 * (sy.hinv * (double)lR + lapscD * (double)mR);
-                    if (CPLX) lap_tot_I += 2.0 * (sy.hinv * (double)lI + lapscD * (double)mI);
+                    lap_tot_R += 2.0 * (hl * (double)lR + lapscD * (double)mR);
+                    if (CPLX) lap_tot_I += 2.0 * (hl * (double)lI + lapscD * (double)mI);
                 }
             }
         }
@@ -461,11 +481,23 @@ __device__ __forceinline__ void eval_forces(const KSys& sy, const KTables<T>& tb
                                             double* gsum, double* lsum, int wave, int lane, double& vpot,
                                             double& lap_tot_R, double& lap_tot_I)
 {
+    if constexpr (PER) {
+        if (sy.nu) {
+            // non-uniform knots (periodic box only)
+            switch (sy.pot_type) {
+            case 1: eval_pairs<T, D, PP, PER, 1, CPLX, NW, true>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
+            case 2: eval_pairs<T, D, PP, PER, 2, CPLX, NW, true>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
+            case 3: eval_pairs<T, D, PP, PER, 3, CPLX, NW, true>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
+            default: eval_pairs<T, D, PP, PER, 0, CPLX, NW, true>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
+            }
+            return;
+        }
+    }
     switch (sy.pot_type) {
-    case 1: eval_pairs<T, D, PP, PER, 1, CPLX, NW>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
-    case 2: eval_pairs<T, D, PP, PER, 2, CPLX, NW>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
-    case 3: eval_pairs<T, D, PP, PER, 3, CPLX, NW>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
-    default: eval_pairs<T, D, PP, PER, 0, CPLX, NW>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
+    case 1: eval_pairs<T, D, PP, PER, 1, CPLX, NW, false>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
+    case 2: eval_pairs<T, D, PP, PER, 2, CPLX, NW, false>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
+    case 3: eval_pairs<T, D, PP, PER, 3, CPLX, NW, false>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
+    default: eval_pairs<T, D, PP, PER, 0, CPLX, NW, false>(sy, pos, dsp, hist, gsum, lsum, wave, lane, vpot, lap_tot_R, lap_tot_I); break;
     }
 }
 
@@ -530,6 +562,18 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     // hc is even and hist 16-byte aligned: cleared two doubles per store
     for (int i = tid; i < Ps * 2 * hc; i += NT) reinterpret_cast<double2*>(hist)[i] = make_double2(0.0, 0.0);
     for (int i = tid; i < Ps * kDsplStride; i += NT) dsp[i] = tb.dspl[i];
+    if (PER && sy.nu) {
+        // lookup cells in coordinate-difference units (after the derivative table)
+        const double ul = C::kUnit / scale;
+        T* ncel = dsp + (size_t)Ps * kDsplStride;
+        for (int i = tid; i < sy.ncell; i += NT) {
+            const double* e = tb.nucell + 4 * i;
+            ncel[4 * i] = (T)(e[0] / ul);
+            ncel[4 * i + 1] = (T)e[1];
+            ncel[4 * i + 2] = (T)(e[2] / ul);
+            ncel[4 * i + 3] = (T)(e[3] / ul);
+        }
+    }
     for (int k = tid; k < K; k += NT) {
         // periodic: kv L is the integer lattice vector n (rounded to remove
         // the FP32 storage error of n / L)
@@ -666,21 +710,28 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     // sum_p beta[m][p] t^p, so B_s = sum_m sum_p beta[m][p] M[s-m][p] with the
     // interval moments M[j][p] = sum over pairs in j of t^p.
     double ex_p = 0.0;
-    for (int k = tid; k < Ps; k += NT) {
-        constexpr double beta[4][4] = {{1.0 / 6, -3.0 / 6, 3.0 / 6, -1.0 / 6},
-                                       {4.0 / 6, 0.0, -6.0 / 6, 3.0 / 6},
-                                       {1.0 / 6, 3.0 / 6, 3.0 / 6, -3.0 / 6},
-                                       {0.0, 0.0, 0.0, 1.0 / 6}};
-        auto B = [&](int s) {
+    constexpr double beta[4][4] = {{1.0 / 6, -3.0 / 6, 3.0 / 6, -1.0 / 6},
+                                   {4.0 / 6, 0.0, -6.0 / 6, 3.0 / 6},
+                                   {1.0 / 6, 3.0 / 6, 3.0 / 6, -3.0 / 6},
+                                   {0.0, 0.0, 0.0, 1.0 / 6}};
+    auto B = [&](int s) {
             double v = 0.0;
             for (int m = 0; m < 4; m++) {
                 const int j = s - m;
                 if (j < 0 || j > Ps - 1) continue;
+                if (sy.nu) {
+                    // pieces of interval j in s (non-uniform knots)
+                    const double* bj = tb.nubeta + (j * 4 + m) * 4;
 #pragma unroll
-                for (int p = 0; p < 4; p++) v += beta[m][p] * hist[(j * 4 + p) * hc];
+                    for (int p = 0; p < 4; p++) v += bj[p] * hist[(j * 4 + p) * hc];
+                } else {
+#pragma unroll
+                    for (int p = 0; p < 4; p++) v += beta[m][p] * hist[(j * 4 + p) * hc];
+                }
             }
             return v;
-        };
+    };
+    for (int k = tid; k < Ps; k += NT) {
         double o;
         if (k == 0) o = B(1);
         else if (k == 1) o = B(0) + B(2);
@@ -690,6 +741,8 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
         ex_p += tb.uR[sy.off_pair + k] * o;
         if (active) out.O_store(w, sy.P, sy.off_pair + k, o);
     }
+    if (out.bsum && active)
+        for (int s = tid; s < Ps + 2; s += NT) out.bsum[w * (Ps + 2) + s] = B(s);
 
     double sums[9] = {sR1, sI1, sR1I1, sR2, sI2, sr2, vext, vpot, ex_k + ex_p};
     walker_sum<WPW>(sums, red, wave, lane);
@@ -789,7 +842,9 @@ size_t eval_lds_bytes(const KSys& sy, bool per_particle, int wpw)
     const size_t nt = (size_t)sy.N * sy.D + 2 * (size_t)sy.Kh;
     // the k-vector table and the derivative table start 16-byte aligned
     const size_t head = (kRedDoubles * sizeof(double) + nd * sizeof(double) + nt * sizeof(T) + 15) & ~(size_t)15;
-    return head + (4 * (size_t)sy.Kh + (size_t)sy.P_s * kDsplStride) * sizeof(T);
+    // non-uniform knots: the lookup cells follow the derivative table
+    const size_t cells = sy.nu ? 4 * (size_t)sy.ncell : 0;
+    return head + (4 * (size_t)sy.Kh + (size_t)sy.P_s * kDsplStride + cells) * sizeof(T);
 }
 
 template <typename T, int D, bool PP, bool PER, typename TO>
@@ -820,10 +875,10 @@ static hipError_t launch_eval_d(const KSys& sy, const KTables<T>& tb, const doub
 template <typename T, typename TO>
 hipError_t launch_eval(const KSys& sy, const KTables<T>& tb, const double* R, int64_t W, double* expo, double* ER,
                        double* EI, TO* O, double* other, double* grad, double* lap, double* outer,
-                       const T* rho_in, hipStream_t st)
+                       const T* rho_in, hipStream_t st, double* bsum)
 {
     if (W <= 0) return hipSuccess;
-    EvalOut<TO> o{expo, ER, EI, O, other, grad, lap, outer};
+    EvalOut<TO> o{expo, ER, EI, O, other, grad, lap, outer, bsum};
     const bool pp = grad != nullptr;
     switch (sy.D) {
     case 1: return launch_eval_d<T, 1, TO>(sy, tb, R, W, o, rho_in, pp, st);
@@ -866,12 +921,12 @@ template size_t eval_lds_bytes<float>(const KSys&, bool, int);
 template size_t eval_lds_bytes<double>(const KSys&, bool, int);
 template hipError_t launch_eval<double, double>(const KSys&, const KTables<double>&, const double*, int64_t, double*,
                                                 double*, double*, double*, double*, double*, double*, double*,
-                                                const double*, hipStream_t);
+                                                const double*, hipStream_t, double*);
 template hipError_t launch_eval<float, float>(const KSys&, const KTables<float>&, const double*, int64_t, double*,
                                               double*, double*, float*, double*, double*, double*, double*,
-                                              const float*, hipStream_t);
+                                              const float*, hipStream_t, double*);
 template hipError_t launch_eval<float, double>(const KSys&, const KTables<float>&, const double*, int64_t, double*,
                                                double*, double*, double*, double*, double*, double*, double*,
-                                               const float*, hipStream_t);
+                                               const float*, hipStream_t, double*);
 
 }  // namespace tdvmc

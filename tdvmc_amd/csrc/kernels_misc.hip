@@ -20,14 +20,56 @@ __device__ double spline_coeff(const double* u, int Ps, int s)
     return u[Ps - 1];   // s == Ps
 }
 
+// Non-uniform knots: piece m of interval j is sum_p beta[j][m][p] s^p in
+// s = r - g_j, so u = sum_p A_p s^p with A_p = sum_m c_{j+m} beta[j][m][p]
+// (c: the extended coefficients of spline_coeff); the derivatives are taken
+// in s directly (u' = A1 + 2 A2 s + 3 A3 s^2, u'' = 2 A2 + 6 A3 s).  Row P_s-1
+// of spl holds the constant beyond rc, row P_s-1 of dspl u'(rc), u''(rc).
+template <typename T>
+__device__ void prep_nu_interval(const KSys& sy, const double* uR, const double* uI, const double* beta, T* spl,
+                                 T* dspl, int j)
+{
+    const int Ps = sy.P_s;
+    for (int part = 0; part < 2; part++) {
+        const double* u = (part == 0 ? uR : uI) + sy.off_pair;
+        double A[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int m = 0; m < 4; m++) {
+            const double c = spline_coeff(u, Ps, j + m);
+            for (int p = 0; p < 4; p++) A[p] += c * beta[(j * 4 + m) * 4 + p];
+        }
+        for (int p = 0; p < 4; p++) spl[j * 8 + part * 4 + p] = (T)A[p];
+        T* dr = dspl + j * kDsplStride + part * 4;
+        dr[0] = (T)A[1];
+        dr[1] = (T)(2.0 * A[2]);
+        dr[2] = (T)(3.0 * A[3]);
+        dr[3] = (T)(6.0 * A[3]);
+        if (j == Ps - 2) {
+            // r = rc: the last interval at its end, s = g_{P-1} - g_{P-2}; beyond
+            // rc the outer-pair value
+            T* de = dspl + (Ps - 1) * kDsplStride + part * 4;
+            const double hl = sy.hlast;
+            de[0] = (T)(A[1] + hl * (2.0 * A[2] + hl * 3.0 * A[3]));
+            de[1] = (T)(2.0 * A[2] + 6.0 * A[3] * hl);
+            de[2] = (T)0;
+            de[3] = (T)0;
+            T* so = spl + (Ps - 1) * 8 + part * 4;
+            so[0] = (T)u[Ps - 1];
+            so[1] = so[2] = so[3] = (T)0;
+        }
+    }
+}
+
 template <typename T>
 __device__ void prep_tables_body(const KSys& sy, const double* __restrict__ uR, const double* __restrict__ uI,
                                  const int* __restrict__ kmap, const T* __restrict__ kv, T* __restrict__ spl,
-                                 T* __restrict__ dspl, T* __restrict__ alpha, T* __restrict__ kag, int j0, int jstep)
+                                 T* __restrict__ dspl, T* __restrict__ alpha, T* __restrict__ kag,
+                                 const double* __restrict__ nubeta, int j0, int jstep)
 {
     const int Ps = sy.P_s;
     for (int j = j0; j < Ps - 1 + sy.Kh; j += jstep) {
-        if (j < Ps - 1) {
+        if (j < Ps - 1 && sy.nu) {
+            prep_nu_interval(sy, uR, uI, nubeta, spl, dspl, j);
+        } else if (j < Ps - 1) {
             for (int part = 0; part < 2; part++) {
                 const double* u = (part == 0 ? uR : uI) + sy.off_pair;
                 const double c0 = spline_coeff(u, Ps, j), c1 = spline_coeff(u, Ps, j + 1);
@@ -80,9 +122,10 @@ __device__ void prep_tables_body(const KSys& sy, const double* __restrict__ uR, 
 template <typename T>
 __global__ void prep_tables_kernel(KSys sy, const double* __restrict__ uR, const double* __restrict__ uI,
                                    const int* __restrict__ kmap, const T* __restrict__ kv, T* __restrict__ spl,
-                                   T* __restrict__ dspl, T* __restrict__ alpha, T* __restrict__ kag)
+                                   T* __restrict__ dspl, T* __restrict__ alpha, T* __restrict__ kag,
+                                   const double* __restrict__ nubeta)
 {
-    prep_tables_body(sy, uR, uI, kmap, kv, spl, dspl, alpha, kag, threadIdx.x + blockIdx.x * blockDim.x,
+    prep_tables_body(sy, uR, uI, kmap, kv, spl, dspl, alpha, kag, nubeta, threadIdx.x + blockIdx.x * blockDim.x,
                      blockDim.x * gridDim.x);
 }
 
@@ -91,7 +134,8 @@ template <typename T>
 __global__ void prep_tables_batch_kernel(const BatchParams<T>* __restrict__ bp)
 {
     const BatchParams<T> b = bp[blockIdx.x];
-    prep_tables_body(b.sy, b.uR, b.uI, b.kmap, b.kv, b.spl, b.dspl, b.alpha, b.kag, threadIdx.x, blockDim.x);
+    prep_tables_body(b.sy, b.uR, b.uI, b.kmap, b.kv, b.spl, b.dspl, b.alpha, b.kag, nullptr, threadIdx.x,
+                     blockDim.x);
 }
 
 // x (concatenated packed parameters) -> per-system uR, uI
@@ -325,18 +369,19 @@ hipError_t launch_adjoint_outer(const double* outer, const double* gbar, int64_t
 
 template <typename T>
 hipError_t launch_prep_tables(const KSys& sy, const double* uR, const double* uI, const int* kmap, const T* kv, T* spl,
-                              T* dspl, T* alpha, T* kag, hipStream_t st)
+                              T* dspl, T* alpha, T* kag, const double* nubeta, hipStream_t st)
 {
+    if (sy.nu && !nubeta) return hipErrorInvalidValue;
     const int n = sy.P_s - 1 + sy.Kh;
     const int blocks = (n + 255) / 256;
     hipLaunchKernelGGL(prep_tables_kernel<T>, dim3(blocks), dim3(256), 0, st, sy, uR, uI, kmap, kv, spl, dspl, alpha,
-                       kag);
+                       kag, nubeta);
     return hipGetLastError();
 }
 template hipError_t launch_prep_tables<float>(const KSys&, const double*, const double*, const int*, const float*,
-                                              float*, float*, float*, float*, hipStream_t);
+                                              float*, float*, float*, float*, const double*, hipStream_t);
 template hipError_t launch_prep_tables<double>(const KSys&, const double*, const double*, const int*, const double*,
-                                               double*, double*, double*, double*, hipStream_t);
+                                               double*, double*, double*, double*, const double*, hipStream_t);
 
 hipError_t launch_axpy_params(int P, const double* x0, const double* xdot, double c, double* y, hipStream_t st)
 {

@@ -116,9 +116,29 @@ __device__ __forceinline__ T pair_u(T r, T rc, T hinv, int Ps, const Vec4<T>* sp
     return r <= rc ? u : u_out;
 }
 
-// Intervals of the (extended) spline table in LDS.
+// u(r) on non-uniform knots (periodic box): cell c = min((int)(r cin), ncell-1)
+// gives the interval (j, or j + 1 past the cell's knot b) and its left knot;
+// the piece is a cubic in s = (r - g) unitR (real length).  Row P_s - 1 of
+// spl is the constant outer-pair value beyond rc.
+template <typename T>
+__device__ __forceinline__ T pair_u_nu(T r, T cin, int cmax, T unitR, const Vec4<T>* cells, const Vec4<T>* spl)
+{
+    int c = (int)(r * cin);
+    c = c < cmax ? c : cmax;
+    const Vec4<T> e = cells[c];
+    const bool right = r > e.x;
+    const int j = (int)e.y + (right ? 1 : 0);
+    const T s = (r - (right ? e.w : e.z)) * unitR;
+    const Vec4<T> cf = spl[j];
+    return cf.x + s * (cf.y + s * (cf.z + s * cf.w));
+}
+
+// Rows of the spline table in LDS (uniform knots: intervals, extended in a
+// periodic box to the largest minimum-image distance; non-uniform knots: P_s
+// interval rows, then the lookup cells).
 __host__ __device__ inline int sweep_table_len(const KSys& sy)
 {
+    if (sy.nu) return sy.P_s + sy.ncell;
     if (!sy.periodic) return sy.P_s - 1;
     // largest minimum-image distance sqrt(D)/2 L, in knot intervals, + margin
     const double xmax = 0.5 * sqrt((double)sy.D) * sy.L * sy.hinv;
@@ -128,7 +148,7 @@ __host__ __device__ inline int sweep_table_len(const KSys& sy)
 
 // One workgroup's share of the sweep: walkers blk * WPB .. (blk + 1) * WPB - 1
 // of the system described by (sy, tb, a).
-template <typename T, int D, int LPW, int KPL, bool PER>
+template <typename T, int D, int LPW, int KPL, bool PER, bool NU = false>
 __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb, const SweepArgs& a, int64_t blk)
 {
     using C = Coord<T, PER>;
@@ -149,7 +169,27 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
     S* pos = posb + (size_t)grp * N * D;
 
     const T u_out = (T)tb.uR[sy.outer_idx];
-    for (int i = tid; i < NT; i += kThreads) {
+    // real length per coordinate-difference unit
+    const double scale = PER ? sy.Linv : 1.0;
+    const double unit_len = C::kUnit / scale;
+    if constexpr (NU) {
+        // P_s rows (the last one the outer constant), then the cells in
+        // coordinate-difference units
+        for (int i = tid; i < Ps; i += kThreads) {
+            spl[i].x = tb.spl[i * 8 + 0];
+            spl[i].y = tb.spl[i * 8 + 1];
+            spl[i].z = tb.spl[i * 8 + 2];
+            spl[i].w = tb.spl[i * 8 + 3];
+        }
+        for (int i = tid; i < sy.ncell; i += kThreads) {
+            const double* e = tb.nucell + 4 * i;
+            spl[Ps + i].x = (T)(e[0] / unit_len);
+            spl[Ps + i].y = (T)e[1];
+            spl[Ps + i].z = (T)(e[2] / unit_len);
+            spl[Ps + i].w = (T)(e[3] / unit_len);
+        }
+    }
+    for (int i = NU ? NT : tid; i < NT; i += kThreads) {
         if (i < Ps - 1) {
             spl[i].x = tb.spl[i * 8 + 0];
             spl[i].y = tb.spl[i * 8 + 1];
@@ -169,7 +209,6 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
     }
     // lengths in box units when periodic (scale = 1/L), unscaled otherwise;
     // diff() and raw() return multiples of C::kUnit of that unit
-    const double scale = PER ? sy.Linv : 1.0;
     const bool active = w < a.W;
     double* Rw = a.R + (active ? w : 0) * (size_t)N * D;
     if (active) {
@@ -179,6 +218,14 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
     if (!active) return;
 
     const T hinv = (T)(sy.hinv / scale * C::kUnit), rc = (T)(sy.rc * scale / C::kUnit);
+    const T cin = (T)(sy.cinv * unit_len), unitR = (T)unit_len;
+    const Vec4<T>* cells = spl + Ps;
+    auto uof = [&](T r) -> T {
+        if constexpr (NU)
+            return pair_u_nu<T>(r, cin, sy.ncell - 1, unitR, cells, spl);
+        else
+            return pair_u<T, PER>(r, rc, hinv, Ps, spl, u_out);
+    };
     const T step = (T)(sy.step * scale);
     // one-body factor on raw coordinates: -ob sum raw^2
     const T ob = sy.one_body ? (T)(tb.uR[0] / (scale * scale) * C::kUnit * C::kUnit) : (T)0;
@@ -216,7 +263,7 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
         }
     }
     // u at distance 0: the moved particle meets itself in the partner loop
-    const T u_zero = pair_u<T, PER>((T)0, rc, hinv, Ps, spl, u_out);
+    const T u_zero = uof((T)0);
 
     const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
     const uint32_t ep = a.epoch ? *a.epoch : 0u;
@@ -267,7 +314,7 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
                 const T dm = C::diff((S)0, C::add((S)0, dl[c]));
                 s2 += dm * dm;
             }
-            uself = pair_u<T, PER>(Prec<T>::sqrt_(s2), rc, hinv, Ps, spl, u_out) - u_zero;
+            uself = uof(Prec<T>::sqrt_(s2)) - u_zero;
         }
         const int nb = (int)((ntr - base) < LPW ? (ntr - base) : LPW);
         for (int tt = 0; tt < nb; tt++) {
@@ -291,8 +338,8 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
                     ro2 += dOld * dOld;
                     rn2 += dNew * dNew;
                 }
-                const T uo = pair_u<T, PER>(Prec<T>::sqrt_(ro2), rc, hinv, Ps, spl, u_out);
-                const T un = pair_u<T, PER>(Prec<T>::sqrt_(rn2), rc, hinv, Ps, spl, u_out);
+                const T uo = uof(Prec<T>::sqrt_(ro2));
+                const T un = uof(Prec<T>::sqrt_(rn2));
                 delta += un - uo;
             };
             // rounds in which every lane of the group has a partner run under a
@@ -377,10 +424,10 @@ __device__ __forceinline__ void sweep_body(const KSys& sy, const KTables<T>& tb,
     if (gl == 0 && a.acc) a.acc[w] += nacc;
 }
 
-template <typename T, int D, int LPW, int KPL, bool PER>
+template <typename T, int D, int LPW, int KPL, bool PER, bool NU>
 __global__ __launch_bounds__(kThreads) void sweep_kernel(KSys sy, KTables<T> tb, SweepArgs a)
 {
-    sweep_body<T, D, LPW, KPL, PER>(sy, tb, a, blockIdx.x);
+    sweep_body<T, D, LPW, KPL, PER, NU>(sy, tb, a, blockIdx.x);
 }
 
 // Batch of independent systems in one launch: workgroup b belongs to system
@@ -427,7 +474,8 @@ template <typename T, int D, int LPW, int KPL, bool PER>
 static hipError_t launch_sweep_k(const KSys& sy, const KTables<T>& tb, const SweepArgs& a, hipStream_t st)
 {
     const size_t lds = sweep_lds_bytes<T>(sy, LPW);
-    auto kern = sweep_kernel<T, D, LPW, KPL, PER>;
+    if (sy.nu && (!PER || !tb.nucell)) return hipErrorInvalidValue;
+    auto kern = sy.nu ? sweep_kernel<T, D, LPW, KPL, PER, PER> : sweep_kernel<T, D, LPW, KPL, PER, false>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;

@@ -30,10 +30,15 @@ Outputs (inputs and expected outputs only; no reference source):
   resume_cases.json   -- the driver's successive-run coordinate files (text)
                          and the coordinates its loader reads from config/
   sample_bulk3d_sw16.json -- <E_R> +- error of 256 reference Metropolis chains
+  eval_nurbs_<name>.npz -- as eval_<name>.npz on non-uniform knots (USE_NURBS:
+                         BosonsBulk::SetNodes with the knots in `grid`)
+  spline_weights.json -- SplineFactory::GetWeights tables and BC factors on
+                         uniform and NURBS knot vectors
 
     python tests/golden/make_golden.py obs    # only the observables fixtures
     python tests/golden/make_golden.py drv    # only the driver fixtures (needs `make -C oracle drv`)
     python tests/golden/make_golden.py sample # only the reference sampling fixture
+    python tests/golden/make_golden.py nurbs  # only the non-uniform knot fixtures and spline tables
 """
 import ctypes as C
 import json
@@ -203,6 +208,88 @@ def eval_large():
         np.savez_compressed(os.path.join(HERE, f"eval_{name}.npz"), D=D, N=N, L=L, P=P, sysparams=spa, time=time,
                             R=R, uR=uR, uI=uI, phiR=phiR, phiI=phiI, expo=expo, O=O, ER=ER, EI=EI, other=oth,
                             grad=grad, lap=lap)
+
+
+NURBS_CASES = [
+    # name, D, N, L, P, sysparams, W, seed, time, knot exponent (g_k = rc (k / (P-1))^a, then jittered)
+    ("bulk3d_n32_p12", 3, 32, 4.5, 12, [0.7, 2.0], 6, 41, 0.0, 1.4),
+    ("bulk2d_n25_p10", 2, 25, 6.0, 10, [0.6, 2.0], 6, 42, 0.0, 0.8),
+    ("bulk1d_n20_p9", 1, 20, 20.0, 9, [0.5, 3.0], 6, 43, 0.0, 1.2),
+    ("bulk3d_quench_p8", 3, 16, 3.5, 8, [0.7, 2.0, 0.5, 0.9, 4.0], 4, 44, 1.0, 1.6),
+]
+
+
+def nurbs_grid(P, rc, a, rng):
+    """Strictly increasing knots from 0 to rc: a power-law spacing with the
+    interior knots jittered by up to 15 % of their gaps."""
+    g = rc * (np.arange(P) / (P - 1)) ** a
+    gaps = np.diff(g)
+    jit = rng.uniform(-0.15, 0.15, P - 2) * np.minimum(gaps[:-1], gaps[1:])
+    g[1:-1] += jit
+    return g
+
+
+def nurbs_cases():
+    """BosonsBulk on non-uniform knots (SetNodes, src/PhysicalSystems/BosonsBulk.cpp:35-44),
+    evaluated by the reference on fed configurations (ref_bosonsbulk_eval_nurbs)."""
+    r = o.ref()
+    for (name, D, N, L, P, sp, W, seed, time, a) in NURBS_CASES:
+        rng = np.random.default_rng(seed)
+        grid = nurbs_grid(P, L / 2.0, a, rng)
+        R = rng.uniform(-L / 2, L / 2, (W, N, D))
+        R[W // 2:] += rng.integers(-2, 3, (W - W // 2, N, D)) * L
+        # one pair per walker on an interior knot: the lower_bound tie rule (a
+        # pair at exactly L/2 = rc along an axis is left out: the reference's
+        # minimum image moves it 1e-10 inwards, src/Utils.cpp:266-281, where the
+        # kernels keep it at L/2)
+        for w in range(W):
+            d = np.zeros(D)
+            d[0] = grid[1 + (w % (P - 2))]
+            R[w, 1] = R[w, 0] + d
+        uR = rng.normal(0, 0.3, P)
+        uI = rng.normal(0, 0.2, P)
+        phiR, phiI = 0.05, -0.02
+        S = P + 2
+        spa = np.array(sp, dtype=np.float64)
+        expo, O, ER, EI = np.zeros(W), np.zeros((W, P)), np.zeros(W), np.zeros(W)
+        oth, sD, sD2 = np.zeros((W, 9)), np.zeros((W, S, N, D)), np.zeros((W, S, N))
+        R = np.ascontiguousarray(R)
+        r.ref_bosonsbulk_eval_nurbs(N, D, L, P, f(grid), f(spa), len(sp), time, W, f(R), f(uR), f(uI), phiR, phiI,
+                                    f(expo), f(O), f(ER), f(EI), f(oth), f(sD), f(sD2))
+        np.savez_compressed(os.path.join(HERE, f"eval_nurbs_{name}.npz"), D=D, N=N, L=L, P=P, sysparams=spa,
+                            time=time, grid=grid, R=R, uR=uR, uI=uI, phiR=phiR, phiI=phiI, expo=expo, O=O, ER=ER,
+                            EI=EI, other=oth, sD=sD, sD2=sD2)
+
+
+def spline_weights():
+    """SplineFactory::GetWeights and the zero-slope BC factors (uniform = true)
+    on the knot vectors BosonsBulk builds: uniform (BosonsBulk.cpp:60-66) and
+    from a NURBS_GRID (SetNodes, :35-44)."""
+    r = o.ref()
+    r.ref_spline_weights.restype = C.c_int
+    dp = C.POINTER(C.c_double)
+    r.ref_spline_weights.argtypes = [dp, C.c_int, dp, dp, dp]
+    cases = []
+    rng = np.random.default_rng(51)
+    for (P, rc, grid) in ((8, 2.5, None), (12, 4.0, None), (10, 3.0, nurbs_grid(10, 3.0, 1.3, rng)),
+                          (16, 7.5, nurbs_grid(16, 7.5, 0.8, rng))):
+        if grid is None:
+            nodes = np.array([i * rc / (P - 1.0) for i in range(-3, P + 3)])
+        else:
+            nodes = list(grid)
+            for i in range(3):
+                nodes.insert(0, -grid[i + 1])
+                nodes.append(2.0 * grid[P - 1] - grid[P - 2 - i])
+            nodes = np.array(nodes)
+        S = nodes.size - 4
+        w, bcs, bce = np.zeros(S * 16), np.zeros(6), np.zeros(6)
+        assert r.ref_spline_weights(f(nodes), nodes.size, f(w), f(bcs), f(bce)) == S
+        cases.append(dict(P=P, rc=rc, grid=None if grid is None else grid.tolist(), nodes=nodes.tolist(),
+                          weights=w.reshape(S, 4, 4).tolist(), bcFactorsStart=bcs.reshape(2, 3).tolist(),
+                          bcFactorsEnd=bce.reshape(2, 3).tolist()))
+    with open(os.path.join(HERE, "spline_weights.json"), "w") as fh:
+        json.dump(dict(source="SplineFactory::GetWeights (src/SplineFactory.cpp:54-101), "
+                              "SetBoundaryConditions3_1D_OR_2 / _CO_2 (:423-456, :563-596)", cases=cases), fh)
 
 
 def vmcsampler_case():
@@ -494,9 +581,14 @@ if __name__ == "__main__":
     if sys.argv[1:] == ["sample"]:
         sample_case()
         sys.exit(0)
+    if sys.argv[1:] == ["nurbs"]:
+        nurbs_cases()
+        spline_weights()
+        sys.exit(0)
     if sys.argv[1:] != ["obs"]:
         nic()
         eval_cases()
+        nurbs_cases()
         move_cases()
         configs()
     obs_cases()

@@ -24,12 +24,14 @@ def oracle_system(cfg):
     D = int(cfg["DIM"])
     K = int(cfg.get("K_VECTORS", 0))
     rc = float(cfg.get("SPLINE_CUTOFF", 0.0)) or L / 2.0
+    grid = np.array(cfg["NURBS_GRID"], dtype=np.float64) if cfg.get("USE_NURBS") else None
     kv = o.kvectors(D, K, L) if K else np.zeros((0, D))
     sp = tuple(cfg.get("SYSTEM_PARAMS", ()))
     return o.System(N=int(cfg["N"]), D=D, L=L, P_s=int(cfg["N_PARAM"]), rc=rc, kvec=kv,
                     one_body=int(cfg.get("ONE_BODY", 0)), hbar2_2m=float(cfg.get("HBAR2_2M", 1.0)) or 1.0,
                     pot_type=POT[cfg.get("PAIR_POTENTIAL", "SQUARE_WELL")], pot=sp,
-                    pot_cutoff=rc if L > 0 else 1e300, trap_omega=float(cfg.get("TRAP_OMEGA", 0.0)))
+                    pot_cutoff=(rc if grid is None else float(grid[-1])) if L > 0 else 1e300,
+                    trap_omega=float(cfg.get("TRAP_OMEGA", 0.0)), grid=grid)
 
 
 def rel(a, b):

@@ -36,6 +36,9 @@ def test_fp64_vs_reference_golden(path):
     z = np.load(path, allow_pickle=False)
     N, D, L, P = int(z["N"]), int(z["D"]), float(z["L"]), int(z["P"])
     cfg = make_cfg(N, D, L, P, pot="SQUARE_WELL", sysparams=[float(x) for x in z["sysparams"]])
+    if "grid" in z.files:
+        # eval_nurbs_*: non-uniform knots
+        cfg.update(USE_NURBS=1, NURBS_GRID=[float(x) for x in z["grid"]])
     print("case", os.path.basename(path), flush=True)
     with _ctx(cfg) as ctx:
         ctx.set_params(z["uR"], z["uI"], float(z["phiR"]), float(z["phiI"]))

@@ -38,8 +38,10 @@ def test_nic_random_vs_reference(golden_dir):
 
 def _case(path):
     z = np.load(path, allow_pickle=False)
+    # eval_nurbs_*: non-uniform knots (USE_NURBS), the knots stored as `grid`
+    grid = z["grid"] if "grid" in z.files else None
     s = o.System(N=int(z["N"]), D=int(z["D"]), L=float(z["L"]), P_s=int(z["P"]), pot_type=o.POT_SQUARE,
-                 pot=tuple(float(x) for x in z["sysparams"]), hbar2_2m=1.0)
+                 pot=tuple(float(x) for x in z["sysparams"]), hbar2_2m=1.0, grid=grid)
     return z, s
 
 
