@@ -11,7 +11,7 @@ one measurement (ln psi, O_k, E_L) per walker, the moments S / F and the FP64
 Cholesky solve -- all on the device, replayed from a captured hipGraph.
 
 Printed: one JSON line.  value = walker-samples/s of the mixed-precision form
-(FP32 pair / spline / phase arithmetic, bf16x3 MFMA moments, FP64 solve);
+(FP32 pair / spline / phase arithmetic, split-bf16 MFMA moments, FP64 solve);
 the same line carries the FP64 plain form's rate, TDVP steps/s, the roofline
 of the dominant kernel (the sweep), and the reference CPU path timed on this
 host.
@@ -32,16 +32,22 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3     # MI355X published vector / matrix FP32 peak
 PEAK_FP64_TFLOPS = 78.6
 
-# HBM bytes per launch of the two hot kernels at 65536 walkers, from one
-# rocprofv3 pass each of FETCH_SIZE and WRITE_SIZE (KiB per dispatch), scaled
-# by what the same counters read for an elementwise kernel that reads and
-# writes 1 GiB (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000).  Algorithmic bytes:
-# sweep = coordinates in + out (2 W N D 8) + rho_k out (W K 8) = 1.213 GB,
-# measurement = coordinates + rho_k in + O_k (FP32) and energies out = 0.753 GB.
-MEASURED_TRAFFIC = {
-    ("bulk3d_lj343_k256", 65536): {"sweep": 1.1290e9, "eval": 7.073e8,
-                                   "source": "profiles/traffic_bulk3d_lj343_k256.md"},
-}
+def measured_traffic(workload, W):
+    """HBM bytes per launch of the sweep and the measurement, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary kept in profiles/ for this
+    workload and walker count (its `traffic-json:` line), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic-json:"):
+                    d = json.loads(line.split(":", 1)[1])
+                    if d.get("walkers") == W:
+                        d["source"] = os.path.relpath(path, ROOT)
+                        return d
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def sweep_flops_per_move(N, D, K):
@@ -205,46 +211,87 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_mib": mib, "launches": 10, "bytes_read": n * 4, "bytes_written": n * 4}))
 
 
-def cpu_baseline(budget_s=15.0):
-    """Reference BosonsBulk plug-in (compiled from the reference sources,
-    oracle/_ref) on one host core: N = 343 3-D, 40 pair-spline parameters,
-    square-well pair potential (the reference has neither the LJ form in this
-    class nor the k-space term), one walker doing walker-samples of N moves +
-    one measurement, like the reference's sampling loop."""
-    try:
-        import ctypes as C
+def _cpu_worker(args):
+    """One host core: the reference BosonsBulk plug-in (oracle/_ref/libref.so,
+    compiled from the reference sources) sampling one walker for budget_s:
+    walker-samples of N moves + one measurement, as the reference's loop."""
+    import ctypes as C
 
-        from oracle import oracle as o
-
-        if not o.ref_available():
-            return None
-        r = o.ref()
-    except Exception:
-        return None
+    from oracle import oracle as o
     from tdvmc_amd.workloads import mcmillan_params
 
+    seed, budget_s = args
+    r = o.ref()
     N, D, P = 343, 3, 40
     L = (N / (0.365 / 2.556 ** 3)) ** (1 / 3)
     n = 7
     g = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3)
-    rng = np.random.default_rng(1)
+    rng = np.random.default_rng(seed)
     R = np.ascontiguousarray((g + rng.uniform(-0.05, 0.05, g.shape)) * (L / n), dtype=np.float64)
     uR = np.ascontiguousarray(mcmillan_params(P, L / 2, 1.1 * 2.556, 0.6 * 2.556))
     sp = np.array([2.2, 1.0])
     f = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
     mean = C.c_double()
-    # calibrate with one sample, then run for ~budget_s
     t0 = time.perf_counter()
-    r.ref_bosonsbulk_sample(N, D, L, P, f(sp), 2, f(R), f(uR), 0.3 * 2.556, 7, 1, N, C.byref(mean))
+    r.ref_bosonsbulk_sample(N, D, L, P, f(sp), 2, f(R), f(uR), 0.3 * 2.556, seed, 1, N, C.byref(mean))
     t1 = time.perf_counter() - t0
     n_samp = max(2, int(budget_s / max(t1, 1e-6)))
     t0 = time.perf_counter()
-    r.ref_bosonsbulk_sample(N, D, L, P, f(sp), 2, f(R), f(uR), 0.3 * 2.556, 8, n_samp, N, C.byref(mean))
-    el = time.perf_counter() - t0
-    return dict(value=n_samp / el, unit="walker-samples/s", cores=1, kind="reference",
-                sample=f"{n_samp} walker-samples (N=343 3-D, {N} moves + 1 measurement each) of the reference "
+    r.ref_bosonsbulk_sample(N, D, L, P, f(sp), 2, f(R), f(uR), 0.3 * 2.556, seed + 1000, n_samp, N,
+                            C.byref(mean))
+    return n_samp, time.perf_counter() - t0
+
+
+def host_cores():
+    """Cores this process may run on, at most 16 (one GPU's share of a box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(budget_s=15.0):
+    """Reference BosonsBulk plug-in on every host core this process may use
+    (one forked worker per core, each an independent walker, before this
+    process touches the GPU): N = 343 3-D, 40 pair-spline parameters,
+    square-well pair potential (the reference's BosonsBulk has the square well
+    built in and no k-space term, so this is the pair-only part of the
+    workload).  Throughput = total walker-samples / wall time."""
+    try:
+        from oracle import oracle as o
+
+        if not o.ref_available():
+            return None
+    except Exception:
+        return None
+    import multiprocessing as mp
+
+    cores = host_cores()
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_cpu_worker, [(11 + 7 * c, budget_s) for c in range(cores)])
+    wall = time.perf_counter() - t0
+    total = sum(n for n, _ in res)
+    # each worker's own rate, summed (the pool's start-up is not sample time)
+    rate = sum(n / el for n, el in res)
+    return dict(value=rate, unit="walker-samples/s", cores=cores, kind="reference", cpu=cpu_model(),
+                nproc=os.cpu_count(), wall_s=wall,
+                sample=f"{total} walker-samples (N=343 3-D, {343} moves + 1 measurement each) of the reference "
                        f"BosonsBulk plug-in, 40 pair parameters, square-well potential, no k-space term; "
-                       f"single thread, {el:.1f} s")
+                       f"{cores} processes (one walker each) for about {budget_s:.0f} s")
 
 
 def run_batch_c4(steps, warmup):
@@ -328,6 +375,8 @@ def main():
         print(json.dumps(res))
         return
 
+    # the host baseline forks its workers before this process touches the GPU
+    cpu = None if args.no_cpu else cpu_baseline()
     make = WORKLOADS[args.workload]
     kw = {"W": args.walkers} if args.walkers else {}
     cfg = make(**kw)
@@ -354,7 +403,7 @@ def main():
                                 W * measure_flops(N, D, K), PEAK_FP32_TFLOPS),
     }
     dom = max(rl, key=lambda k: rl[k]["avg_ms"])
-    tr = MEASURED_TRAFFIC.get((args.workload, W))
+    tr = measured_traffic(args.workload, W)
     if tr:
         for k in rl:
             rl[k]["traffic"] = tr[k]
@@ -370,7 +419,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 pair/spline/phase + bf16x3 MFMA moments + fp64 solve",
+        "dtype": "fp32 pair/spline/phase + split-bf16 (bf16x6) MFMA moments + fp64 solve",
         "data": "synthetic (McMillan-initialised spline, seeded k-space amplitudes, jittered-lattice walkers)",
         "config": {"workload": args.workload, "N": N, "DIM": D, "P": info.P, "K": K,
                    "walkers": W, "step": "RK4 TDVP step = 4 x (1 sweep of N moves + measure + "
@@ -400,7 +449,7 @@ def main():
                                      "ms_per_step": r3["elapsed"] / r3["steps"] * 1e3,
                                      "kernel_ms": {k: (v[0] / max(v[1], 1)) for k, v in r3["prof"].items()},
                                      "kernel_launches": {k: v[1] for k, v in r3["prof"].items()}}
-    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
+    line["cpu_baseline"] = cpu
     print(json.dumps(line))
 
 

@@ -23,7 +23,7 @@
  *     context are not thread-safe, distinct contexts may be used concurrently;
  *     calls returning host data synchronise the context's stream;
  *   - precision: TDVMC_FP64 (plain form) or TDVMC_MIXED (FP32 pair / spline /
- *     phase arithmetic, bf16x3 MFMA moments, FP64 solve); TDVMC_PRECISION_DEFAULT
+ *     phase arithmetic, split-bf16 (bf16x6) MFMA moments, FP64 solve); TDVMC_PRECISION_DEFAULT
  *     selects the context's configured precision.
  */
 #ifndef TDVMC_H

@@ -12,6 +12,14 @@
  *   tdo_nic_coord / tdo_displacement  -> 48 known answers of src/test/Tests.h:76-156
  *   tdo_bspline                        -> Cox-de Boor recursion, partition of unity,
  *                                         finite differences (tests/test_oracle_math.py)
+ *   tdo_bspline_nu (non-uniform knots) -> golden vectors of the compiled reference's
+ *                                         BosonsBulk on NURBS_GRID knots
+ *                                         (tests/golden/eval_nurbs_*.npz), partition of
+ *                                         unity, finite differences, and the uniform
+ *                                         basis on a uniform grid (tests/test_knots_cpu.py)
+ *   tdo_basis_sums                     -> partition of unity (sum_s B_s = pairs within
+ *                                         rc) and the BC map onto tdo_eval's O
+ *                                         (tests/test_gpu_nurbs.py)
  *   tdo_eval                           -> golden vectors written by the compiled
  *                                         reference (oracle/_ref, BosonsBulk) plus
  *                                         linearity of ln(psi) in alpha, finite
@@ -32,7 +40,13 @@
  *   tdo_pair_potential                 -> square well / quench: reference eval goldens;
  *                                         Lennard-Jones: LJ::GetPotential values
  *                                         (tests/golden/driver_cases.json)
- *   tdo_metropolis                     -> exact two-particle distribution exp(2u(r))
+ *   tdo_metropolis                     -> exact two-particle distribution exp(2u(r));
+ *                                         <E_L> against 256 chains of the compiled
+ *                                         reference (tests/golden/sample_bulk3d_sw16.json)
+ *   (oracle.py) ode_step               -> exact solution of a linear ODE for every ODE
+ *                                         type (tests/test_oracle_math.py); the gauge shift
+ *                                         of AdjustParameters against the reference driver
+ *                                         (tests/golden/driver_cases.json)
  *   tdo_gr_grid / tdo_kshells /        -> golden vectors of the compiled reference's
  *   tdo_observables                       CalculateAdditionalSystemProperties, the
  *                                         reference's kVectors / kNorm files, S(k) of a

@@ -91,7 +91,7 @@ size_t sweep_lds_bytes(const KSys& sy, int lanes_per_walker);
 // ---- moments (kernels_moments.hip)
 // O [W][P] (float or double), E_R, E_I [W] (double) ->
 //   mean[P+2] = (<O>, <E_R>, <E_I>), S [P][P] = cov(O, O), F [2][P] = cov(O, E_R), cov(O, E_I)
-// mode 0: FP64 (MFMA f64); mode 1: bf16x3 split products on MFMA with FP32
+// mode 0: FP64 (MFMA f64); mode 1: split-bf16 (bf16x6) products on MFMA with FP32
 // accumulation over 256-walker slices and FP64 reduction across slices.
 struct MomentsWork {
     void* xt;                  // packed centred operands (size from moments_work_bytes)

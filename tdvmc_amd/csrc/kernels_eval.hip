@@ -259,15 +259,16 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
                     inside = r2 <= rc2lo;
                     const bool band = r2 <= rc2hi && !inside;
                     if (__builtin_amdgcn_ballot_w64(band)) {   // rare: some lane near rc
-                        if (band) {
-                            double e2 = 0.0;
+                        // keeps the exact test behind the branch (the compiler
+                        // would otherwise evaluate it for every pair)
+                        asm volatile("" ::: "memory");
+                        double e2 = 0.0;
 #pragma unroll
-                            for (int a = 0; a < D; a++) {
-                                const double de = C::diff_exact(rn[a], cp[a]);
-                                e2 = fma(de, de, e2);
-                            }
-                            inside = e2 <= rc2x;
+                        for (int a = 0; a < D; a++) {
+                            const double de = C::diff_exact(rn[a], cp[a]);
+                            e2 = fma(de, de, e2);
                         }
+                        inside = inside || (band && e2 <= rc2x);
                     }
                 } else {
                     inside = r <= rc;

@@ -10,7 +10,8 @@
 //                   three bf16 terms x = x0 + x1 + x2 (exact: 24 significant
 //                   bits) and each product is formed from the split terms on
 //                   v_mfma_f32_32x32x16_bf16: x0*y0 + x0*y1 + x1*y0 (the
-//                   "bf16x3" products) + x1*y1 + x0*y2 + x2*y0, leaving out
+//                   "bf16x3" products) + x1*y1 + x0*y2 + x2*y0 (together
+//                   "bf16x6", Henry, Tang, Heinecke, ARITH 2019), leaving out
 //                   terms of relative size 2^-24 and below.  With two-term
 //                   splits (16 bits) the per-entry residual of ~2^-18 left
 //                   F = <O E> - <O><E> 1.4e-6 off FP64 on sampled N = 343
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(64) void gemm64_kernel(const double* __restrict__ x
 }
 
 // ---- pass 3b (mode 1): split-bf16 MFMA, six products per k-step
-__global__ __launch_bounds__(64) void gemm_bf16x3_kernel(const __bf16* __restrict__ xt, int64_t ksteps, int nb,
+__global__ __launch_bounds__(64) void gemm_bf16x6_kernel(const __bf16* __restrict__ xt, int64_t ksteps, int nb,
                                                          int groups, double* __restrict__ partial)
 {
     const int lane = threadIdx.x, ntiles = nb * (nb + 1) / 2;
@@ -342,7 +343,7 @@ hipError_t launch_moments(const TO* O, const double* ER, const double* EI, int64
         if (blocks > 16384) blocks = 16384;
         hipLaunchKernelGGL(pack_bf16_kernel<TO>, dim3((unsigned)blocks), dim3(256), 0, st, O, ER, EI, W, P, mean,
                            g.Wp, g.Pp, static_cast<__bf16*>(wk.xt));
-        hipLaunchKernelGGL(gemm_bf16x3_kernel, dim3(g.ntiles * g.groups), dim3(64), 0, st,
+        hipLaunchKernelGGL(gemm_bf16x6_kernel, dim3(g.ntiles * g.groups), dim3(64), 0, st,
                            static_cast<const __bf16*>(wk.xt), g.ksteps, g.nb, g.groups, wk.partial);
     }
     hipLaunchKernelGGL(reduce_tiles_kernel, dim3(g.ntiles), dim3(256), 0, st, wk.partial, g.ntiles, g.nb, g.tile,

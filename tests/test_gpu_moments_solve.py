@@ -50,31 +50,31 @@ def test_moments_fp64(W, P):
     assert np.array_equal(m["S"], m["S"].T)
 
 
-def test_moments_bf16x3_realistic():
+def test_moments_split_bf16_realistic():
     # O with the statistics of a 343-particle measurement: large means, small
     # relative fluctuations (this is where un-centred bf16 products would fail)
     W, P = 65536, 296
     O, ER, EI = _samples(W, P, seed=11, scale=30.0, mean=1000.0)
-    print("moments bf16x3", W, P, flush=True)
+    print("moments split-bf16", W, P, flush=True)
     with _ctx() as ctx:
         m = ctx.moments(O, ER, EI, precision=tdvmc.MIXED)
         m64 = ctx.moments(O, ER, EI, precision=tdvmc.FP64)
     ref = o.moments(O, ER, EI)
     eS, eR, eI = fro(m["S"], ref["S"]), fro(m["FR"], ref["FR"]), fro(m["FI"], ref["FI"])
-    print(f"bf16x3 relative Frobenius errors: S {eS:.2e}  F_R {eR:.2e}  F_I {eI:.2e}")
+    print(f"split-bf16 relative Frobenius errors: S {eS:.2e}  F_R {eR:.2e}  F_I {eI:.2e}")
     assert eS < 1e-6 and eR < 1e-6 and eI < 1e-6
     assert fro(m64["S"], ref["S"]) < 1e-12
     assert fro(m["mO"], ref["mO"]) < 1e-13
 
 
-def test_moments_bf16x3_small_sample():
+def test_moments_split_bf16_small_sample():
     W, P = 1000, 20
     O, ER, EI = _samples(W, P, seed=5, scale=2.0, mean=10.0)
     with _ctx() as ctx:
         m = ctx.moments(O, ER, EI, precision=tdvmc.MIXED)
     ref = o.moments(O, ER, EI)
     e = fro(m["S"], ref["S"])
-    print(f"bf16x3 W=1000 relative Frobenius error of S: {e:.2e}")
+    print(f"split-bf16 W=1000 relative Frobenius error of S: {e:.2e}")
     assert e < 1e-5
 
 

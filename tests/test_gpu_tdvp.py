@@ -81,7 +81,7 @@ def test_mixed_step_close_to_fp64():
     step = res[0] - np.concatenate([uR, uI])
     err = np.max(np.abs(res[1] - res[0])) / np.max(np.abs(step))
     print(f"mixed vs fp64 RK4 step relative difference {err:.2e}")
-    # measured 5e-7 (FP32 pair arithmetic, bf16x3 moments, FP64 solve)
+    # measured 5e-7 (FP32 pair arithmetic, split-bf16 moments, FP64 solve)
     assert err < 1e-5
 
 
