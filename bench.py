@@ -179,6 +179,17 @@ def run_drift(cfg, steps, therm_sweeps=100):
             E = np.array(E)
             out[name] = dict(E=E, steps_per_s=steps / spent, walker_samples_per_s=4 * steps * W / spent,
                              finite=bool(np.all(np.isfinite(uR)) and np.all(np.isfinite(E))))
+    # control: the same walkers and sweeps (4 stages of MC_NTHERMSTEPS moves per
+    # step) with the parameters held at their start values.  What it shows is
+    # the chains still relaxing towards |psi|^2, not the TDVP dynamics.
+    with t.Context(cfg) as ctx:
+        ctx.set_walkers(R0)
+        E = [ctx.measure(t.MIXED)["localEnergyR"]]
+        for s in range(steps):
+            for k in range(4):
+                ctx.sweep(info.therm_moves, seed=900 + 4 * s + k, precision=t.MIXED)
+            E.append(ctx.measure(t.MIXED)["localEnergyR"])
+        out["fixed_params"] = dict(E=np.array(E))
     d = out["mixed"]["E"] - out["fp64"]["E"]
     res = {
         "mode": "drift", "workload": None, "walkers": W, "N": info.N, "DIM": info.DIM, "P": info.P,
@@ -187,6 +198,10 @@ def run_drift(cfg, steps, therm_sweeps=100):
         "max_abs_mixed_minus_fp64": float(np.max(np.abs(d))),
         "max_abs_mixed_minus_fp64_over_sigma": float(np.max(np.abs(d)) / sigma),
     }
+    Ef = out["fixed_params"]["E"]
+    res["fixed_params"] = {"E0": float(Ef[0]), "E_end": float(Ef[-1]), "drift": float(Ef[-1] - Ef[0]),
+                           "E_t": [float(x) for x in Ef],
+                           "what": "same walkers and sweeps, parameters held at their start values (mixed form)"}
     for name in ("fp64", "mixed"):
         E = out[name]["E"]
         res[name] = {"E0": float(E[0]), "E_end": float(E[-1]), "drift": float(E[-1] - E[0]),
