@@ -57,6 +57,12 @@ struct tdvmc_ctx {
     Config cfg;
     KSys sy{};
     NuKnots knots;                    // non-uniform knots (USE_NURBS)
+    // moments over one column per k / -k pair (the two slots hold the same
+    // |rho_k|^2): computed column -> O column, and -> its full indices
+    std::vector<int> mom_cols, mom_outmap;
+    int* dMomCols = nullptr;
+    int* dMomOut = nullptr;
+    double* dMeanCols = nullptr;
     double* dNuCell = nullptr;        // device copies of knots.cells / knots.beta
     double* dNuBeta = nullptr;
     std::vector<double> pot;          // SYSTEM_PARAMS
@@ -418,6 +424,18 @@ int build_system(tdvmc_ctx* ctx)
         ctx->kmap.push_back(i);
         ctx->kmap.push_back(partner[i]);
     }
+    ctx->mom_cols.clear();
+    ctx->mom_outmap.clear();
+    for (int k = 0; k < s.off_k; k++) {   // one-body and pair parameters: their own columns
+        ctx->mom_cols.push_back(k);
+        ctx->mom_outmap.push_back(k);
+        ctx->mom_outmap.push_back(-1);
+    }
+    for (size_t r = 0; r + 1 < ctx->kmap.size(); r += 2) {
+        ctx->mom_cols.push_back(s.off_k + ctx->kmap[r]);
+        ctx->mom_outmap.push_back(s.off_k + ctx->kmap[r]);
+        ctx->mom_outmap.push_back(ctx->kmap[r + 1] >= 0 ? s.off_k + ctx->kmap[r + 1] : -1);
+    }
     s.Kh = (int)ctx->kmap.size() / 2;
     ctx->kv_d.assign(4 * (size_t)std::max(s.Kh, 1), 0.0);
     ctx->kv_f.assign(4 * (size_t)std::max(s.Kh, 1), 0.0f);
@@ -627,7 +645,7 @@ int ensure_samples(tdvmc_ctx* ctx, int64_t n)
 int ensure_moments(tdvmc_ctx* ctx, int64_t n)
 {
     if (n <= ctx->mw_samples) return TDVMC_OK;
-    const int P = P_of(ctx);
+    const int P = (int)ctx->mom_cols.size();   // the computed columns (k / -k pairs folded)
     size_t x0, p0, c0, x1, p1, c1;
     moments_work_bytes(n, P, 0, &x0, &p0, &c0);
     moments_work_bytes(n, P, 1, &x1, &p1, &c1);
@@ -713,11 +731,17 @@ int enqueue_moments(tdvmc_ctx* ctx, int prec, int64_t n)
     hipEvent_t ev{};
     int rc = prof_begin(ctx, "moments", &ev);
     if (rc) return rc;
+    // the two slots of a k / -k pair hold the same O: one computed column each
+    MomentsWork wk = ctx->mw;
+    wk.cols = ctx->dMomCols;
+    wk.outmap = ctx->dMomOut;
+    wk.P_cols = (int)ctx->mom_cols.size();
+    wk.mean_cols = ctx->dMeanCols;
     if (prec == TDVMC_MIXED) {
-        HIPC(launch_moments<float>(ctx->dO32, ctx->dER, ctx->dEI, n, P, 1, ctx->mw, ctx->dMean, ctx->dS, ctx->dF,
+        HIPC(launch_moments<float>(ctx->dO32, ctx->dER, ctx->dEI, n, P, 1, wk, ctx->dMean, ctx->dS, ctx->dF,
                                    ctx->stream));
     } else {
-        HIPC(launch_moments<double>(ctx->dO64, ctx->dER, ctx->dEI, n, P, 0, ctx->mw, ctx->dMean, ctx->dS, ctx->dF,
+        HIPC(launch_moments<double>(ctx->dO64, ctx->dER, ctx->dEI, n, P, 0, wk, ctx->dMean, ctx->dS, ctx->dF,
                                     ctx->stream));
     }
     return prof_end(ctx, "moments", ev);
@@ -1144,6 +1168,13 @@ int alloc_ctx(tdvmc_ctx* ctx)
     HIPC(dalloc(&ctx->dEpoch, 1));
     HIPC(launch_fill_u32(ctx->dEpoch, 0u, 1, ctx->stream));
     const int Ps = ctx->sy.P_s, K = std::max(ctx->sy.Kh, 1);
+    HIPC(dalloc(&ctx->dMomCols, ctx->mom_cols.size()));
+    HIPC(dalloc(&ctx->dMomOut, ctx->mom_outmap.size()));
+    HIPC(dalloc(&ctx->dMeanCols, ctx->mom_cols.size() + 2));
+    HIPC(hipMemcpyAsync(ctx->dMomCols, ctx->mom_cols.data(), ctx->mom_cols.size() * sizeof(int),
+                        hipMemcpyHostToDevice, ctx->stream));
+    HIPC(hipMemcpyAsync(ctx->dMomOut, ctx->mom_outmap.data(), ctx->mom_outmap.size() * sizeof(int),
+                        hipMemcpyHostToDevice, ctx->stream));
     HIPC(dalloc(&ctx->dKmap, ctx->kmap.size()));
     HIPC(hipMemcpyAsync(ctx->dKmap, ctx->kmap.data(), ctx->kmap.size() * sizeof(int), hipMemcpyHostToDevice,
                         ctx->stream));
@@ -1212,6 +1243,9 @@ void release(tdvmc_ctx* ctx)
     dfree(ctx->tf.spl);
     dfree(ctx->dNuCell);
     dfree(ctx->dNuBeta);
+    dfree(ctx->dMomCols);
+    dfree(ctx->dMomOut);
+    dfree(ctx->dMeanCols);
     dfree(ctx->dKmap);
     dfree(ctx->tf.dspl);
     dfree(ctx->td.dspl);

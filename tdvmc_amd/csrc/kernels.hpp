@@ -98,6 +98,13 @@ struct MomentsWork {
     double* partial;           // per-slice partial tiles
     double* colsum;            // per-block column sums
     size_t xt_bytes, partial_bytes, colsum_bytes;
+    // optional fold of identical O columns (null cols: every column computed):
+    // computed column c reads O column cols[c] (c < P_cols) and its results
+    // go to the full indices outmap[2c], outmap[2c + 1] (-1: none)
+    const int* cols = nullptr;
+    const int* outmap = nullptr;
+    int P_cols = 0;
+    double* mean_cols = nullptr;   // [P_cols + 2] means of the computed columns
 };
 void moments_work_bytes(int64_t W, int P, int mode, size_t* xt, size_t* partial, size_t* colsum);
 template <typename TO>

@@ -256,8 +256,11 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
                 if constexpr (!PER && POT != 0) vs += pair_pot<T, POT>(pc, r, r2, rinv);
                 bool inside;
                 if constexpr (kExactCut) {
-                    inside = r2 <= rc2lo;
-                    const bool band = r2 <= rc2hi && !inside;
+                    // below the band's top: inside unless in the band, and
+                    // when no lane of the wave is in the band that is all
+                    const bool below = r2 <= rc2hi;
+                    const bool band = below && r2 > rc2lo;
+                    inside = below;
                     if (__builtin_amdgcn_ballot_w64(band)) {   // rare: some lane near rc
                         // keeps the exact test behind the branch (the compiler
                         // would otherwise evaluate it for every pair)
@@ -268,7 +271,7 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
                             const double de = C::diff_exact(rn[a], cp[a]);
                             e2 = fma(de, de, e2);
                         }
-                        inside = inside || (band && e2 <= rc2x);
+                        inside = below && (!band || e2 <= rc2x);
                     }
                 } else {
                     inside = r <= rc;

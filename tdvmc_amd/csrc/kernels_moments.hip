@@ -19,6 +19,14 @@
 //                   accumulation is folded into FP64 registers every 256
 //                   walkers, and the walker slices are reduced in FP64.
 // Only tiles on or above the diagonal are computed; F is column Pa-2 / Pa-1.
+//
+// Folded columns (MomentsWork::cols): the contraction runs over a subset of
+// O's columns (one column per group of identical ones, such as the two slots
+// of a k / -k pair that share |rho_k|^2), and the results are written to
+// every full index of the group (MomentsWork::outmap).  Rows and columns of
+// such a group are then bit-identical in S and F, so the directions along
+// which S is singular by construction stay exactly singular in both
+// precisions (the Tikhonov term then holds them at eps).
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -67,10 +75,17 @@ Geometry geometry(int64_t W, int P, int mode)
     return g;
 }
 
+// O column of computed column c (c < P): cols[c], or c; ld is O's row stride
+struct ColSrc {
+    const int* cols;
+    int ld;
+    __device__ __forceinline__ int operator()(int c) const { return cols ? cols[c] : c; }
+};
+
 // ---- pass 1: column sums of X in FP64, per row chunk
 template <typename TO>
 __global__ void colsum_kernel(const TO* __restrict__ O, const double* __restrict__ ER, const double* __restrict__ EI,
-                              int64_t W, int P, int chunks, double* __restrict__ part)
+                              int64_t W, int P, ColSrc cs, int chunks, double* __restrict__ part)
 {
     const int Pa = P + 2;
     const int c = blockIdx.x * kColBlock + threadIdx.x;
@@ -80,9 +95,10 @@ __global__ void colsum_kernel(const TO* __restrict__ O, const double* __restrict
     if (c >= Pa) return;
     double s = 0.0;
     if (c < P) {
+        const int oc = cs(c);
         // independent loads in flight; the sum keeps walker order
 #pragma unroll 8
-        for (int64_t w = w0; w < w1; w++) s += (double)O[w * P + c];
+        for (int64_t w = w0; w < w1; w++) s += (double)O[w * cs.ld + oc];
     } else if (c == P)
         for (int64_t w = w0; w < w1; w++) s += ER[w];
     else
@@ -93,7 +109,8 @@ __global__ void colsum_kernel(const TO* __restrict__ O, const double* __restrict
 // one workgroup of 256 threads per column: strided partial sums, then a fixed
 // reduction tree (wavefront butterflies, then the 4 wavefronts in order)
 __global__ __launch_bounds__(256) void colmean_kernel(const double* __restrict__ part, int chunks, int Pa, int64_t W,
-                                                      double* __restrict__ mean)
+                                                      double* __restrict__ mean, const int* __restrict__ outmap,
+                                                      int P_full, double* __restrict__ mean_full)
 {
     __shared__ double red[4];
     const int c = blockIdx.x, tid = threadIdx.x;
@@ -102,21 +119,34 @@ __global__ __launch_bounds__(256) void colmean_kernel(const double* __restrict__
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if ((tid & 63) == 0) red[tid >> 6] = s;
     __syncthreads();
-    if (tid == 0) mean[c] = (((red[0] + red[1]) + red[2]) + red[3]) / (double)W;
+    if (tid == 0) {
+        const double m = (((red[0] + red[1]) + red[2]) + red[3]) / (double)W;
+        mean[c] = m;
+        if (outmap) {
+            // computed column c -> its full indices; E_R, E_I -> P_full, P_full + 1
+            const int Pc = Pa - 2;
+            if (c < Pc) {
+                for (int k = 0; k < 2; k++)
+                    if (outmap[2 * c + k] >= 0) mean_full[outmap[2 * c + k]] = m;
+            } else {
+                mean_full[P_full + (c - Pc)] = m;
+            }
+        }
+    }
 }
 
 __device__ __forceinline__ double xval(const float* O, const double* ER, const double* EI, int64_t W, int P,
-                                       const double* mean, int64_t w, int c)
+                                       ColSrc cs, const double* mean, int64_t w, int c)
 {
     if (w >= W || c >= P + 2) return 0.0;
-    const double v = c < P ? (double)O[w * P + c] : (c == P ? ER[w] : EI[w]);
+    const double v = c < P ? (double)O[w * cs.ld + cs(c)] : (c == P ? ER[w] : EI[w]);
     return v - mean[c];
 }
 __device__ __forceinline__ double xval(const double* O, const double* ER, const double* EI, int64_t W, int P,
-                                       const double* mean, int64_t w, int c)
+                                       ColSrc cs, const double* mean, int64_t w, int c)
 {
     if (w >= W || c >= P + 2) return 0.0;
-    const double v = c < P ? O[w * P + c] : (c == P ? ER[w] : EI[w]);
+    const double v = c < P ? O[w * cs.ld + cs(c)] : (c == P ? ER[w] : EI[w]);
     return v - mean[c];
 }
 
@@ -124,7 +154,7 @@ __device__ __forceinline__ double xval(const double* O, const double* ER, const 
 //      column c = 16 cb + pp, walker w = 4 s + kk
 template <typename TO>
 __global__ void pack64_kernel(const TO* __restrict__ O, const double* __restrict__ ER, const double* __restrict__ EI,
-                              int64_t W, int P, const double* __restrict__ mean, int64_t Wp, int Pp,
+                              int64_t W, int P, ColSrc cs, const double* __restrict__ mean, int64_t Wp, int Pp,
                               double* __restrict__ xt)
 {
     const int64_t total = Wp * Pp;
@@ -139,7 +169,7 @@ __global__ void pack64_kernel(const TO* __restrict__ O, const double* __restrict
         const int cb = (int)(rest / S4);
         const int64_t w = 4 * s + kk;
         const int c = 16 * cb + pp;
-        xt[idx] = xval(O, ER, EI, W, P, mean, w, c);
+        xt[idx] = xval(O, ER, EI, W, P, cs, mean, w, c);
     }
 }
 
@@ -148,7 +178,7 @@ __global__ void pack64_kernel(const TO* __restrict__ O, const double* __restrict
 constexpr int kSplit = 3;
 template <typename TO>
 __global__ void pack_bf16_kernel(const TO* __restrict__ O, const double* __restrict__ ER, const double* __restrict__ EI,
-                                 int64_t W, int P, const double* __restrict__ mean, int64_t Wp, int Pp,
+                                 int64_t W, int P, ColSrc cs, const double* __restrict__ mean, int64_t Wp, int Pp,
                                  __bf16* __restrict__ xt)
 {
     // one thread per (cb, s, h, l32): 8 walkers of one column
@@ -168,7 +198,7 @@ __global__ void pack_bf16_kernel(const TO* __restrict__ O, const double* __restr
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int64_t w = 16 * s + 8 * h + j;
-            float x = (float)xval(O, ER, EI, W, P, mean, w, c);
+            float x = (float)xval(O, ER, EI, W, P, cs, mean, w, c);
 #pragma unroll
             for (int q = 0; q < kSplit; q++) {
                 const __bf16 xq = (__bf16)x;
@@ -252,22 +282,29 @@ __global__ __launch_bounds__(64) void gemm_bf16x6_kernel(const __bf16* __restric
     double acc64[16];
 #pragma unroll
     for (int v = 0; v < 16; v++) acc64[v] = 0.0;
+    // The leading term x0*y0 carries nearly all of each product: its 16-walker
+    // MFMA sums go to FP64 at every k-step, so FP32 rounds only one MFMA's sum
+    // of it (a 256-walker FP32 accumulator left S errors of ~1e-4 absolute on
+    // entries ~1e5, enough to make S + eps I indefinite at eps = 1e-4).  The
+    // five smaller terms (2^-8 of it and below) share an FP32 accumulator
+    // folded every 256 walkers.
+    const floatx16 zero = {};
     for (int64_t f = f0; f < f1; f++) {
-        floatx16 acc;
-#pragma unroll
-        for (int v = 0; v < 16; v++) acc[v] = 0.0f;
+        floatx16 acc = zero;
         for (int64_t s = f * 16; s < f * 16 + 16; s++) {
             const bf16x8* a = A + s * (kSplit * 64) + lane;
             const bf16x8* b = B + s * (kSplit * 64) + lane;
             const bf16x8 a0 = a[0], a1 = a[64], a2 = a[128];
             const bf16x8 b0 = b[0], b1 = b[64], b2 = b[128];
+            const floatx16 lead = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, zero, 0, 0, 0);
             // smallest terms first
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b2, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, b0, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc, 0, 0, 0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc, 0, 0, 0);
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc64[v] += (double)lead[v];
         }
 #pragma unroll
         for (int v = 0; v < 16; v++) acc64[v] += (double)acc[v];
@@ -280,7 +317,8 @@ __global__ __launch_bounds__(64) void gemm_bf16x6_kernel(const __bf16* __restric
 
 // ---- pass 4: sum slices, scale 1/W, scatter to S (both triangles) and F
 __global__ void reduce_tiles_kernel(const double* __restrict__ partial, int ntiles, int nb, int tile, int groups,
-                                    int P, int64_t W, double* __restrict__ S, double* __restrict__ F)
+                                    int P, int64_t W, const int* __restrict__ outmap, int P_full,
+                                    double* __restrict__ S, double* __restrict__ F)
 {
     const int t = blockIdx.x;
     int I, J;
@@ -294,11 +332,22 @@ __global__ void reduce_tiles_kernel(const double* __restrict__ partial, int ntil
         for (int g = 0; g < groups; g++) s += partial[((size_t)g * ntiles + t) * tt + e];
         s *= inv;
         const int i = I * tile + e / tile, j = J * tile + e % tile;
+        // full indices of computed columns i and j (one, or two for a folded group)
+        int fi[2] = {i, -1}, fj[2] = {j, -1};
+        if (outmap) {
+            if (i < P) fi[0] = outmap[2 * i], fi[1] = outmap[2 * i + 1];
+            if (j < P) fj[0] = outmap[2 * j], fj[1] = outmap[2 * j + 1];
+        }
         if (i < P && j < P) {
-            S[(size_t)i * P + j] = s;
-            S[(size_t)j * P + i] = s;
+            for (int a = 0; a < 2; a++)
+                for (int b = 0; b < 2; b++)
+                    if (fi[a] >= 0 && fj[b] >= 0) {
+                        S[(size_t)fi[a] * P_full + fj[b]] = s;
+                        S[(size_t)fj[b] * P_full + fi[a]] = s;
+                    }
         } else if (i < P && (j == P || j == P + 1)) {
-            F[(size_t)(j - P) * P + i] = s;
+            for (int a = 0; a < 2; a++)
+                if (fi[a] >= 0) F[(size_t)(j - P) * P_full + fi[a]] = s;
         }
     }
 }
@@ -321,33 +370,40 @@ hipError_t launch_moments(const TO* O, const double* ER, const double* EI, int64
                           const MomentsWork& wk, double* mean, double* S, double* F, hipStream_t st)
 {
     if (W <= 0) return hipErrorInvalidValue;
-    Geometry g = geometry(W, P, mode);
+    // computed columns: all P of O, or the wk.P_cols of a fold
+    const bool fold = wk.cols != nullptr;
+    if (fold && (!wk.outmap || !wk.mean_cols || wk.P_cols < 1 || wk.P_cols > P)) return hipErrorInvalidValue;
+    const int Pc = fold ? wk.P_cols : P;
+    const ColSrc cs{wk.cols, P};
+    double* mc = fold ? wk.mean_cols : mean;   // means of the computed columns
+    Geometry g = geometry(W, Pc, mode);
     size_t xb, pb, cb;
-    moments_work_bytes(W, P, mode, &xb, &pb, &cb);
+    moments_work_bytes(W, Pc, mode, &xb, &pb, &cb);
     if (wk.xt_bytes < xb || wk.partial_bytes < pb || wk.colsum_bytes < cb) return hipErrorInvalidValue;
 
     hipLaunchKernelGGL(colsum_kernel<TO>, dim3((unsigned)ceil_div(g.Pa, kColBlock), g.chunks), dim3(kColBlock), 0, st,
-                       O, ER, EI, W, P, g.chunks, wk.colsum);
-    hipLaunchKernelGGL(colmean_kernel, dim3((unsigned)g.Pa), dim3(256), 0, st, wk.colsum, g.chunks, g.Pa, W, mean);
+                       O, ER, EI, W, Pc, cs, g.chunks, wk.colsum);
+    hipLaunchKernelGGL(colmean_kernel, dim3((unsigned)g.Pa), dim3(256), 0, st, wk.colsum, g.chunks, g.Pa, W, mc,
+                       wk.outmap, P, mean);
     if (mode == 0) {
         const int64_t total = g.Wp * g.Pp;
         int64_t blocks = ceil_div(total, 256);
         if (blocks > 16384) blocks = 16384;
-        hipLaunchKernelGGL(pack64_kernel<TO>, dim3((unsigned)blocks), dim3(256), 0, st, O, ER, EI, W, P, mean, g.Wp,
-                           g.Pp, static_cast<double*>(wk.xt));
+        hipLaunchKernelGGL(pack64_kernel<TO>, dim3((unsigned)blocks), dim3(256), 0, st, O, ER, EI, W, Pc, cs, mc,
+                           g.Wp, g.Pp, static_cast<double*>(wk.xt));
         hipLaunchKernelGGL(gemm64_kernel, dim3(g.ntiles * g.groups), dim3(64), 0, st, static_cast<const double*>(wk.xt),
                            g.ksteps, g.nb, g.groups, wk.partial);
     } else {
         const int64_t total = (int64_t)(g.Pp / 32) * (g.Wp / 16) * 64;
         int64_t blocks = ceil_div(total, 256);
         if (blocks > 16384) blocks = 16384;
-        hipLaunchKernelGGL(pack_bf16_kernel<TO>, dim3((unsigned)blocks), dim3(256), 0, st, O, ER, EI, W, P, mean,
+        hipLaunchKernelGGL(pack_bf16_kernel<TO>, dim3((unsigned)blocks), dim3(256), 0, st, O, ER, EI, W, Pc, cs, mc,
                            g.Wp, g.Pp, static_cast<__bf16*>(wk.xt));
         hipLaunchKernelGGL(gemm_bf16x6_kernel, dim3(g.ntiles * g.groups), dim3(64), 0, st,
                            static_cast<const __bf16*>(wk.xt), g.ksteps, g.nb, g.groups, wk.partial);
     }
     hipLaunchKernelGGL(reduce_tiles_kernel, dim3(g.ntiles), dim3(256), 0, st, wk.partial, g.ntiles, g.nb, g.tile,
-                       g.groups, P, W, S, F);
+                       g.groups, Pc, W, wk.outmap, P, S, F);
     return hipGetLastError();
 }
 

@@ -65,6 +65,19 @@ def test_c3_mixed_sample_moments_vs_fp64(phase):
     eE = abs(step["localEnergyR"] - m["mE"][0]) / abs(m["mE"][0])
     print(f"mixed vs FP64 at N=343, K=256, W={W}: S {eS:.2e}  F_R {eR:.2e}  F_I {eI:.2e}  <O> {eO:.2e}  "
           f"<E_R> {eE:.2e}", flush=True)
+    # the two slots of a k / -k pair hold the same |rho_k|^2: their rows of S
+    # are identical, so S stays positive semi-definite along (e_k - e_-k)
+    # (an error there of ~1e-4 absolute broke S + eps I at eps = 1e-4 and sent
+    # a real-time run off)
+    P_s = cfg["N_PARAM"]
+    kv = o.kvectors(3, cfg["K_VECTORS"], cfg["LBOX"])
+    twins = [(a, b) for a in range(len(kv)) for b in range(a + 1, len(kv)) if np.all(kv[a] == -kv[b])]
+    assert len(twins) == 128
+    for a, b in twins:
+        assert np.array_equal(step["S"][P_s + a], step["S"][P_s + b])
+        assert step["FR"][P_s + a] == step["FR"][P_s + b]
+    w = np.linalg.eigvalsh(step["S"])
+    assert w.min() >= -1e-9 * w.max(), (w.min(), w.max())
     assert eS <= 1e-6
     assert eR <= 1e-6
     assert eI <= (1e-6 if phase == "real" else 2e-6)
