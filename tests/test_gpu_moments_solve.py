@@ -62,7 +62,8 @@ def test_moments_split_bf16_realistic():
     ref = o.moments(O, ER, EI)
     eS, eR, eI = fro(m["S"], ref["S"]), fro(m["FR"], ref["FR"]), fro(m["FI"], ref["FI"])
     print(f"split-bf16 relative Frobenius errors: S {eS:.2e}  F_R {eR:.2e}  F_I {eI:.2e}")
-    assert eS < 1e-6 and eR < 1e-6 and eI < 1e-6
+    # measured 3.9e-9 / 1.7e-9 / 1.5e-9 (three-term split, leading product to FP64 every 16 walkers)
+    assert eS < 5e-8 and eR < 5e-8 and eI < 5e-8
     assert fro(m64["S"], ref["S"]) < 1e-12
     assert fro(m["mO"], ref["mO"]) < 1e-13
 
@@ -75,7 +76,7 @@ def test_moments_split_bf16_small_sample():
     ref = o.moments(O, ER, EI)
     e = fro(m["S"], ref["S"])
     print(f"split-bf16 W=1000 relative Frobenius error of S: {e:.2e}")
-    assert e < 1e-5
+    assert e < 1e-7   # measured 8.4e-9
 
 
 @pytest.mark.parametrize("imag,pre", [(0, 0), (1, 0), (0, 1), (1, 1)])

@@ -8,13 +8,13 @@ measurement, FP32 pair / spline / phase arithmetic, split-bf16 MFMA moments)
 on 16384 walkers; then the same final walkers in the FP64 plain form (itself
 checked against the oracle on a subset) and the oracle's FP64 moments of
 those values.  S and F must agree within 1e-6 (relative Frobenius) for the
-workload as specified (real parameters).  Measured: S 2.4e-7, F_R 8.5e-7.
+workload as specified (real parameters).  Measured: S 2.2e-7, F_R 8.5e-7.
 
 What is left is the FP32 pair arithmetic itself: per-pair rounding of r
 (~1e-7 relative) moves E_L by ~5e-4 (std) on |E_L| ~ 800 and O_pair by ~8e-5
 on O ~ 2250, and F = <O E> - <O><E> is small against those scales because O
 and E_L correlate weakly.  With imaginary parameters (the 'complex' case),
-F_I = cov(O, E_I) of the much smaller E_I lands at 1.05e-6; its bound is 2e-6.
+F_I = cov(O, E_I) of the much smaller E_I lands at 1.03e-6; its bound is 2e-6.
 """
 import numpy as np
 import pytest
