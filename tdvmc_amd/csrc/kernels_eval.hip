@@ -237,9 +237,9 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
             T gR[D], gI[D], pR[D], pI[D];
 #pragma unroll
             for (int a = 0; a < D; a++) gR[a] = gI[a] = pR[a] = pI[a] = (T)0;
-            // Laplacian pieces: lR / lI = sum u''/hinv, mR / mI = sum u'/r (without
-            // per-particle output; combined once per segment), or the full
-            // per-pair term (with it)
+            // Laplacian pieces: mR / mI = sum u'/r (without per-particle output;
+            // sum u'' is linear in the interval moments and follows from the
+            // histogram at the end), or the full per-pair term lR / lI (with it)
             T lR = (T)0, lI = (T)0, mR = (T)0, mI = (T)0, plR = (T)0, plI = (T)0, vs = (T)0;
 
             auto pair = [&]() {
@@ -304,7 +304,21 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
                     lds_add(hb + 3 * HC, t3);
                     // u'(r), u''(r) per unit length on interval j
                     const T* c = dsp + j * kDsplStride;
-                    const T upR = c[0] + t * (c[1] + t * c[2]);
+                    T c0, c1, c2, c3;
+                    if constexpr (sizeof(T) == 4) {
+                        // one 16-byte LDS read (rows are 16-byte aligned)
+                        const float4 cv = *reinterpret_cast<const float4*>(c);
+                        c0 = cv.x;
+                        c1 = cv.y;
+                        c2 = cv.z;
+                        c3 = cv.w;
+                    } else {
+                        c0 = c[0];
+                        c1 = c[1];
+                        c2 = c[2];
+                        c3 = c[3];
+                    }
+                    const T upR = c0 + t * (c1 + t * c2);
                     const T fR = upR * rinv;
 #pragma unroll
                     for (int a = 0; a < D; a++) {
@@ -312,12 +326,16 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
                         pR[a] -= fR * d[a];
                     }
                     if constexpr (PP) {
-                        const T lfR = hinvT * (c[1] + t * c[3]) + lapsc * fR;
+                        const T lfR = hinvT * (c1 + t * c3) + lapsc * fR;
                         lR += lfR;
                         plR += lfR;
                     } else {
-                        lR += c[1] + t * c[3];
                         mR += fR;
+                        // u'' is not needed per pair here (it follows from the
+                        // histogram), but with three entries used the compiler
+                        // narrows the read to 12 bytes, which measured slower
+                        // in this loop: the empty operand keeps it 16 bytes
+                        if constexpr (sizeof(T) == 4) asm volatile("" ::"v"(c3));
                     }
                     if constexpr (CPLX) {
                         const T upI = c[4] + t * (c[5] + t * c[6]);
@@ -332,7 +350,6 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
                             lI += lfI;
                             plI += lfI;
                         } else {
-                            lI += c[5] + t * c[7];
                             mI += fI;
                         }
                     }
@@ -401,8 +418,9 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
                     if (CPLX) lds_add(lsum + N + ra, (double)lI);
                 } else {
                     // every pair term enters the Laplacians of both particles
-                    lap_tot_R += 2.0 * (hl * (double)lR + lapscD * (double)mR);
-                    if (CPLX) lap_tot_I += 2.0 * (hl * (double)lI + lapscD * (double)mI);
+                    // (the u'' part is added from the histogram)
+                    lap_tot_R += 2.0 * lapscD * (double)mR;
+                    if (CPLX) lap_tot_I += 2.0 * lapscD * (double)mI;
                 }
             }
         }
@@ -425,11 +443,15 @@ __device__ __forceinline__ void eval_kgrad(const KSys& sy, const KTables<T>& tb,
     const int N = sy.N, K = sy.Kh;
     const int kchunk = (K + NW - 1) / NW;
     const int k_lo = wave * kchunk, k_hi = min(K, (wave + 1) * kchunk);
+    // the amplitude table is written before the launch and only read here:
+    // read through the constant address space, its wave-uniform entries
+    // become scalar loads (no vector memory instruction per k)
+    const __attribute__((address_space(4))) T* kag = (const __attribute__((address_space(4))) T*)tb.kag;
     // sum over this wave's k of alpha |k|^2 (the constant part of the Laplacian)
     T cR = (T)0, cI = (T)0;
     for (int k = k_lo; k < k_hi; k++) {
-        cR += tb.kag[8 * k + 3];
-        if (CPLX) cI += tb.kag[8 * k + 7];
+        cR += kag[8 * k + 3];
+        if (CPLX) cI += kag[8 * k + 7];
     }
     for (int rb = 0; rb < N; rb += 64) {
         const int n = rb + lane;
@@ -441,8 +463,10 @@ __device__ __forceinline__ void eval_kgrad(const KSys& sy, const KTables<T>& tb,
 #pragma unroll
         for (int a = 0; a < D; a++) gR[a] = gI[a] = (T)0;
         T lR = (T)0, lI = (T)0;   // - sum alpha |k|^2 Re(e^{ikr} conj rho)
+        // two k per pass: the LDS reads take immediate offsets
+#pragma unroll 2
         for (int k = k_lo; k < k_hi; k++) {
-            const T* ka = tb.kag + 8 * k;
+            const __attribute__((address_space(4))) T* ka = kag + 8 * k;
             const T* qs = kq + k * 4;
             T ph = (T)0;
 #pragma unroll
@@ -735,6 +759,16 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
             }
             return v;
     };
+    // sum over pairs of u''(r) = hl (d2_j + t e3_j) (uniform knots: hl = 1/h,
+    // t in knot units; non-uniform: hl = 1, t = s) is sum_j hl (d2_j M[j][0] +
+    // e3_j M[j][1]); each pair counts for both of its particles
+    const double hl = (PER && sy.nu) ? 1.0 : sy.hinv;
+    for (int j = tid; !PP && j < Ps; j += NT) {
+        const double m0 = hist[j * 4 * hc], m1 = hist[(j * 4 + 1) * hc];
+        const T* c = dsp + j * kDsplStride;
+        sR2 += 2.0 * hl * ((double)c[1] * m0 + (double)c[3] * m1);
+        if (cplx) sI2 += 2.0 * hl * ((double)c[5] * m0 + (double)c[7] * m1);
+    }
     for (int k = tid; k < Ps; k += NT) {
         double o;
         if (k == 0) o = B(1);
