@@ -75,6 +75,8 @@ typedef struct {
     int K_terms;           /* k-space terms the kernels evaluate: one per +-k pair, plus unpaired k */
     int gr_bin_count;      /* GR_BIN_COUNT: g(r) bins requested */
     int use_nurbs;         /* USE_NURBS: pair spline on the knots of NURBS_GRID (rc = last knot) */
+    int bc_origin;         /* SPLINE_BC_ORIGIN: boundary-condition family at r = 0 (1..4) */
+    int bc_cutoff;         /* SPLINE_BC_CUTOFF: boundary-condition family at r = rc (1..4) */
 } tdvmc_info;
 
 /* Moments of one sampling stage; fields mirror SimulationStepData
@@ -154,6 +156,16 @@ int tdvmc_eval_configs(tdvmc_ctx* ctx, const double* R, int64_t W, int precision
  * (src/PhysicalSystems/BosonsBulk.cpp:179-218); tdvmc_splined_function.hpp
  * exposes them as WFParts::SplinedFunction::splineSums. */
 int tdvmc_basis_sums(tdvmc_ctx* ctx, const double* R, int64_t W, int precision, double* B);
+
+/* Boundary-condition factors of the pair spline: WFParts::SplinedFunction::
+ * bcFactorsStart / bcFactorsEnd (src/WFParts/SplinedFunction.h:10-40) as
+ * BosonsBulk::InitSystem fills them (src/PhysicalSystems/BosonsBulk.cpp:75-82,
+ * SetBoundaryConditions3_1D_OR_k / _CO_k of src/SplineFactory.cpp:377-656 with
+ * uniform = true; k = SPLINE_BC_ORIGIN / SPLINE_BC_CUTOFF, 2 by default).
+ * start and end receive [rows][3] (room for 9 doubles each), n_start and
+ * n_end the row counts; O_i = start[i] . (B_0, B_1, B_2) for i < n_start,
+ * O_{P_pair-n_end+i} = end[i] . (B_{S-3}, B_{S-2}, B_{S-1}), S = P_pair + 2. */
+int tdvmc_spline_bc(tdvmc_ctx* ctx, double* start, int* n_start, double* end, int* n_end);
 
 /* Device-pointer variant of tdvmc_eval_configs (no host copies; enqueued on
  * `stream` (a hipStream_t) or the context stream when NULL; returns without

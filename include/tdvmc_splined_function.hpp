@@ -11,9 +11,10 @@
 //                    (SplineFactory.cpp:54-101); computed here from the Cox-de Boor
 //                    recursion on the knots, not from that routine's closed forms
 //   bcFactorsStart / bcFactorsEnd, np1 .. np3, numberOf*Parameters
-//                    the zero-slope boundary conditions at 0 and rc
-//                    (SetBoundaryConditions3_1D_OR_2 / _CO_2 with uniform = true,
-//                    as BosonsBulk::InitSystem passes, BosonsBulk.cpp:75-82)
+//                    the boundary conditions at 0 and rc: BosonsBulk's zero-slope
+//                    pair (SetBoundaryConditions3_1D_OR_2 / _CO_2 with uniform =
+//                    true, BosonsBulk.cpp:75-82), or in of() the context's
+//                    families (tdvmc_spline_bc)
 //   splineSums       [s]: basis sums B_s = sum over pairs r_ij <= rc of B_s(r_ij)
 //                    for one configuration (fill_sums: tdvmc_basis_sums on the GPU)
 //
@@ -79,25 +80,44 @@ public:
         splineWeights.assign(numberOfSplines, std::vector<std::vector<double>>(4, std::vector<double>(4, 0.0)));
         for (int s = 0; s < numberOfSplines; s++)
             for (int p = 0; p < 4; p++) piece(s, p, splineWeights[s][p]);
+        // BosonsBulk's zero-slope pair (of() replaces it by the context's rows)
         bcFactorsStart = {{0.0, 1.0, 0.0}, {1.0, 0.0, 1.0}};
         bcFactorsEnd = {{1.0, 0.0, 1.0}, {0.0, 1.0, 0.0}};
+        set_counts(P);
+        splineSums.assign(numberOfSplines, 0.0);
+    }
+
+    // numberOf*Parameters and np1 .. np3 from the factor rows (BosonsBulk.cpp:78-83)
+    void set_counts(int P)
+    {
         numberOfSpecialParametersStart = (int)bcFactorsStart.size();
         numberOfSpecialParametersEnd = (int)bcFactorsEnd.size();
         numberOfStandardParameters = P - numberOfSpecialParametersStart - numberOfSpecialParametersEnd;
         np1 = numberOfSpecialParametersStart;
         np2 = np1 + numberOfStandardParameters;
         np3 = np2 + numberOfSpecialParametersEnd;
-        splineSums.assign(numberOfSplines, 0.0);
     }
 
-    // The context's pair spline (its config's N_PARAM, rc and NURBS_GRID).
+    // The context's pair spline (its config's N_PARAM, rc and NURBS_GRID, and
+    // the boundary-condition rows of SPLINE_BC_ORIGIN / SPLINE_BC_CUTOFF).
     static SplinedFunction of(tdvmc_ctx* ctx, const std::vector<double>& grid = {})
     {
         tdvmc_info info{};
         if (tdvmc_get_info(ctx, &info) != TDVMC_OK) throw std::runtime_error(tdvmc_last_error(ctx));
         if (info.use_nurbs && grid.empty())
             throw std::invalid_argument("SplinedFunction::of: pass the NURBS_GRID of a USE_NURBS config");
-        return SplinedFunction(info.P_pair, info.rc, info.use_nurbs ? grid : std::vector<double>{});
+        SplinedFunction f(info.P_pair, info.rc, info.use_nurbs ? grid : std::vector<double>{});
+        double a[9], b[9];
+        int na = 0, nb = 0;
+        if (tdvmc_spline_bc(ctx, a, &na, b, &nb) != TDVMC_OK) throw std::runtime_error(tdvmc_last_error(ctx));
+        f.bcFactorsStart.assign(na, std::vector<double>(3));
+        f.bcFactorsEnd.assign(nb, std::vector<double>(3));
+        for (int i = 0; i < na; i++)
+            for (int m = 0; m < 3; m++) f.bcFactorsStart[i][m] = a[i * 3 + m];
+        for (int i = 0; i < nb; i++)
+            for (int m = 0; m < 3; m++) f.bcFactorsEnd[i][m] = b[i * 3 + m];
+        f.set_counts(info.P_pair);
+        return f;
     }
 
     // splineSums of configuration R [N][DIM] (FP64 evaluation on the device).

@@ -37,6 +37,7 @@ class _CSystem(C.Structure):
         ("K", C.c_int), ("kvec", _dp), ("one_body", C.c_int), ("hbar2_2m", C.c_double),
         ("pot_type", C.c_int), ("n_pot", C.c_int), ("pot", C.c_double * 5),
         ("pot_cutoff", C.c_double), ("trap_omega", C.c_double), ("grid", _dp),
+        ("bc_origin", C.c_int), ("bc_cutoff", C.c_int),
     ]
 
 
@@ -59,6 +60,8 @@ class System:
     pot_cutoff: float = 0.0   # default: rc
     trap_omega: float = 0.0
     grid: np.ndarray = None   # non-uniform knots (NURBS_GRID, P_s values from 0); rc = grid[-1]
+    bc_origin: int = 2        # boundary-condition family at r = 0 (1..4; BosonsBulk: 2)
+    bc_cutoff: int = 2        # boundary-condition family at r = rc (1..4; BosonsBulk: 2)
 
     def __post_init__(self):
         if self.grid is not None:
@@ -90,6 +93,7 @@ class System:
             s.pot[i] = v
         s.pot_cutoff, s.trap_omega = self.pot_cutoff, self.trap_omega
         s.grid = _ptr(self.grid) if self.grid is not None else None
+        s.bc_origin, s.bc_cutoff = self.bc_origin, self.bc_cutoff
         s._keep = (self.kvec, self.grid)
         return s
 
@@ -111,6 +115,12 @@ def lib():
         L.tdo_bspline_nu.argtypes = [C.c_double, _dp, C.c_int, _dp, _dp, _dp]
         L.tdo_basis_sums.restype = C.c_int
         L.tdo_basis_sums.argtypes = [C.POINTER(_CSystem), _dp, _dp]
+        L.tdo_bc_factors.restype = C.c_int
+        L.tdo_bc_factors.argtypes = [C.c_int, C.c_int, _dp]
+        L.tdo_spline_coeffs.restype = None
+        L.tdo_spline_coeffs.argtypes = [C.POINTER(_CSystem), _dp, _dp]
+        L.tdo_spline_ops.restype = None
+        L.tdo_spline_ops.argtypes = [C.POINTER(_CSystem), _dp, _dp]
         L.tdo_pair_potential.restype = C.c_double
         L.tdo_pair_potential.argtypes = [C.POINTER(_CSystem), C.c_double, C.c_double]
         L.tdo_eval_batch.restype = C.c_int
@@ -213,6 +223,31 @@ def bspline_nu(r, grid, P_s):
     b, db, d2b = np.zeros(4), np.zeros(4), np.zeros(4)
     j = lib().tdo_bspline_nu(float(r), _ptr(g), int(P_s), _ptr(b), _ptr(db), _ptr(d2b))
     return j, b, db, d2b
+
+
+def bc_factors(family, at_cutoff):
+    """Boundary-condition rows of a cubic family 1..4 (tdo_bc_factors):
+    array [rows][3] of weights on splines 0..2 (origin) or S-3..S-1 (cutoff)."""
+    f = np.zeros(9)
+    n = lib().tdo_bc_factors(int(family), 1 if at_cutoff else 0, _ptr(f))
+    return f.reshape(3, 3)[:n].copy()
+
+
+def spline_coeffs(sysd: System, u):
+    """Effective spline coefficients c = T^T u (P_s + 2 values) of the pair
+    parameters u [P_s] under the system's boundary conditions."""
+    u = _f64(u)
+    c = np.zeros(sysd.P_s + 2)
+    lib().tdo_spline_coeffs(C.byref(sysd.c()), _ptr(u), _ptr(c))
+    return c
+
+
+def spline_ops(sysd: System, B):
+    """O = T B of the basis sums B [P_s + 2]."""
+    B = _f64(B)
+    O = np.zeros(sysd.P_s)
+    lib().tdo_spline_ops(C.byref(sysd.c()), _ptr(B), _ptr(O))
+    return O
 
 
 def basis_sums(sysd: System, R):

@@ -385,6 +385,38 @@ int ref_spline_weights(const double* nodes, int n, double* w, double* bcs, doubl
     return (int)W.size();
 }
 
+// The reference's cubic boundary-condition factors of one family
+// (SetBoundaryConditions3_1D_OR_k / _CO_k, src/SplineFactory.cpp:377-656)
+// on the knot vector nodes [n] with the given uniform flag: rows into
+// f [rows][3]; returns the row count, or -1 for an unknown family.
+int ref_bc_factors(int family, int at_cutoff, const double* nodes, int n, int uniform, double* f)
+{
+    vector<double> nv(nodes, nodes + n);
+    vector<vector<double> > b;
+    const double rc = nv[n - 4];
+    const bool u = uniform != 0;
+    if (!at_cutoff) {
+        switch (family) {
+        case 1: SplineFactory::SetBoundaryConditions3_1D_OR_1(nv, b, u); break;
+        case 2: SplineFactory::SetBoundaryConditions3_1D_OR_2(nv, b, u); break;
+        case 3: SplineFactory::SetBoundaryConditions3_1D_OR_3(nv, b, u); break;
+        case 4: SplineFactory::SetBoundaryConditions3_1D_OR_4(nv, b, u); break;
+        default: return -1;
+        }
+    } else {
+        switch (family) {
+        case 1: SplineFactory::SetBoundaryConditions3_1D_CO_1(nv, b, rc, u); break;
+        case 2: SplineFactory::SetBoundaryConditions3_1D_CO_2(nv, b, rc, u); break;
+        case 3: SplineFactory::SetBoundaryConditions3_1D_CO_3(nv, b, rc, u); break;
+        case 4: SplineFactory::SetBoundaryConditions3_1D_CO_4(nv, b, rc, u); break;
+        default: return -1;
+        }
+    }
+    for (size_t i = 0; i < b.size(); i++)
+        for (int k = 0; k < 3; k++) f[i * 3 + k] = b[i][k];
+    return (int)b.size();
+}
+
 int ref_write_vector(const char* dir, const char* name, const double* v, int n, const char* header)
 {
     OUT_DIR = dir;

@@ -17,6 +17,12 @@
  *                                         (tests/golden/eval_nurbs_*.npz), partition of
  *                                         unity, finite differences, and the uniform
  *                                         basis on a uniform grid (tests/test_knots_cpu.py)
+ *   tdo_bc_factors / tdo_spline_coeffs -> the reference's SetBoundaryConditions3_1D_OR_k /
+ *   / tdo_spline_ops                      _CO_k rows (tests/golden/bc_factors.json and
+ *                                         oracle/_ref live), the conditions each family
+ *                                         imposes on u (u' = 0, u'' = 0 at 0 and rc), T and
+ *                                         T^T transposes (tests/test_bc_families_cpu.py);
+ *                                         family pair (2, 2) through the eval goldens
  *   tdo_basis_sums                     -> partition of unity (sum_s B_s = pairs within
  *                                         rc) and the BC map onto tdo_eval's O
  *                                         (tests/test_gpu_nurbs.py)
@@ -75,6 +81,8 @@ typedef struct {
     double trap_omega;  /* external 0.5*omega^2*|r|^2 per particle (0 = none) */
     const double* grid; /* NULL: uniform knots; else the P_s knots of NURBS_GRID
                            (0 = g_0 < ... < g_{P_s-1}, rc = g_{P_s-1}) */
+    int bc_origin;      /* boundary-condition family at r = 0: 1..4 (0 -> 2, BosonsBulk's) */
+    int bc_cutoff;      /* boundary-condition family at r = rc: 1..4 (0 -> 2) */
 } tdo_system;
 
 /* ------------------------------------------------------------------ */
@@ -220,33 +228,86 @@ int tdo_bspline_nu(double r, const double* g, int P_s, double* b, double* db, do
     return j;
 }
 
-/* Parameter -> spline map T of the zero-slope boundary conditions at r = 0
- * (src/SplineFactory.cpp:423-456, uniform branch) and at r = rc
- * (src/SplineFactory.cpp:563-596, uniform branch), assembled as in
- * src/PhysicalSystems/BosonsBulk.cpp:158-177:
+/* Boundary-condition factors of the cubic pair spline, the uniform branch
+ * of src/SplineFactory.cpp (BosonsBulk passes uniform = true for uniform and
+ * NURBS knots alike, src/PhysicalSystems/BosonsBulk.cpp:76-77).  Row i gives
+ * parameter i's weights on splines 0, 1, 2 (origin, SetBoundaryConditions3_1D_OR_k,
+ * :377-515) or on splines S-3, S-2, S-1 (cutoff, ..._CO_k, :517-656):
+ *   1: no condition (identity, three rows)      2: u' = 0 (two rows)
+ *   3: u'' = 0 (two rows)                       4: u' = u'' = 0 (one row)
+ * Returns the row count, or 0 for an unknown family. */
+int tdo_bc_factors(int family, int at_cutoff, double f[3][3])
+{
+    memset(f, 0, 9 * sizeof(double));
+    switch (family) {
+    case 1:
+        f[0][0] = f[1][1] = f[2][2] = 1.0;
+        return 3;
+    case 2:
+        if (at_cutoff) {
+            f[0][0] = f[0][2] = 1.0;
+            f[1][1] = 1.0;
+        } else {
+            f[0][1] = 1.0;
+            f[1][0] = f[1][2] = 1.0;
+        }
+        return 2;
+    case 3:
+        if (at_cutoff) {
+            f[0][0] = 1.0; f[0][2] = -1.0;
+            f[1][1] = 1.0; f[1][2] = 2.0;
+        } else {
+            f[0][0] = -1.0; f[0][2] = 1.0;
+            f[1][0] = 2.0; f[1][1] = 1.0;
+        }
+        return 2;
+    case 4:
+        f[0][0] = f[0][1] = f[0][2] = 1.0;
+        return 1;
+    default:
+        return 0;
+    }
+}
+
+/* The parameter -> spline map T of src/PhysicalSystems/BosonsBulk.cpp:158-177
+ * (RefreshLocalOperators): the first n0 parameters are the origin rows over
+ * B_0..B_2, the next P_s - n0 - n1 are B_3, B_4, ..., the last n1 the cutoff
+ * rows over B_{S-3}..B_{S-1}, S = P_s + 2 (n0 + n1 = 4 keeps the reference's
+ * parameter count, :84-91).  With BosonsBulk's families (2, 2):
  *   O_0 = B_1, O_1 = B_0 + B_2, O_k = B_{k+1} (2 <= k <= P_s-3),
  *   O_{P_s-2} = B_{P_s-1} + B_{P_s+1}, O_{P_s-1} = B_{P_s}.
  * tdo_spline_coeffs returns c = T^T u (effective spline coefficients). */
-void tdo_spline_coeffs(int P_s, const double* u, double* c)
+static int bc_rows(const tdo_system* s, double f0[3][3], double f1[3][3], int* n1)
 {
-    int S = P_s + 2;
-    for (int s = 0; s < S; s++) c[s] = 0.0;
-    c[1] += u[0];
-    c[0] += u[1];
-    c[2] += u[1];
-    for (int k = 2; k <= P_s - 3; k++) c[k + 1] += u[k];
-    c[P_s - 1] += u[P_s - 2];
-    c[P_s + 1] += u[P_s - 2];
-    c[P_s] += u[P_s - 1];
+    int n0 = tdo_bc_factors(s->bc_origin ? s->bc_origin : 2, 0, f0);
+    *n1 = tdo_bc_factors(s->bc_cutoff ? s->bc_cutoff : 2, 1, f1);
+    return n0;
 }
 
-void tdo_spline_ops(int P_s, const double* B, double* O)
+void tdo_spline_coeffs(const tdo_system* s, const double* u, double* c)
 {
-    O[0] = B[1];
-    O[1] = B[0] + B[2];
-    for (int k = 2; k <= P_s - 3; k++) O[k] = B[k + 1];
-    O[P_s - 2] = B[P_s - 1] + B[P_s + 1];
-    O[P_s - 1] = B[P_s];
+    const int P_s = s->P_s, S = P_s + 2;
+    double f0[3][3], f1[3][3];
+    int n1;
+    const int n0 = bc_rows(s, f0, f1, &n1);
+    for (int k = 0; k < S; k++) c[k] = 0.0;
+    for (int i = 0; i < n0; i++)
+        for (int m = 0; m < 3; m++) c[m] += f0[i][m] * u[i];
+    for (int i = n0; i < P_s - n1; i++) c[3 + i - n0] += u[i];
+    for (int i = 0; i < n1; i++)
+        for (int m = 0; m < 3; m++) c[S - 3 + m] += f1[i][m] * u[P_s - n1 + i];
+}
+
+void tdo_spline_ops(const tdo_system* s, const double* B, double* O)
+{
+    const int P_s = s->P_s, S = P_s + 2;
+    double f0[3][3], f1[3][3];
+    int n1;
+    const int n0 = bc_rows(s, f0, f1, &n1);
+    for (int i = 0; i < n0; i++) O[i] = f0[i][0] * B[0] + f0[i][1] * B[1] + f0[i][2] * B[2];
+    for (int i = n0; i < P_s - n1; i++) O[i] = B[3 + i - n0];
+    for (int i = 0; i < n1; i++)
+        O[P_s - n1 + i] = f1[i][0] * B[S - 3] + f1[i][1] * B[S - 2] + f1[i][2] * B[S - 1];
 }
 
 /* ------------------------------------------------------------------ */
@@ -320,8 +381,8 @@ int tdo_eval(const tdo_system* s, const double* R, const double* uR, const doubl
         free(cR); free(cI); free(B); free(Ol); free(gR); free(gI); free(lR); free(lI); free(rho);
         return -1;
     }
-    tdo_spline_coeffs(Ps, uR + off_pair, cR);
-    tdo_spline_coeffs(Ps, uI + off_pair, cI);
+    tdo_spline_coeffs(s, uR + off_pair, cR);
+    tdo_spline_coeffs(s, uI + off_pair, cI);
 
     double outer = 0.0, Vint = 0.0, Vext = 0.0;
     double d[3], b[4], db[4], d2b[4];
@@ -356,7 +417,7 @@ int tdo_eval(const tdo_system* s, const double* R, const double* uR, const doubl
             }
         }
     }
-    tdo_spline_ops(Ps, B, Ol + off_pair);
+    tdo_spline_ops(s, B, Ol + off_pair);
 
     /* one-body Gaussian factor: ln psi += alpha * (-sum |r|^2) */
     if (s->one_body) {
@@ -570,7 +631,7 @@ int64_t tdo_metropolis(const tdo_system* s, double* R, const double* uR, double 
     const int N = s->N, D = s->D, Ps = s->P_s;
     const int off_pair = s->one_body ? 1 : 0;
     double* cR = calloc(Ps + 2, sizeof(double));
-    tdo_spline_coeffs(Ps, uR + off_pair, cR);
+    tdo_spline_coeffs(s, uR + off_pair, cR);
     tdo_rng rng;
     uint64_t x = seed;
     for (int i = 0; i < 4; i++) rng.s[i] = tdo_splitmix(&x);

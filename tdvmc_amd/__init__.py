@@ -48,7 +48,7 @@ class Info(C.Structure):
         ("precision", C.c_int), ("samples_per_walker", C.c_int), ("timestep", C.c_double),
         ("mc_step", C.c_double), ("therm_moves", C.c_int64), ("init_moves", C.c_int64), ("time", C.c_double),
         ("reg_eps", C.c_double), ("precondition", C.c_int), ("K_terms", C.c_int), ("gr_bin_count", C.c_int),
-        ("use_nurbs", C.c_int),
+        ("use_nurbs", C.c_int), ("bc_origin", C.c_int), ("bc_cutoff", C.c_int),
     ]
 
 
@@ -108,6 +108,7 @@ SYMBOLS = {
     "tdvmc_tdvp_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int]),
     "tdvmc_adjoint": (C.c_int, [C.c_void_p, _dp, C.c_int64, _dp, C.c_int, _dp, _dp]),
     "tdvmc_basis_sums": (C.c_int, [C.c_void_p, _dp, C.c_int64, C.c_int, _dp]),
+    "tdvmc_spline_bc": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int), _dp, C.POINTER(C.c_int)]),
     "tdvmc_adjoint_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int, C.c_void_p,
                                         C.c_void_p, C.c_void_p]),
     "tdvmc_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
@@ -216,6 +217,14 @@ class Context:
         info = Info()
         self._check(lib().tdvmc_get_info(self._h, C.byref(info)))
         return info
+
+    def spline_bc(self):
+        """Boundary-condition rows (bcFactorsStart [n0][3], bcFactorsEnd [n1][3])
+        of the pair spline (tdvmc_spline_bc)."""
+        a, b = np.zeros(9), np.zeros(9)
+        n0, n1 = C.c_int(), C.c_int()
+        self._check(lib().tdvmc_spline_bc(self._h, _ptr(a), C.byref(n0), _ptr(b), C.byref(n1)))
+        return a.reshape(3, 3)[:n0.value].copy(), b.reshape(3, 3)[:n1.value].copy()
 
     @property
     def N(self):

@@ -57,6 +57,7 @@ struct tdvmc_ctx {
     Config cfg;
     KSys sy{};
     NuKnots knots;                    // non-uniform knots (USE_NURBS)
+    int bc_family[2] = {2, 2};        // SPLINE_BC_ORIGIN, SPLINE_BC_CUTOFF
     // moments over one column per k / -k pair (the two slots hold the same
     // |rho_k|^2): computed column -> O column, and -> its full indices
     std::vector<int> mom_cols, mom_outmap;
@@ -337,6 +338,29 @@ int build_system(tdvmc_ctx* ctx)
         s.ncell = ctx->knots.ncell;
         s.cinv = ctx->knots.cinv;
         s.rc = c.NURBS_GRID.back();
+    }
+    // boundary conditions of the pair spline (BosonsBulk::InitSystem,
+    // BosonsBulk.cpp:75-91, with the family selectable): the knots follow from
+    // N_PARAM, so the two families must leave P_s + 2 - 6 + n0 + n1 = P_s
+    // parameters, as the reference's count check requires
+    {
+        const int f0 = c.SPLINE_BC_ORIGIN ? c.SPLINE_BC_ORIGIN : 2;
+        const int f1 = c.SPLINE_BC_CUTOFF ? c.SPLINE_BC_CUTOFF : 2;
+        s.bc_n0 = bc_factors(f0, false, s.bc_f0);
+        s.bc_n1 = bc_factors(f1, true, s.bc_f1);
+        if (s.bc_n0 == 0 || s.bc_n1 == 0) {
+            ctx->err = "SPLINE_BC_ORIGIN and SPLINE_BC_CUTOFF must be 1, 2, 3 or 4";
+            return TDVMC_ERR_BAD_CONFIG;
+        }
+        if (s.bc_n0 + s.bc_n1 != 4) {
+            ctx->err = "SPLINE_BC_ORIGIN " + std::to_string(f0) + " with SPLINE_BC_CUTOFF " + std::to_string(f1) +
+                       " fixes " + std::to_string(s.bc_n0 + s.bc_n1) + " boundary parameters; N_PARAM knots need 4 " +
+                       "(BosonsBulk.cpp:84-91: need " + std::to_string(c.N_PARAM - 4 + s.bc_n0 + s.bc_n1) +
+                       " parameters, got " + std::to_string(c.N_PARAM) + ")";
+            return TDVMC_ERR_BAD_CONFIG;
+        }
+        ctx->bc_family[0] = f0;
+        ctx->bc_family[1] = f1;
     }
     s.h = s.rc / (s.P_s - 1);
     s.hinv = 1.0 / s.h;
@@ -1863,6 +1887,8 @@ int tdvmc_get_info(tdvmc_ctx* ctx, tdvmc_info* info)
     info->K_terms = ctx->sy.Kh;
     info->gr_bin_count = ctx->cfg.GR_BIN_COUNT;
     info->use_nurbs = ctx->sy.nu;
+    info->bc_origin = ctx->bc_family[0];
+    info->bc_cutoff = ctx->bc_family[1];
     return TDVMC_OK;
 }
 
@@ -2213,6 +2239,21 @@ int tdvmc_basis_sums(tdvmc_ctx* ctx, const double* R, int64_t W, int precision, 
     if (!ctx) return TDVMC_ERR_BAD_ARG;
     CHECK_ARG(B != nullptr, "tdvmc_basis_sums: B is null");
     return eval_configs_host(ctx, R, W, precision, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, B);
+}
+
+int tdvmc_spline_bc(tdvmc_ctx* ctx, double* start, int* n_start, double* end, int* n_end)
+{
+    if (!ctx) return TDVMC_ERR_BAD_ARG;
+    CHECK_ARG(start && n_start && end && n_end, "tdvmc_spline_bc: null output");
+    const KSys& s = ctx->sy;
+    for (int i = 0; i < 3; i++)
+        for (int m = 0; m < 3; m++) {
+            start[i * 3 + m] = i < s.bc_n0 ? s.bc_f0[i][m] : 0.0;
+            end[i * 3 + m] = i < s.bc_n1 ? s.bc_f1[i][m] : 0.0;
+        }
+    *n_start = s.bc_n0;
+    *n_end = s.bc_n1;
+    return TDVMC_OK;
 }
 
 int tdvmc_observables_info(tdvmc_ctx* ctx, int* n_gr, int* n_sk, double* gr_grid, double* sk_grid)

@@ -29,6 +29,12 @@ struct KSys {
     int ncell;             // lookup cells, the last one for every r beyond rc
     double cinv;           // cells per unit length
     double hlast;          // last knot gap g_{P_s-1} - g_{P_s-2}
+    // boundary conditions of the pair spline: the first bc_n0 parameters
+    // weight splines 0..2 by the rows of bc_f0, the last bc_n1 weight splines
+    // P_s-1..P_s+1 by the rows of bc_f1, the others are splines 3, 4, ...
+    // (bc_n0 + bc_n1 = 4; BosonsBulk's zero-slope pair: 2 + 2)
+    int bc_n0, bc_n1;
+    double bc_f0[3][3], bc_f1[3][3];
 };
 
 // Per-precision device tables derived from the parameters (written by

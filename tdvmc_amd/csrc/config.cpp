@@ -121,6 +121,8 @@ std::vector<Item> items_of(Config& c)
         {"PRECISION", ItemType::String, &c.PRECISION},
         {"REG_EPS", ItemType::Double, &c.REG_EPS},
         {"MOVE_ORDER", ItemType::String, &c.MOVE_ORDER},
+        {"SPLINE_BC_ORIGIN", ItemType::Int, &c.SPLINE_BC_ORIGIN},
+        {"SPLINE_BC_CUTOFF", ItemType::Int, &c.SPLINE_BC_CUTOFF},
     };
 }
 

@@ -47,6 +47,8 @@ struct Config {
     std::string PRECISION;          // FP64 | MIXED ("" -> FP64)
     double REG_EPS = 0;             // 0 -> 0.001 (src/TDVMC.cpp:1737)
     std::string MOVE_ORDER;         // RANDOM | SEQUENTIAL ("" -> RANDOM)
+    int SPLINE_BC_ORIGIN = 0;       // pair-spline boundary condition at r = 0: 1..4 (0 -> 2)
+    int SPLINE_BC_CUTOFF = 0;       // ... at r = rc: 1..4 (0 -> 2)
 };
 
 // Parses a config JSON text with the reference's ConfigItem::setValue

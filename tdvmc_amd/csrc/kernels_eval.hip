@@ -769,13 +769,18 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
         sR2 += 2.0 * hl * ((double)c[1] * m0 + (double)c[3] * m1);
         if (cplx) sI2 += 2.0 * hl * ((double)c[5] * m0 + (double)c[7] * m1);
     }
+    // boundary-condition map O = T B: origin rows over B_0..B_2, then
+    // B_3, B_4, ..., then cutoff rows over B_{P_s-1}..B_{P_s+1}
     for (int k = tid; k < Ps; k += NT) {
         double o;
-        if (k == 0) o = B(1);
-        else if (k == 1) o = B(0) + B(2);
-        else if (k <= Ps - 3) o = B(k + 1);
-        else if (k == Ps - 2) o = B(Ps - 1) + B(Ps + 1);
-        else o = B(Ps);
+        if (k < sy.bc_n0) {
+            o = sy.bc_f0[k][0] * B(0) + sy.bc_f0[k][1] * B(1) + sy.bc_f0[k][2] * B(2);
+        } else if (k < Ps - sy.bc_n1) {
+            o = B(3 + k - sy.bc_n0);
+        } else {
+            const double* f = sy.bc_f1[k - (Ps - sy.bc_n1)];
+            o = f[0] * B(Ps - 1) + f[1] * B(Ps) + f[2] * B(Ps + 1);
+        }
         ex_p += tb.uR[sy.off_pair + k] * o;
         if (active) out.O_store(w, sy.P, sy.off_pair + k, o);
     }

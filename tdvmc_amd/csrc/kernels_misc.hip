@@ -8,16 +8,24 @@ namespace tdvmc {
 
 namespace {
 
-// Effective spline coefficient c_s = (T^T u)_s of the zero-slope boundary
-// maps (src/SplineFactory.cpp:423-456 and :563-596, uniform branch, as
-// assembled in src/PhysicalSystems/BosonsBulk.cpp:158-177).
-__device__ double spline_coeff(const double* u, int Ps, int s)
+// Effective spline coefficient c_s = (T^T u)_s of the boundary-condition map
+// (src/SplineFactory.cpp:377-656, uniform branch, assembled as in
+// src/PhysicalSystems/BosonsBulk.cpp:158-177): splines 0..2 take the origin
+// rows, splines S-3..S-1 (S = P_s + 2) the cutoff rows, spline s in between
+// parameter bc_n0 + s - 3.  With BosonsBulk's zero-slope pair: c_0 = c_2 = u_1,
+// c_1 = u_0, c_s = u_{s-1}, c_{P_s-1} = c_{P_s+1} = u_{P_s-2}, c_{P_s} = u_{P_s-1}.
+__device__ double spline_coeff(const KSys& sy, const double* u, int s)
 {
-    if (s == 0 || s == 2) return u[1];
-    if (s == 1) return u[0];
-    if (s <= Ps - 2) return u[s - 1];
-    if (s == Ps - 1 || s == Ps + 1) return u[Ps - 2];
-    return u[Ps - 1];   // s == Ps
+    const int Ps = sy.P_s, S = Ps + 2;
+    if (s < 3) {
+        double c = 0.0;
+        for (int i = 0; i < sy.bc_n0; i++) c += sy.bc_f0[i][s] * u[i];
+        return c;
+    }
+    if (s < S - 3) return u[sy.bc_n0 + s - 3];
+    double c = 0.0;
+    for (int i = 0; i < sy.bc_n1; i++) c += sy.bc_f1[i][s - (S - 3)] * u[Ps - sy.bc_n1 + i];
+    return c;
 }
 
 // Non-uniform knots: piece m of interval j is sum_p beta[j][m][p] s^p in
@@ -34,7 +42,7 @@ __device__ void prep_nu_interval(const KSys& sy, const double* uR, const double*
         const double* u = (part == 0 ? uR : uI) + sy.off_pair;
         double A[4] = {0.0, 0.0, 0.0, 0.0};
         for (int m = 0; m < 4; m++) {
-            const double c = spline_coeff(u, Ps, j + m);
+            const double c = spline_coeff(sy, u, j + m);
             for (int p = 0; p < 4; p++) A[p] += c * beta[(j * 4 + m) * 4 + p];
         }
         for (int p = 0; p < 4; p++) spl[j * 8 + part * 4 + p] = (T)A[p];
@@ -72,8 +80,8 @@ __device__ void prep_tables_body(const KSys& sy, const double* __restrict__ uR, 
         } else if (j < Ps - 1) {
             for (int part = 0; part < 2; part++) {
                 const double* u = (part == 0 ? uR : uI) + sy.off_pair;
-                const double c0 = spline_coeff(u, Ps, j), c1 = spline_coeff(u, Ps, j + 1);
-                const double c2 = spline_coeff(u, Ps, j + 2), c3 = spline_coeff(u, Ps, j + 3);
+                const double c0 = spline_coeff(sy, u, j), c1 = spline_coeff(sy, u, j + 1);
+                const double c2 = spline_coeff(sy, u, j + 2), c3 = spline_coeff(sy, u, j + 3);
                 // u(t) on knot interval j for the four uniform cubic B-spline pieces
                 const double A1 = (c2 - c0) / 2.0, A2 = (c0 - 2.0 * c1 + c2) / 2.0;
                 const double A3 = (-c0 + 3.0 * c1 - 3.0 * c2 + c3) / 6.0;

@@ -153,4 +153,29 @@ bool nu_build(const std::vector<double>& grid, double rmax, NuKnots* out, std::s
     return true;
 }
 
+int bc_factors(int family, bool at_cutoff, double f[3][3])
+{
+    for (int i = 0; i < 3; i++)
+        for (int m = 0; m < 3; m++) f[i][m] = 0.0;
+    // each family's rows as (row, spline, weight) entries
+    struct E { int i, m; double v; };
+    static const E or_1[] = {{0, 0, 1}, {1, 1, 1}, {2, 2, 1}};
+    static const E or_2[] = {{0, 1, 1}, {1, 0, 1}, {1, 2, 1}};
+    static const E or_3[] = {{0, 0, -1}, {0, 2, 1}, {1, 0, 2}, {1, 1, 1}};
+    static const E co_2[] = {{0, 0, 1}, {0, 2, 1}, {1, 1, 1}};
+    static const E co_3[] = {{0, 0, 1}, {0, 2, -1}, {1, 1, 1}, {1, 2, 2}};
+    static const E all_4[] = {{0, 0, 1}, {0, 1, 1}, {0, 2, 1}};
+    const E* e = nullptr;
+    int ne = 0, rows = 0;
+    switch (family) {
+    case 1: e = or_1; ne = 3; rows = 3; break;   // the same identity at both ends
+    case 2: e = at_cutoff ? co_2 : or_2; ne = 3; rows = 2; break;
+    case 3: e = at_cutoff ? co_3 : or_3; ne = 4; rows = 2; break;
+    case 4: e = all_4; ne = 3; rows = 1; break;
+    default: return 0;
+    }
+    for (int k = 0; k < ne; k++) f[e[k].i][e[k].m] = e[k].v;
+    return rows;
+}
+
 }  // namespace tdvmc

@@ -1,5 +1,6 @@
-// Non-uniform pair-spline knots (USE_NURBS with NURBS_GRID): the host-side
-// tables the kernels look intervals and basis pieces up in.
+// Host-side pair-spline tables: the boundary-condition factors of the cubic
+// families, and for non-uniform knots (USE_NURBS with NURBS_GRID) the tables
+// the kernels look intervals and basis pieces up in.
 #pragma once
 
 #include <string>
@@ -24,6 +25,15 @@ struct NuKnots {
 };
 
 constexpr int kMaxNuCells = 512;
+
+// Boundary-condition factors of cubic family 1..4 at the origin
+// (SetBoundaryConditions3_1D_OR_k, src/SplineFactory.cpp:377-515) or at the
+// cutoff (_CO_k, :517-656), the uniform branch that BosonsBulk passes for
+// uniform and NURBS knots alike (BosonsBulk.cpp:76-77): row i holds parameter
+// i's weights on splines 0..2 (origin) or S-3..S-1 (cutoff), rows past the
+// count are zero.  1: no condition (3 rows), 2: u' = 0 (2), 3: u'' = 0 (2),
+// 4: u' = u'' = 0 (1).  Returns the row count, 0 for an unknown family.
+int bc_factors(int family, bool at_cutoff, double f[3][3]);
 
 // Validates grid (P_s values, g_0 = 0, strictly increasing, g_{P_s-1} <=
 // rmax) and fills the tables.  Returns false with a message on bad input.

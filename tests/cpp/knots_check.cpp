@@ -6,13 +6,29 @@
 // and d/dr of the four pieces; interval P_s - 1 is the pseudo-interval
 // beyond rc).  Built by __graft_entry__.build(); compared
 // with the oracle's Cox-de Boor basis by tests/test_knots_cpu.py.
+// With the single argument "bc": the boundary-condition factors of
+// tdvmc::bc_factors instead, one line per family and end:
+// "origin|cutoff k rows f00 f01 f02 f10 ..." (tests/test_bc_families_cpu.py).
 #include "../../tdvmc_amd/csrc/knots.hpp"
 
 #include <cstdio>
 #include <iostream>
+#include <string>
 
-int main()
+int main(int argc, char** argv)
 {
+    if (argc == 2 && std::string(argv[1]) == "bc") {
+        for (int side = 0; side < 2; side++)
+            for (int k = 1; k <= 4; k++) {
+                double f[3][3];
+                const int rows = tdvmc::bc_factors(k, side == 1, f);
+                std::printf("%s %d %d", side ? "cutoff" : "origin", k, rows);
+                for (int i = 0; i < rows; i++)
+                    for (int m = 0; m < 3; m++) std::printf(" %.17g", f[i][m]);
+                std::printf("\n");
+            }
+        return 0;
+    }
     int P = 0, n = 0;
     std::cin >> P;
     std::vector<double> g(P);

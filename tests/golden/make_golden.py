@@ -292,6 +292,29 @@ def spline_weights():
                               "SetBoundaryConditions3_1D_OR_2 / _CO_2 (:423-456, :563-596)", cases=cases), fh)
 
 
+def bc_factors():
+    """The reference's cubic boundary-condition factors, every family at both
+    ends (SetBoundaryConditions3_1D_OR_1..4 / _CO_1..4, src/SplineFactory.cpp:
+    377-656), with uniform = true as BosonsBulk passes it (BosonsBulk.cpp:76-77),
+    on a uniform knot vector."""
+    r = o.ref()
+    r.ref_bc_factors.restype = C.c_int
+    dp = C.POINTER(C.c_double)
+    r.ref_bc_factors.argtypes = [C.c_int, C.c_int, dp, C.c_int, C.c_int, dp]
+    P, rc = 10, 3.0
+    nodes = np.array([i * rc / (P - 1.0) for i in range(-3, P + 3)])
+    out = {}
+    for side, name in ((0, "origin"), (1, "cutoff")):
+        for fam in (1, 2, 3, 4):
+            fv = np.zeros(9)
+            n = r.ref_bc_factors(fam, side, f(nodes), nodes.size, 1, f(fv))
+            assert n > 0
+            out[f"{name}_{fam}"] = fv.reshape(3, 3)[:n].tolist()
+    with open(os.path.join(HERE, "bc_factors.json"), "w") as fh:
+        json.dump(dict(source="SetBoundaryConditions3_1D_OR_k / _CO_k (src/SplineFactory.cpp:377-656), "
+                              "uniform = true", factors=out), fh, indent=1)
+
+
 def vmcsampler_case():
     """The 64-particle 3-D configuration and the 100 spline parameters held in
     src/VMCSampler.h:18-29 (a prototype sampler the driver does not use; its
@@ -584,6 +607,9 @@ if __name__ == "__main__":
     if sys.argv[1:] == ["nurbs"]:
         nurbs_cases()
         spline_weights()
+        sys.exit(0)
+    if sys.argv[1:] == ["bc"]:
+        bc_factors()
         sys.exit(0)
     if sys.argv[1:] != ["obs"]:
         nic()

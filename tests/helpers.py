@@ -31,7 +31,9 @@ def oracle_system(cfg):
                     one_body=int(cfg.get("ONE_BODY", 0)), hbar2_2m=float(cfg.get("HBAR2_2M", 1.0)) or 1.0,
                     pot_type=POT[cfg.get("PAIR_POTENTIAL", "SQUARE_WELL")], pot=sp,
                     pot_cutoff=(rc if grid is None else float(grid[-1])) if L > 0 else 1e300,
-                    trap_omega=float(cfg.get("TRAP_OMEGA", 0.0)), grid=grid)
+                    trap_omega=float(cfg.get("TRAP_OMEGA", 0.0)), grid=grid,
+                    bc_origin=int(cfg.get("SPLINE_BC_ORIGIN", 0)) or 2,
+                    bc_cutoff=int(cfg.get("SPLINE_BC_CUTOFF", 0)) or 2)
 
 
 def rel(a, b):

@@ -105,7 +105,8 @@ def test_files_refuse_bad_arguments(tmp_path):
 
 KIND = {"STRING": 0, "INT": 1, "DOUBLE": 2, "ARR_INT": 3, "ARR_DOUBLE": 4}
 EXTENSION_KEYS = ["HBAR2_2M", "PAIR_POTENTIAL", "K_VECTORS", "ONE_BODY", "TRAP_OMEGA", "SPLINE_CUTOFF", "N_WALKERS",
-                  "SAMPLES_PER_WALKER", "SEED", "PRECISION", "REG_EPS", "MOVE_ORDER"]
+                  "SAMPLES_PER_WALKER", "SEED", "PRECISION", "REG_EPS", "MOVE_ORDER", "SPLINE_BC_ORIGIN",
+                  "SPLINE_BC_CUTOFF"]
 
 
 def _ref_json(kind, value):
