@@ -14,12 +14,13 @@
 // factors the NB x NB diagonal block (wave-level ordering, no workgroup
 // barrier), one thread per row solves the rows below it against that block,
 // the panel is written back, and the trailing lower triangle is updated by
-// all 1024 threads in 4 x 4 register tiles (a rank-NB update with the panel
-// read from LDS; every tile's loads are issued before its stores).  The
+// all 1024 threads in 16 x 16 tiles on FP64 MFMA (a rank-NB update with the
+// panel read from LDS; every tile's loads are issued before its stores).  The
 // triangular solves are blocked the same way: each NB x NB diagonal block is
-// staged in LDS and solved by one wavefront with lane exchanges, the rest of
-// the right-hand side is updated by the whole workgroup.  The factor lives
-// column-major in the work buffer (L2-resident for P <= 581 with NB = 32).
+// staged in LDS and solved by one wavefront with v_readlane exchanges, the
+// rest of the right-hand side is updated by the whole workgroup.  The factor
+// lives column-major in the work buffer (L2-resident for P <= 581 with NB =
+// 32), with 1 / L[c][c] on its diagonal so that the solves only multiply.
 
 #include <algorithm>
 
@@ -82,7 +83,6 @@ __device__ __forceinline__ void solve_body(const SolveArgs& a)
     double* pan = reinterpret_cast<double*>(smem);   // [rows below][NB + 1] panel; [NB][NB] blocks in the solves
     __shared__ double red[kThreads / 64];
     __shared__ double ipiv[NB];          // 1 / L[c][c]
-    __shared__ double dval[NB];          // L[c][c]
     __shared__ double ldc[NB * NB];      // diagonal block, column-major: ldc[c * NB + k] = L[k][c]
     __shared__ int nbad;
     const double* mO = a.mean;
@@ -166,7 +166,7 @@ __device__ __forceinline__ void solve_body(const SolveArgs& a)
         // diagonal block by one wavefront in registers: lane l holds row
         // l mod NB, column entries reach the other lanes by v_readlane.  Every
         // lane updates its whole row (the entries above the diagonal are
-        // never read), and L[c][c] is written from dval at the end.
+        // never read), and the diagonal gets 1 / L[c][c] from ipiv at the end.
         if (wv == 0) {
             const int l = lane & (NB - 1);
             double av[NB];
@@ -175,19 +175,24 @@ __device__ __forceinline__ void solve_body(const SolveArgs& a)
 #pragma unroll
             for (int c = 0; c < NB; c++) {
                 const double d = readlane_f(av[c], c);
-                double pj;
+                double ipj;
                 if (d > 0.0) {
-                    pj = sqrt(d);
+                    // 1 / sqrt(d) from the hardware estimate and two Newton
+                    // steps (each squares the relative error): a shorter
+                    // dependent chain than a correctly rounded sqrt and a
+                    // divide, within an ulp or two of their quotient
+                    double y = __builtin_amdgcn_rsq(d);
+                    const double hd = 0.5 * d;
+                    y = y * (1.5 - hd * y * y);
+                    y = y * (1.5 - hd * y * y);
+                    ipj = y;
                 } else {
-                    pj = dorig[p0 + c];
+                    // the unreduced diagonal entry stands in as the pivot
+                    ipj = 1.0 / dorig[p0 + c];
                     if (lane == 0) nbad++;
                 }
-                const double ipj = 1.0 / pj;
                 av[c] *= ipj;
-                if (lane == 0) {
-                    ipiv[c] = ipj;
-                    dval[c] = pj;
-                }
+                if (lane == 0) ipiv[c] = ipj;
 #pragma unroll
                 for (int k = c + 1; k < NB; k++) av[k] -= av[c] * readlane_f(av[c], k);
                 // one column's lane reads at a time (scalar registers)
@@ -196,8 +201,10 @@ __device__ __forceinline__ void solve_body(const SolveArgs& a)
             if (lane < NB) {
 #pragma unroll
                 for (int k = 0; k < NB; k++) ldc[k * NB + lane] = av[k];
-                // same wavefront, after the writes above and lane 0's dval writes
-                ldc[lane * NB + lane] = dval[lane];
+                // the factor keeps 1 / L[c][c] on its diagonal: the triangular
+                // solves multiply by it (same wavefront, after the writes above
+                // and lane 0's ipiv writes)
+                ldc[lane * NB + lane] = ipiv[lane];
             }
         }
         __syncthreads();
@@ -282,11 +289,11 @@ __device__ __forceinline__ void solve_body(const SolveArgs& a)
         }
         __syncthreads();
         if (wv == 0) {
-            // lane l owns row p0 + l
+            // lane l owns row p0 + l; the diagonal holds 1 / L[j][j]
             double tR = lane < nb ? bR[p0 + lane] : 0.0, tI = lane < nb ? bI[p0 + lane] : 0.0;
             for (int j = 0; j < nb; j++) {
-                const double ljj = pan[j * NB + j];
-                const double xR = __shfl(tR, j, 64) / ljj, xI = __shfl(tI, j, 64) / ljj;
+                const double rjj = pan[j * NB + j];
+                const double xR = readlane_f(tR, j) * rjj, xI = readlane_f(tI, j) * rjj;
                 if (lane == j) {
                     tR = xR;
                     tI = xI;
@@ -346,8 +353,8 @@ __device__ __forceinline__ void solve_body(const SolveArgs& a)
         if (wv == 0) {
             double tR = lane < nb ? bR[p0 + lane] : 0.0, tI = lane < nb ? bI[p0 + lane] : 0.0;
             for (int j = nb - 1; j >= 0; j--) {
-                const double ljj = pan[j * NB + j];
-                const double xR = __shfl(tR, j, 64) / ljj, xI = __shfl(tI, j, 64) / ljj;
+                const double rjj = pan[j * NB + j];
+                const double xR = readlane_f(tR, j) * rjj, xI = readlane_f(tI, j) * rjj;
                 if (lane == j) {
                     tR = xR;
                     tI = xI;
