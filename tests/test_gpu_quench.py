@@ -1,8 +1,11 @@
 """Real-time TDVP after a quench (2-D bulk, N = 400, V0 = 5 -> 7 at t = 0,
 48 spline + 512 k-space parameters, RK4, Tikhonov eps = 1e-4), reduced to
 8192 walkers and a few steps: the mixed form and the FP64 form start from
-the same walkers and parameters, and their <E_L>(t) agree within the
-statistical error of the two runs."""
+the same walkers and parameters with the same random numbers, so their
+chains part only where an accept decision differs by rounding.  Their
+<E_L>(t) differ by at most 1.1e-3 of the statistical error of the two runs
+over 4 steps (measured); the bound is 0.05 of it, and two independent runs
+would differ by about one."""
 import numpy as np
 import pytest
 
@@ -37,4 +40,4 @@ def test_quench_mixed_tracks_fp64():
     diff = emx[:, 0] - e64[:, 0]
     sigma = np.sqrt(emx[:, 1] ** 2 + e64[:, 1] ** 2)
     print("E(t) fp64", e64[:, 0], "mixed", emx[:, 0], "diff / sigma", diff / sigma)
-    assert np.all(np.abs(diff) < 5 * sigma)
+    assert np.all(np.abs(diff) < 0.05 * sigma)
