@@ -86,8 +86,8 @@ def test_nurbs_eval_vs_oracle(name):
     print(f"mixed errors {name}: O {e_O:.2e} E_L {e_E:.2e} grad {e_g:.2e}")
     # the bounds of the uniform-knot mixed form (test_gpu_eval.py::test_mixed_vs_oracle)
     assert e_O < 2e-6
-    assert e_E < 5e-5
-    assert e_g < 5e-5
+    assert e_E < 1e-5
+    assert e_g < 1e-5
 
 
 def _u(s, uR, r):
