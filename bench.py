@@ -31,11 +31,14 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3     # MI355X published vector / matrix FP32 peak
 PEAK_FP64_TFLOPS = 78.6
+PEAK_BF16_TFLOPS = 2500.0    # MI355X dense bf16 MFMA
+PEAK_HBM_GBPS = 8000.0
 
 def measured_traffic(workload, W):
-    """HBM bytes per launch of the sweep and the measurement, from the
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary kept in profiles/ for this
-    workload and walker count (its `traffic-json:` line), or None."""
+    """HBM bytes per launch of the sweep and the measurement, and per stage of
+    the moments, from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary kept in
+    profiles/ for this workload and walker count (its `traffic-json:` line),
+    or None."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
     try:
         with open(path) as fh:
@@ -418,11 +421,28 @@ def main():
                                 W * measure_flops(N, D, K), PEAK_FP32_TFLOPS),
     }
     dom = max(rl, key=lambda k: rl[k]["avg_ms"])
+    # the moments S, F of one stage (column sums and means, centring pack,
+    # split-bf16 GEMM, tile reduction) against the dense bf16 MFMA peak: six
+    # bf16 products per entry of the lower triangle over the computed columns
+    # (one per k / -k pair, plus E_R and E_I)
+    n_col = info.one_body + info.P_pair + info.K_terms + 2
+    mom_flops = W * n_col * (n_col + 1) / 2 * 2 * 6
+    mom_ms = avg["moments"]
+    rl["moments"] = {"bound": "mfma", "achieved": mom_flops / (mom_ms * 1e-3) / 1e12 if mom_ms > 0 else 0.0,
+                     "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s", "traffic": None,
+                     "kernel": "moments segment (colsum, colmean, pack_bf16, gemm_bf16x6, reduce_tiles)",
+                     "flops_per_launch": mom_flops, "avg_ms": mom_ms, "columns": n_col}
+    rl["moments"]["frac"] = rl["moments"]["achieved"] / PEAK_BF16_TFLOPS
     tr = measured_traffic(args.workload, W)
     if tr:
         for k in rl:
-            rl[k]["traffic"] = tr[k]
-            rl[k]["traffic_source"] = tr["source"]
+            if k in tr:
+                rl[k]["traffic"] = tr[k]
+                rl[k]["traffic_source"] = tr["source"]
+        if rl["moments"]["traffic"] and mom_ms > 0:
+            gbps = rl["moments"]["traffic"] / (mom_ms * 1e-3) / 1e9
+            rl["moments"]["hbm_GBps"] = gbps
+            rl["moments"]["hbm_frac"] = gbps / PEAK_HBM_GBPS
     line = {
         "metric": "walker-samples/s + TDVP steps/s, N=343 3D, mixed vs FP64 @ S,F rel err 1e-6",
         "value": value,

@@ -106,6 +106,29 @@ def test_batch_measure_matches_single_and_oracle():
         assert rel(r["FI"], m["FI"]) < 1e-9
 
 
+def test_batch_measure_boundary_families():
+    """Each system of the batch with its own boundary-condition families."""
+    pairs = [(1, 4), (3, 3), (4, 1), (2, 3)]
+    cfgs = [dict(c, SPLINE_BC_ORIGIN=a, SPLINE_BC_CUTOFF=b) for c, (a, b) in zip(_cfgs(), pairs)]
+    print("batch measure, boundary families", flush=True)
+    with tdvmc.Batch(cfgs) as b:
+        walkers = []
+        for i, c in enumerate(b.systems):
+            s, uR, uI = _params(cfgs[i], i)
+            c.set_params(uR, uI)
+            walkers.append((s, c.get_walkers(), uR, uI))
+        res = b.measure(precision=tdvmc.FP64)
+    for i, (s, R, uR, uI) in enumerate(walkers):
+        assert (s.bc_origin, s.bc_cutoff) == pairs[i]
+        out = o.eval_configs(s, R, uR, uI)
+        m = o.moments(out["O"], out["ER"], out["EI"])
+        r = res[i]
+        assert rel(r["mO"], m["mO"]) < 1e-10
+        assert rel(r["mE"], m["mE"]) < 1e-10
+        assert rel(r["S"], m["S"]) < 1e-10
+        assert rel(r["FR"], m["FR"]) < 1e-9
+
+
 @pytest.mark.parametrize("solver", [0, 1])
 def test_batch_euler_step_vs_oracle(solver):
     cfgs = _cfgs(therm=0, IMAGINARY_TIME=1, LINEAR_EQUATION_SOLVER_TYPE=solver, REG_EPS=1e-3)
