@@ -473,6 +473,19 @@ def main():
         line["mixed_over_fp64"] = value / v64
         line["fp64_tdvp_steps_per_s"] = fp64["steps"] / fp64["elapsed"]
         line["fp64_kernel_ms"] = {k: (v[0] / max(v[1], 1)) for k, v in fp64["prof"].items()}
+        # the plain FP64 form's two big kernels against the FP64 vector peak
+        # (same algorithmic flops as the mixed form's)
+        f64ms = line["fp64_kernel_ms"]
+        lpw64 = fp64["lanes"]
+        kpl64 = next(m for m in (0, 2, 4, 8) if m * lpw64 >= info.K_terms)
+        line["roofline_fp64"] = {}
+        for k, name, fl in (("sweep", f"sweep_kernel<double, {D}, {lpw64}, {kpl64}, true>",
+                             W * info.therm_moves * sweep_flops_per_move(N, D, K)),
+                            ("eval", f"eval_kernel<double, {D}, false, true, double>", W * measure_flops(N, D, K))):
+            ach = fl / (f64ms[k] * 1e-3) / 1e12 if f64ms.get(k, 0) > 0 else 0.0
+            line["roofline_fp64"][k] = {"bound": "valu", "achieved": ach, "peak": PEAK_FP64_TFLOPS,
+                                        "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS, "kernel": name,
+                                        "flops_per_launch": fl, "avg_ms": f64ms[k]}
     if not args.no_batch:
         line["batch_trap2d_64"] = run_batch_c4(max(1, args.steps // 2), 1)
     if not args.no_obs:
