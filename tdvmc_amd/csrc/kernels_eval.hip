@@ -463,8 +463,9 @@ __device__ __forceinline__ void eval_kgrad(const KSys& sy, const KTables<T>& tb,
 #pragma unroll
         for (int a = 0; a < D; a++) gR[a] = gI[a] = (T)0;
         T lR = (T)0, lI = (T)0;   // - sum alpha |k|^2 Re(e^{ikr} conj rho)
-        // two k per pass: the LDS reads take immediate offsets
-#pragma unroll 2
+        // four k per pass: the LDS reads take immediate offsets, and the
+        // scalar loads of four amplitudes are in flight together
+#pragma unroll 4
         for (int k = k_lo; k < k_hi; k++) {
             const __attribute__((address_space(4))) T* ka = kag + 8 * k;
             const T* qs = kq + k * 4;
