@@ -203,22 +203,25 @@ __device__ __forceinline__ void eval_pairs(const KSys& sy, const typename Coord<
     // Diagonal tiles take s = 1..32 (s = 32 by lanes < 32 only), so every
     // unordered pair occurs once.  A last block of at most 32 particles is
     // held twice in the ring (slot q: particle q mod 32) and met in 32 steps
-    // s = 0..31: 32 consecutive slots hold every residue mod 32 once.  Row
-    // sums stay in registers, column sums rotate with the column coordinates;
-    // both reach LDS once per tile segment.  The steps of all tiles are split
-    // evenly over the walker's NW waves.  Off-diagonal tiles of full blocks run without
-    // per-step validity tests.
+    // s = 0..31: 32 consecutive slots hold every residue mod 32 once.  A last
+    // block of r <= 32 particles meets itself in s = 1 .. r - 1 only (lane l
+    // meets slot l + s, which is a particle of the block only for l + s < r
+    // when s <= 32).  Row sums stay in registers, column sums rotate with the
+    // column coordinates; both reach LDS once per tile segment.  The steps of
+    // all tiles are split evenly over the walker's NW waves.  Off-diagonal
+    // tiles of full blocks run without per-step validity tests.
     const int nfull = N / 64, rem = N - 64 * nfull;
     const int nblk = nfull + (rem > 0 ? 1 : 0);
     const bool fold = rem > 0 && rem <= 32 && nfull > 0;
-    const int total = nfull * (nfull - 1) / 2 * 64 + nblk * 32 + (rem > 0 ? nfull * (fold ? 32 : 64) : 0);
+    const int dlast = (rem > 0 && rem <= 32) ? rem - 1 : 32;   // steps of the last block's diagonal tile
+    const int total = nfull * (nfull - 1) / 2 * 64 + nfull * 32 + (rem > 0 ? dlast + nfull * (fold ? 32 : 64) : 0);
     const int g0 = total * wave / NW, g1 = total * (wave + 1) / NW;
     int off = 0;
     for (int A = 0; A < nblk; A++) {
         for (int Bk = A; Bk < nblk; Bk++) {
             const bool diag = A == Bk;
             const bool folded = !diag && fold && Bk == nblk - 1;
-            const int cnt = (diag || folded) ? 32 : 64;
+            const int cnt = diag ? (rem > 0 && A == nblk - 1 ? dlast : 32) : (folded ? 32 : 64);
             const int lo = max(g0, off), hi = min(g1, off + cnt);
             off += cnt;
             if (lo >= hi) continue;
