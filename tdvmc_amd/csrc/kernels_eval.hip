@@ -89,9 +89,9 @@ __device__ __forceinline__ double wave_rotate(double v)
 template <int WPW, int NV>
 __device__ __forceinline__ void walker_sum(double (&v)[NV], double* red, int wave, int lane)
 {
+    // wave sums by DPP steps and lane reads (no LDS round trips)
 #pragma unroll
-    for (int i = 0; i < NV; i++)
-        for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_xor(v[i], o, 64);
+    for (int i = 0; i < NV; i++) v[i] = group_sum<64>(v[i]);
     if constexpr (WPW > 1) {
         __syncthreads();
         if (lane == 0) {
@@ -552,6 +552,21 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     const int tid = threadIdx.x - slot * NT, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (scalar registers)
 
+    // loaded ahead, so that their latency overlaps the rest: the raw
+    // coordinates of this thread's first particle (staged below and used
+    // again by the one-body factor), its first pair parameter of the O
+    // assembly, and the parameters of the exponent
+    const double* Rw = Rg + w * (size_t)N * D;
+    double xr0[D];
+#pragma unroll
+    for (int a = 0; a < D; a++) xr0[a] = tid < N ? Rw[(size_t)tid * D + a] : 0.0;
+    const double ur0 = tid < Ps ? tb.uR[sy.off_pair + tid] : 0.0;
+    // (wave-uniform: through the constant address space they become scalar
+    // loads into scalar registers; the tables are written before the launch)
+    typedef const __attribute__((address_space(4))) double* CDP;
+    const double u_outer = ((CDP)tb.uR)[sy.outer_idx], u_phi = ((CDP)tb.uR)[sy.P];
+    const double obR = sy.one_body ? ((CDP)tb.uR)[0] : 0.0, obI = sy.one_body ? ((CDP)tb.uI)[0] : 0.0;
+
     // any nonzero imaginary parameter?  If none, the imaginary parts of the
     // pair and k-space sums are exact zeros and are skipped (and the LDS
     // layout trades the imaginary gradient sums for histogram columns)
@@ -581,10 +596,9 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     // periodic box: coordinates in box units wrapped into the primary cell
     // (every quantity is invariant under it), stored as Coord values
     const double scale = PER ? sy.Linv : 1.0;
-    const double* Rw = Rg + w * (size_t)N * D;
     for (int i = tid; i < N; i += NT) {
 #pragma unroll
-        for (int a = 0; a < D; a++) pos[a * N + i] = C::load(Rw[(size_t)i * D + a] * scale);
+        for (int a = 0; a < D; a++) pos[a * N + i] = C::load((i == tid ? xr0[a] : Rw[(size_t)i * D + a]) * scale);
         for (int c = 0; c < (cplx ? 2 * D : D); c++) gsum[c * N + i] = 0.0;
         if (PP) {
             lsum[i] = 0.0;
@@ -691,7 +705,6 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     }
 
     // ------------------------------------------------------------ assembly
-    const double obR = sy.one_body ? tb.uR[0] : 0.0, obI = sy.one_body ? tb.uI[0] : 0.0;
     double sR1 = 0, sI1 = 0, sR1I1 = 0, sR2 = lap_tot_R, sI2 = lap_tot_I, sr2 = 0, vext = 0;
     for (int n = tid; n < N; n += NT) {
         double gr[D], gi[D];
@@ -699,7 +712,7 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
 #pragma unroll
         for (int a = 0; a < D; a++) {
             // one-body factor uses the raw (unwrapped, open-space) coordinate
-            const double x = Rw[(size_t)n * D + a];
+            const double x = n == tid ? xr0[a] : Rw[(size_t)n * D + a];
             gr[a] = (double)gsum[a * N + n] - 2.0 * obR * x;
             gi[a] = (cplx ? (double)gsum[(D + a) * N + n] : 0.0) - 2.0 * obI * x;
             r2n += x * x;
@@ -785,7 +798,7 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
             const double* f = sy.bc_f1[k - (Ps - sy.bc_n1)];
             o = f[0] * B(Ps - 1) + f[1] * B(Ps) + f[2] * B(Ps + 1);
         }
-        ex_p += tb.uR[sy.off_pair + k] * o;
+        ex_p += (k == tid ? ur0 : tb.uR[sy.off_pair + k]) * o;
         if (active) out.O_store(w, sy.P, sy.off_pair + k, o);
     }
     if (out.bsum && active)
@@ -805,9 +818,9 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     if (tid == 0 && active) {
         if (sy.one_body) {
             out.O_store(w, sy.P, 0, -sr2);
-            ex += tb.uR[0] * (-sr2);
+            ex += obR * (-sr2);
         }
-        ex += tb.uR[sy.outer_idx] * outer;
+        ex += u_outer * outer;
         const double kinR = -(sR1 - sI1 + sR2) * sy.hbar2_2m;
         const double kinI = -(sR1I1 + sI2) * sy.hbar2_2m;
         out.expo[w] = ex;
@@ -818,7 +831,7 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
             double* o = out.other + w * 9;
             o[0] = kinR;
             o[1] = vpot;
-            o[2] = exp(ex + tb.uR[sy.P]);   // phi R of the parameters in force (device copy)
+            o[2] = exp(ex + u_phi);   // phi R of the parameters in force (device copy)
             o[3] = ex;
             o[4] = sR1;
             o[5] = sI1;
