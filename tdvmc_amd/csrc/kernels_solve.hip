@@ -19,8 +19,10 @@
 // triangular solves are blocked the same way: each NB x NB diagonal block is
 // staged in LDS and solved by one wavefront with v_readlane exchanges, the
 // rest of the right-hand side is updated by the whole workgroup.  The factor
-// lives column-major in the work buffer (L2-resident for P <= 581 with NB =
-// 32), with 1 / L[c][c] on its diagonal so that the solves only multiply.
+// lives column-major in the work buffer (L2-resident for the sizes the LDS
+// panel allows), with 1 / L[c][c] on its diagonal so that the solves only
+// multiply.  NB is 16 for P <= 340 (shorter serial chains in the diagonal
+// blocks and row solves) and 32 above (fewer panels and barriers).
 
 #include <algorithm>
 
@@ -672,6 +674,10 @@ __global__ __launch_bounds__(kThreads) void solve_batch_kernel(const SolveArgs* 
 // holds the NB x NB blocks of the triangular solves)
 size_t solve_lds_bytes(int NB, int P) { return (size_t)(NB + 1) * std::max(P, NB) * sizeof(double); }
 constexpr size_t kMaxPanelLds = 150 * 1024;   // beside the static LDS (NB x NB block, reductions)
+// Block width: 16 up to P = 340, 32 above (measured crossover near P = 345:
+// at P = 296, 373 against 395 us; at P = 420, 738 against 705 us), 16 again
+// where the NB = 32 panel no longer fits in LDS
+constexpr int kNb16MaxP = 340;
 
 template <int NB>
 hipError_t launch_nb(const SolveArgs& a, hipStream_t st)
@@ -705,7 +711,8 @@ hipError_t launch_batch_nb(const SolveArgs* args_dev, int n, int P_max, hipStrea
 hipError_t launch_solve_batch(const SolveArgs* args_dev, int n, int P_max, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
-    if (solve_lds_bytes(32, P_max) <= kMaxPanelLds) return launch_batch_nb<32>(args_dev, n, P_max, st);
+    if (P_max > kNb16MaxP && solve_lds_bytes(32, P_max) <= kMaxPanelLds)
+        return launch_batch_nb<32>(args_dev, n, P_max, st);
     if (solve_lds_bytes(16, P_max) <= kMaxPanelLds) return launch_batch_nb<16>(args_dev, n, P_max, st);
     return hipErrorInvalidValue;
 }
@@ -723,7 +730,7 @@ hipError_t launch_solve(const SolveArgs& a, hipStream_t st)
         hipLaunchKernelGGL(solve_qr_kernel, dim3(1), dim3(kThreads), 2 * (size_t)a.P * sizeof(int), st, a);
         return hipGetLastError();
     }
-    if (solve_lds_bytes(32, a.P) <= kMaxPanelLds) return launch_nb<32>(a, st);
+    if (a.P > kNb16MaxP && solve_lds_bytes(32, a.P) <= kMaxPanelLds) return launch_nb<32>(a, st);
     if (solve_lds_bytes(16, a.P) <= kMaxPanelLds) return launch_nb<16>(a, st);
     return hipErrorInvalidValue;
 }

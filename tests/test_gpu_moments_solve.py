@@ -80,9 +80,9 @@ def test_moments_split_bf16_small_sample():
 
 
 @pytest.mark.parametrize("imag,pre", [(0, 0), (1, 0), (0, 1), (1, 1)])
-# 32-column blocks up to P = 581, 16-column blocks beyond; last block of
-# 12, 1, 8, 16 (NB = 32) and 8, 12 (NB = 16) columns
-@pytest.mark.parametrize("P", [12, 33, 296, 560, 600, 1100])
+# 16-column blocks up to P = 340 and past P = 581, 32-column blocks between;
+# last block of 12, 1, 8, 8, 12 (NB = 16) and 1, 8, 16 (NB = 32) columns
+@pytest.mark.parametrize("P", [12, 33, 296, 600, 1100, 353, 392, 560])
 def test_solve_vs_oracle(imag, pre, P):
     rng = np.random.default_rng(P + 10 * imag + pre)
     A = rng.normal(size=(P, P))
@@ -111,14 +111,15 @@ def test_solve_too_large_is_an_error():
             ctx.solve(m["mO"], m["mE"], S, m["FR"], m["FI"], imag_time=1, eps=1e-3)
 
 
-def test_solve_nonpositive_pivot_in_later_panel():
-    """A non-positive pivot in the second 32-column panel (and in the
-    partial last one) is counted and handled as in the reference."""
-    P = 70
+@pytest.mark.parametrize("P,bad", [(70, (40, 66)), (390, (40, 388))])
+def test_solve_nonpositive_pivot_in_later_panel(P, bad):
+    """A non-positive pivot in a later panel (the third of 16 columns, the
+    second of 32) and in the partial last one is counted and handled as in
+    the reference."""
     rng = np.random.default_rng(7)
     A = rng.normal(size=(P, P))
     S = A @ A.T / P + 0.5 * np.eye(P)
-    for i in (40, 66):
+    for i in bad:
         S[i, :] = 0.0
         S[:, i] = 0.0
         S[i, i] = -0.5
