@@ -37,10 +37,15 @@ namespace {
 constexpr int kThreads = 256;
 // Histogram columns: 16 for four-wavefront walkers with real parameters (no
 // bank conflicts among the 16 lanes of one LDS pass; the imaginary gradient
-// sums would take that LDS otherwise), 8 otherwise (smaller walkers spend
-// more on clearing and reducing the histogram than on its conflicts).
+// sums would take that LDS otherwise), 4 for one-wavefront walkers (four of
+// them share a workgroup, and the smaller slots let more workgroups share a
+// CU), 8 otherwise (smaller walkers spend more on clearing and reducing the
+// histogram than on its conflicts).
 constexpr int kHistColsC = 8;
-__host__ __device__ constexpr int hist_cols(bool cplx, int nw) { return (cplx || nw < 4) ? kHistColsC : 2 * kHistColsC; }
+__host__ __device__ constexpr int hist_cols(bool cplx, int nw)
+{
+    return nw == 1 ? kHistColsC / 2 : ((cplx || nw < 4) ? kHistColsC : 2 * kHistColsC);
+}
 
 template <typename TO>
 struct EvalOut {
@@ -851,9 +856,11 @@ __global__ __launch_bounds__(kThreads, 4) void eval_kernel(KSys sy, KTables<T> t
 
 // Batch of independent systems in one launch.  Each system has its own
 // walker shape: 1, 2 or 4 wavefronts per walker (4, 2 or 1 walkers per
-// workgroup); a workgroup belongs to one system.
+// workgroup); a workgroup belongs to one system.  Small walkers are bound by
+// latency more than by issue, so the FP32 form is compiled for 6 waves per
+// SIMD (80 registers) instead of 4; the FP64 form keeps 4 (it would spill).
 template <typename T, int D, bool PER>
-__global__ __launch_bounds__(kThreads, 4) void eval_batch_kernel(EvalBatch<T> bt)
+__global__ __launch_bounds__(kThreads, sizeof(T) == 4 ? 6 : 4) void eval_batch_kernel(EvalBatch<T> bt)
 {
     int lo = 0, hi = bt.n;
     while (hi - lo > 1) {
