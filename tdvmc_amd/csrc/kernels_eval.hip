@@ -98,7 +98,7 @@ __device__ __forceinline__ void walker_sum(double (&v)[NV], double* red, int wav
 #pragma unroll
     for (int i = 0; i < NV; i++) v[i] = group_sum<64>(v[i]);
     if constexpr (WPW > 1) {
-        __syncthreads();
+        // (red is written only here, once per walker: no barrier before the writes)
         if (lane == 0) {
 #pragma unroll
             for (int i = 0; i < NV; i++) red[wave * NV + i] = v[i];
@@ -113,7 +113,7 @@ __device__ __forceinline__ void walker_sum(double (&v)[NV], double* red, int wav
         }
     }
 }
-constexpr int kRedDoubles = 40;   // >= 4 waves x 9 values
+constexpr int kRedDoubles = 40;   // >= 4 waves x 10 values
 
 // Pair potential of one pair in coordinate-difference units (constants from
 // PairConsts); POT: 1 square well, 2 Gaussian, 3 Lennard-Jones.
@@ -701,13 +701,10 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
         hist[e * hc] = m;
     }
     __syncthreads();
-    // pairs beyond the cutoff: all pairs minus those in the histogram (exact counts)
-    double outer = 0.0;
-    if (tid == 0) {
-        double inner = 0.0;
-        for (int j = 0; j < Ps; j++) inner += hist[j * 4 * hc];
-        outer = 0.5 * (double)N * (double)(N - 1) - inner;
-    }
+    // pairs in the histogram (exact integer counts: any order of the sum gives
+    // the same bits); the pairs beyond the cutoff follow after the walker sum
+    double inner = 0.0;
+    for (int j = tid; j < Ps; j += NT) inner += hist[j * 4 * hc];
 
     // ------------------------------------------------------------ assembly
     double sR1 = 0, sI1 = 0, sR1I1 = 0, sR2 = lap_tot_R, sI2 = lap_tot_I, sr2 = 0, vext = 0;
@@ -809,7 +806,7 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     if (out.bsum && active)
         for (int s = tid; s < Ps + 2; s += NT) out.bsum[w * (Ps + 2) + s] = B(s);
 
-    double sums[9] = {sR1, sI1, sR1I1, sR2, sI2, sr2, vext, vpot, ex_k + ex_p};
+    double sums[10] = {sR1, sI1, sR1I1, sR2, sI2, sr2, vext, vpot, ex_k + ex_p, inner};
     walker_sum<WPW>(sums, red, wave, lane);
     sR1 = sums[0];
     sI1 = sums[1];
@@ -820,6 +817,8 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
     vext = sums[6];
     vpot = sums[7];
     double ex = sums[8];
+    // pairs beyond the cutoff: all pairs minus those in the histogram
+    const double outer = 0.5 * (double)N * (double)(N - 1) - sums[9];
     if (tid == 0 && active) {
         if (sy.one_body) {
             out.O_store(w, sy.P, 0, -sr2);
