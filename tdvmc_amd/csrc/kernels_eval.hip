@@ -461,7 +461,9 @@ __device__ __forceinline__ void eval_kgrad(const KSys& sy, const KTables<T>& tb,
         cR += kag[8 * k + 3];
         if (CPLX) cI += kag[8 * k + 7];
     }
-    for (int rb = 0; rb < N; rb += 64) {
+    // a last round of at most 32 particles runs below, in half the k steps
+    const int nround = (N % 64 != 0 && N % 64 <= 32) ? N - N % 64 : N;
+    for (int rb = 0; rb < nround; rb += 64) {
         const int n = rb + lane;
         if (n >= N) continue;
         T rn[D];
@@ -506,6 +508,68 @@ __device__ __forceinline__ void eval_kgrad(const KSys& sy, const KTables<T>& tb,
         } else {
             lap_tot_R += lapR;
             if (CPLX) lap_tot_I += lapI;
+        }
+    }
+    if (nround < N) {
+        // last round of rem <= 32 particles: lanes l and l + 32 hold particle
+        // nround + l, the lower half-wave takes the first h of this wave's k,
+        // the upper half the rest.  Both amplitude rows are scalar loads (k
+        // stays wave-uniform per half); each lane selects its own.
+        const int rem = N - nround, hl = lane & 31;
+        const bool upper = lane >= 32;
+        const bool act = hl < rem;
+        const int n = nround + (act ? hl : 0);
+        const int h = (k_hi - k_lo + 1) / 2;
+        T rn[D];
+#pragma unroll
+        for (int a = 0; a < D; a++) rn[a] = C::raw(pos[a * N + n]);
+        T gR[D], gI[D];
+#pragma unroll
+        for (int a = 0; a < D; a++) gR[a] = gI[a] = (T)0;
+        T lR = (T)0, lI = (T)0;
+#pragma unroll 2
+        for (int j = 0; j < h; j++) {
+            const int kA = k_lo + j, kB = k_lo + h + j;   // wave-uniform
+            const bool hasB = kB < k_hi;
+            const __attribute__((address_space(4))) T* kaA = kag + 8 * kA;
+            const __attribute__((address_space(4))) T* kaB = kag + 8 * (hasB ? kB : kA);
+            const int k = upper ? kB : kA;
+            if (act && (!upper || hasB)) {
+                const T* qs = kq + k * 4;
+                T ph = (T)0;
+#pragma unroll
+                for (int a = 0; a < D; a++) ph += qs[a] * rn[a];
+                T sn, cs;
+                Prec<T>::sincos_small(ph, &sn, &cs);
+                const T rr = rho[2 * k], ri = rho[2 * k + 1];
+                const T pre = cs * rr + sn * ri;
+                const T pim = sn * rr - cs * ri;
+#pragma unroll
+                for (int a = 0; a < D; a++) gR[a] += pim * (upper ? kaB[a] : kaA[a]);
+                lR -= pre * (upper ? kaB[3] : kaA[3]);
+                if constexpr (CPLX) {
+#pragma unroll
+                    for (int a = 0; a < D; a++) gI[a] += pim * (upper ? kaB[4 + a] : kaA[4 + a]);
+                    lI -= pre * (upper ? kaB[7] : kaA[7]);
+                }
+            }
+        }
+        if (act) {
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                lds_add(gsum + a * N + n, (double)(gR[a] * (T)(-2.0 * 6.283185307179586)));
+                if (CPLX) lds_add(gsum + (D + a) * N + n, (double)(gI[a] * (T)(-2.0 * 6.283185307179586)));
+            }
+            // the constant part once per particle (lower half)
+            const double lapR = 2.0 * ((upper ? 0.0 : (double)cR) + (double)lR);
+            const double lapI = 2.0 * ((upper ? 0.0 : (double)cI) + (double)lI);
+            if (PP) {
+                lds_add(lsum + n, lapR);
+                if (CPLX) lds_add(lsum + N + n, lapI);
+            } else {
+                lap_tot_R += lapR;
+                if (CPLX) lap_tot_I += lapI;
+            }
         }
     }
 }
