@@ -852,23 +852,28 @@ __device__ __forceinline__ void eval_body(const KSys& sy, const KTables<T>& tb, 
         sR2 += 2.0 * hl * ((double)c[1] * m0 + (double)c[3] * m1);
         if (cplx) sI2 += 2.0 * hl * ((double)c[5] * m0 + (double)c[7] * m1);
     }
+    // each B_s once, into column 1 of histogram row s (the columns past 0
+    // are free after the reduction above; B reads column 0 only)
+    for (int s = tid; s < Ps + 2; s += NT) hist[s * hc + 1] = B(s);
+    __syncthreads();
+    auto Bs = [&](int s) { return hist[s * hc + 1]; };
     // boundary-condition map O = T B: origin rows over B_0..B_2, then
     // B_3, B_4, ..., then cutoff rows over B_{P_s-1}..B_{P_s+1}
     for (int k = tid; k < Ps; k += NT) {
         double o;
         if (k < sy.bc_n0) {
-            o = sy.bc_f0[k][0] * B(0) + sy.bc_f0[k][1] * B(1) + sy.bc_f0[k][2] * B(2);
+            o = sy.bc_f0[k][0] * Bs(0) + sy.bc_f0[k][1] * Bs(1) + sy.bc_f0[k][2] * Bs(2);
         } else if (k < Ps - sy.bc_n1) {
-            o = B(3 + k - sy.bc_n0);
+            o = Bs(3 + k - sy.bc_n0);
         } else {
             const double* f = sy.bc_f1[k - (Ps - sy.bc_n1)];
-            o = f[0] * B(Ps - 1) + f[1] * B(Ps) + f[2] * B(Ps + 1);
+            o = f[0] * Bs(Ps - 1) + f[1] * Bs(Ps) + f[2] * Bs(Ps + 1);
         }
         ex_p += (k == tid ? ur0 : tb.uR[sy.off_pair + k]) * o;
         if (active) out.O_store(w, sy.P, sy.off_pair + k, o);
     }
     if (out.bsum && active)
-        for (int s = tid; s < Ps + 2; s += NT) out.bsum[w * (Ps + 2) + s] = B(s);
+        for (int s = tid; s < Ps + 2; s += NT) out.bsum[w * (Ps + 2) + s] = Bs(s);
 
     double sums[10] = {sR1, sI1, sR1I1, sR2, sI2, sr2, vext, vpot, ex_k + ex_p, inner};
     walker_sum<WPW>(sums, red, wave, lane);
